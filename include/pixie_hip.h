@@ -1,0 +1,129 @@
+/*
+ * pixie_hip.h -- C-ABI of libpixie_hip.so, the MI355X (gfx950) raster hot path
+ * of pixie v6.0.0.
+ *
+ * Plain C types only (pointers, sizes, ints, floats).  Images are premultiplied
+ * RGBA8 (r,g,b,a bytes per pixel, row-major, stride = width) kept resident in
+ * HBM; 8-bit alpha images hold one byte per pixel and read as rgbx(v,v,v,v).
+ * Colours cross the boundary already premultiplied (pixie converts its Color
+ * with chroma's asRgbx before fillShapes, paths.nim:1603).
+ *
+ * Every int-returning function returns 0 on success or a PXH_ERR_* code; the
+ * message is available from pxh_last_error() (thread-local), mirroring the
+ * PixieError text the reference raises (genny's checkError/takeError model,
+ * bindings/bindings.nim:3-10).  Calls are thread-compatible: one host thread
+ * drives the library at a time.
+ *
+ * Enum ordinals are pixie's: BlendMode (common.nim:6-29), WindingRule
+ * (paths.nim:5-8), LineCap / LineJoin (paths.nim:10-16), path command kinds
+ * (paths.nim:18-22).
+ */
+#ifndef PIXIE_HIP_H
+#define PIXIE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PXH_OK = 0,
+  PXH_ERR_DISCRETIZE = 1,   /* "Unable to discretize cubic/quadratic/arc" */
+  PXH_ERR_INT_OVERFLOW = 2, /* "Path int overflow detected" */
+  PXH_ERR_DASH = 3,         /* "Invalid line dash value" */
+  PXH_ERR_NEG_BLUR = 4,     /* "Cannot apply negative blur" */
+  PXH_ERR_PARSE = 5,        /* "Invalid path ..." */
+  PXH_ERR_ARG = 6,
+  PXH_ERR_NOMEM = 7,
+  PXH_ERR_DEVICE = 8,
+  PXH_ERR_CAPACITY = 9
+};
+
+enum { PXH_FORMAT_RGBA8 = 0, PXH_FORMAT_A8 = 1 };
+
+typedef struct pxh_image pxh_image;
+
+/* Library ---------------------------------------------------------------- */
+int pxh_init(void);                 /* selects HIP device 0; idempotent */
+const char* pxh_last_error(void);
+const char* pxh_version(void);
+int pxh_synchronize(void);          /* waits for all queued device work */
+
+/* Images ---------------------------------------------------------------- */
+/* newImage (common.nim:39-47): zero-filled, width and height > 0. */
+pxh_image* pxh_image_new(int32_t width, int32_t height);
+pxh_image* pxh_image_new_a8(int32_t width, int32_t height);
+void pxh_image_free(pxh_image* img);
+int32_t pxh_image_width(const pxh_image* img);
+int32_t pxh_image_height(const pxh_image* img);
+int32_t pxh_image_format(const pxh_image* img);
+void* pxh_image_device_ptr(pxh_image* img);
+/* Host <-> device copies of the whole image (width*height*bpp bytes). */
+int pxh_image_upload(pxh_image* img, const void* data);
+int pxh_image_download(const pxh_image* img, void* data);
+/* copy (common.nim:49-54) into an existing image of the same size/format. */
+int pxh_image_copy(pxh_image* dst, const pxh_image* src);
+/* fill (images.nim:55-57 / internal.nim:55-66). */
+int pxh_image_fill(pxh_image* img, uint32_t rgbx);
+
+/* Paths ----------------------------------------------------------------- */
+/* parsePath (paths.nim:119-262): SVG path string -> pixie command array
+ * (kind, params...).  *cmds is malloc'd; release with pxh_free. */
+int pxh_parse_path(const char* path, float** cmds, int32_t* ncmds);
+void pxh_free(void* p);
+
+/* fillPath (paths.nim:2093-2113) with a solid paint: cmds is a pixie command
+ * array, mat9 a vmath Mat3 (column-major, NULL = identity), rgbx the paint
+ * colour with opacity applied, premultiplied.  Solid paints with alpha 0 are
+ * skipped unless blend is OverwriteBlend (paths.nim:2107). */
+int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                  uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
+
+/* strokePath (paths.nim:2144-2176) with a solid paint. */
+int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                    uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
+                    float miter_limit, const float* dashes, int32_t ndashes,
+                    int32_t blend_mode);
+
+/* fillShapes (paths.nim:1593-1912): flattened, transformed polygons given as
+ * xy pairs and polygon start offsets (npolys + 1 entries, in points). */
+int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, int32_t npolys,
+                    uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
+
+/* Image operations ------------------------------------------------------ */
+/* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.
+ * blendRect (images.nim:468-529, reached from draw images.nim:636-678). */
+int pxh_draw(pxh_image* a, const pxh_image* b, int32_t dx, int32_t dy, int32_t blend_mode);
+/* blur (images.nim:304-365); oob is the premultiplied out-of-bounds colour. */
+int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx);
+/* spread (images.nim:700-758). */
+int pxh_spread(pxh_image* img, float spread);
+/* shadow (images.nim:760-776) into `out` (same size as src); integer offsets. */
+int pxh_shadow(pxh_image* out, const pxh_image* src, float offset_x, float offset_y,
+               float spread, float blur, uint32_t rgbx);
+/* applyOpacity (images.nim:261-277). */
+int pxh_apply_opacity(pxh_image* img, float opacity);
+
+/* Pixel ops on device buffers (test and benchmark access to blender(),
+ * blends.nim:275-299): out[i] = blend(mode, backdrop[i], source[i]). */
+int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_t* source,
+                     uint32_t* out, int64_t n);
+
+/* Host geometry of one fillPath (stroke = 0) or strokePath (stroke = 1) call
+ * without device work: info16 receives the 16-int fill header (startX,
+ * startY, pathWidth, pathHeight, nParts, bandH, ...), parts4 4 ints per band
+ * (entryBase, count, flags, 0), entries4 4 floats per entry (at.y, to.y, m,
+ * b) and windings one int8 per entry.  Arrays are released with pxh_free.
+ * Mirrors the locals of fillShapes (paths.nim:1602-1629) and the output of
+ * partitionSegments (paths.nim:1168-1262); used by the test-suite. */
+int pxh_debug_fill_geometry(const float* cmds, int32_t ncmds, const float* mat9, int32_t w,
+                            int32_t h, int32_t stroke, float stroke_width, int32_t line_cap,
+                            int32_t line_join, float miter_limit, int32_t* info16,
+                            int32_t** parts4, int32_t* nparts, float** entries4,
+                            int8_t** windings, int32_t* nentries);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -24,7 +24,7 @@ class OracleError(RuntimeError):
 class Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in (
         "touched", "rows_trap", "rows_aa", "rows_nonaa", "rows_span2",
-        "dropped", "segments", "partitions")]
+        "dropped", "segments", "partitions", "max_part", "max_sel")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -889,9 +889,24 @@ typedef struct {
   orc_rgbx rgbx;
   int mode;
   orc_stats* st;
+  int64_t cx0, cx1; /* column clip of non-mask writes */
 } canvas_t;
 
+/* Writes of a non-mask fill are confined to the path's pixel span widened by
+ * ORC_WRITE_MARGIN columns on each side.  Only trapezoid slivers computed
+ * from slope-intercept x can reach outside the span, and only by float
+ * error; the clip gives such pathological writes a defined result (pixie
+ * would write them) so that fills can be binned by their bounds. */
+#define ORC_WRITE_MARGIN 16
+
+static inline int px_clipped(canvas_t* c, int64_t x) {
+  if (c->mode == ORC_MASK || (x >= c->cx0 && x < c->cx1)) return 0;
+  if (c->st) c->st->dropped++;
+  return 1;
+}
+
 static inline void px_set(canvas_t* c, int64_t x, int64_t y, orc_rgbx v) {
+  if (px_clipped(c, x)) return;
   uint8_t* p = c->img + (y * c->W + x) * 4;
   p[0] = v.r;
   p[1] = v.g;
@@ -904,6 +919,7 @@ static inline orc_rgbx px_get(canvas_t* c, int64_t x, int64_t y) {
   return v;
 }
 static inline void px_blend(canvas_t* c, int64_t x, int64_t y, orc_rgbx src) {
+  if (px_clipped(c, x)) return;
   px_set(c, x, y, orc_blend(c->mode, px_get(c, x, y), src));
   if (c->st) c->st->touched++;
 }
@@ -958,7 +974,7 @@ static void fill_hits(canvas_t* c, int64_t startX, int64_t y, const hit_t* hits,
 
 static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_rgbx rgbx,
                             int rule, int mode, orc_stats* st) {
-  canvas_t cv = {img, W, H, rgbx, mode, st};
+  canvas_t cv = {img, W, H, rgbx, mode, st, 0, 0};
   segbuf sg = {0};
   shapes_to_segments(sh, 1, &sg);
   if (st) st->segments += sg.n;
@@ -984,11 +1000,14 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
     return ORC_ERR_INT_OVERFLOW;
   }
 
+  cv.cx0 = startX - ORC_WRITE_MARGIN < 0 ? 0 : startX - ORC_WRITE_MARGIN;
+  cv.cx1 = startX + pathWidth + ORC_WRITE_MARGIN > W ? W : startX + pathWidth + ORC_WRITE_MARGIN;
   int nP = 0;
   part_t* P = partition_segments(&sg, startY, pathHeight - startY, &nP);
   if (st) st->partitions += nP;
   int maxE = 0;
   for (int p = 0; p < nP; p++) maxE = P[p].n > maxE ? P[p].n : maxE;
+  if (st && maxE > st->max_part) st->max_part = maxE;
   int cap = maxE > 2 ? maxE : 2;
   int* idx = (int*)malloc(sizeof(int) * (size_t)cap);
   v2* trapAt = (v2*)malloc(sizeof(v2) * (size_t)cap);
@@ -1035,6 +1054,7 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
       }
     }
 
+    if (st && ne > st->max_sel) st->max_sel = ne;
     if (all && ne % 2 == 0) { /* trapezoid fast path, 1691-1872 */
       for (int i = 0; i < ne; i++) {
         int k = idx[i];

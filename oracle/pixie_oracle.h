@@ -63,6 +63,8 @@ typedef struct {
   int64_t dropped;      /* coverage writes outside the path span (UB in pixie) */
   int64_t segments;     /* segments after shapesToSegments */
   int64_t partitions;   /* partitions built */
+  int64_t max_part;     /* largest partition (entries) */
+  int64_t max_sel;      /* most entries selected for one row */
 } orc_stats;
 
 const char* orc_last_error(void);

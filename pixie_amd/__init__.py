@@ -1,0 +1,263 @@
+"""pixie_amd -- pixie's raster hot path on MI355X (gfx950) through libpixie_hip.so.
+
+Python mirror of pixie's Image / Path / Paint API for the hot path
+(fillPath, strokePath, draw, blur, spread, shadow, applyOpacity, fill),
+calling the C-ABI declared in include/pixie_hip.h.  Images live in HBM; pixels
+are premultiplied RGBA8 exactly as pixie's Image.data (common.nim:34-37).
+
+There is no CPU fallback: importing this package fails if the native
+library has not been built, and every call reports device errors.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .paths import Path, mat3, translate, scale, f32  # noqa: F401
+
+__all__ = ["Image", "Path", "PixieError", "parse_path", "lib", "blend_pixels",
+           "BlendMode", "NonZero", "EvenOdd", "mat3", "translate", "scale"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpixie_hip.so")
+
+NonZero, EvenOdd = 0, 1
+ButtCap, RoundCap, SquareCap = 0, 1, 2
+MiterJoin, RoundJoin, BevelJoin = 0, 1, 2
+
+
+class BlendMode:
+    """pixie's BlendMode ordinals (common.nim:6-29)."""
+    Normal, Darken, Multiply, ColorBurn, Lighten, Screen, ColorDodge, Overlay, SoftLight, \
+        HardLight, Difference, Exclusion, Hue, Saturation, Color, Luminosity, Mask, Overwrite, \
+        SubtractMask, ExcludeMask = range(20)
+
+
+class PixieError(RuntimeError):
+    """Raised when a call fails (pixie raises PixieError, common.nim:4)."""
+
+    def __init__(self, code, msg):
+        super().__init__("%s (code %d)" % (msg, code))
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """The loaded native library.  Raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libpixie_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.POINTER
+        vp, f32p, i32p = ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_int32)
+        u32, i32, f = ctypes.c_uint32, ctypes.c_int32, ctypes.c_float
+        L.pxh_init.restype = ctypes.c_int
+        L.pxh_last_error.restype = ctypes.c_char_p
+        L.pxh_version.restype = ctypes.c_char_p
+        L.pxh_image_new.restype = vp
+        L.pxh_image_new.argtypes = [i32, i32]
+        L.pxh_image_new_a8.restype = vp
+        L.pxh_image_new_a8.argtypes = [i32, i32]
+        L.pxh_image_free.argtypes = [vp]
+        L.pxh_image_width.argtypes = [vp]
+        L.pxh_image_height.argtypes = [vp]
+        L.pxh_image_format.argtypes = [vp]
+        L.pxh_image_device_ptr.restype = vp
+        L.pxh_image_device_ptr.argtypes = [vp]
+        L.pxh_image_upload.argtypes = [vp, vp]
+        L.pxh_image_download.argtypes = [vp, vp]
+        L.pxh_image_copy.argtypes = [vp, vp]
+        L.pxh_image_fill.argtypes = [vp, u32]
+        L.pxh_parse_path.argtypes = [ctypes.c_char_p, P(f32p), i32p]
+        L.pxh_free.argtypes = [vp]
+        L.pxh_fill_path.argtypes = [vp, f32p, i32, f32p, u32, i32, i32]
+        L.pxh_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, f32p, i32, i32]
+        L.pxh_fill_shapes.argtypes = [vp, f32p, i32p, i32, u32, i32, i32]
+        L.pxh_draw.argtypes = [vp, vp, i32, i32, i32]
+        L.pxh_blur.argtypes = [vp, f, u32]
+        L.pxh_spread.argtypes = [vp, f]
+        L.pxh_shadow.argtypes = [vp, vp, f, f, f, f, u32]
+        L.pxh_apply_opacity.argtypes = [vp, f]
+        L.pxh_blend_pixels.argtypes = [i32, vp, vp, vp, ctypes.c_int64]
+        L.pxh_debug_fill_geometry.argtypes = [f32p, i32, f32p, i32, i32, i32, f, i32, i32, f,
+                                              i32p, P(i32p), i32p, P(f32p),
+                                              P(P(ctypes.c_int8)), i32p]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise PixieError(rc, lib().pxh_last_error().decode())
+
+
+def init():
+    _check(lib().pxh_init())
+
+
+def synchronize():
+    _check(lib().pxh_synchronize())
+
+
+def pack(c):
+    """(r, g, b, a) premultiplied bytes -> packed uint32 (r in the low byte)."""
+    r, g, b, a = [int(v) & 255 for v in c]
+    return r | (g << 8) | (b << 16) | (a << 24)
+
+
+def _cmds(path):
+    if isinstance(path, str):
+        path = parse_path(path)
+    elif isinstance(path, Path):
+        path = path.array()
+    return np.ascontiguousarray(path, dtype=np.float32)
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if a is not None else None
+
+
+def _mat(m):
+    return None if m is None else np.ascontiguousarray(m, dtype=np.float32)
+
+
+def parse_path(s):
+    """parsePath (paths.nim:119-262) by the native parser."""
+    out = ctypes.POINTER(ctypes.c_float)()
+    n = ctypes.c_int32()
+    _check(lib().pxh_parse_path(s.encode(), ctypes.byref(out), ctypes.byref(n)))
+    if n.value == 0:
+        return np.zeros(0, np.float32)
+    arr = np.ctypeslib.as_array(out, shape=(n.value,)).copy()
+    lib().pxh_free(out)
+    return arr
+
+
+class Image:
+    """A device-resident image (pixie's Image, common.nim:34-37)."""
+
+    def __init__(self, width, height, a8=False):
+        L = lib()
+        h = L.pxh_image_new_a8(width, height) if a8 else L.pxh_image_new(width, height)
+        if not h:
+            raise PixieError(6, L.pxh_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+        self.width, self.height, self.a8 = width, height, a8
+
+    @classmethod
+    def from_array(cls, arr):
+        arr = np.ascontiguousarray(arr, dtype=np.uint8)
+        a8 = arr.ndim == 2
+        img = cls(arr.shape[1], arr.shape[0], a8=a8)
+        img.upload(arr)
+        return img
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.pxh_image_free(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_ptr(self):
+        return lib().pxh_image_device_ptr(self._h)
+
+    def upload(self, arr):
+        shape = (self.height, self.width) if self.a8 else (self.height, self.width, 4)
+        arr = np.ascontiguousarray(arr, dtype=np.uint8)
+        assert arr.shape == shape, (arr.shape, shape)
+        _check(lib().pxh_image_upload(self._h, arr.ctypes.data))
+
+    def download(self):
+        shape = (self.height, self.width) if self.a8 else (self.height, self.width, 4)
+        out = np.empty(shape, np.uint8)
+        _check(lib().pxh_image_download(self._h, out.ctypes.data))
+        return out
+
+    def copy(self):
+        o = Image(self.width, self.height, self.a8)
+        _check(lib().pxh_image_copy(o._h, self._h))
+        return o
+
+    def fill(self, color):
+        _check(lib().pxh_image_fill(self._h, pack(color)))
+
+    def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
+        c = _cmds(path)
+        m = _mat(transform)
+        _check(lib().pxh_fill_path(self._h, _fp(c), len(c), _fp(m), pack(color), windingRule,
+                                   blendMode))
+
+    def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
+                   lineJoin=MiterJoin, miterLimit=4.0, dashes=(), blendMode=0):
+        c = _cmds(path)
+        m = _mat(transform)
+        d = np.ascontiguousarray(dashes, dtype=np.float32) if len(dashes) else None
+        _check(lib().pxh_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color), strokeWidth,
+                                     lineCap, lineJoin, miterLimit, _fp(d),
+                                     0 if d is None else len(d), blendMode))
+
+    def fillShapes(self, shapes, color, windingRule=NonZero, blendMode=0):
+        pts = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float32).reshape(-1, 2)
+                                                   for s in shapes]) if shapes else
+                                   np.zeros((0, 2), np.float32), dtype=np.float32)
+        ofs = np.zeros(len(shapes) + 1, np.int32)
+        for i, s in enumerate(shapes):
+            ofs[i + 1] = ofs[i] + len(s)
+        _check(lib().pxh_fill_shapes(self._h, _fp(pts), ofs.ctypes.data_as(
+            ctypes.POINTER(ctypes.c_int32)), len(shapes), pack(color), windingRule, blendMode))
+
+    def draw(self, other, dx=0, dy=0, blendMode=0):
+        _check(lib().pxh_draw(self._h, other._h, int(dx), int(dy), blendMode))
+
+    def blur(self, radius, outOfBounds=(0, 0, 0, 0)):
+        _check(lib().pxh_blur(self._h, float(radius), pack(outOfBounds)))
+
+    def spread(self, s):
+        _check(lib().pxh_spread(self._h, float(s)))
+
+    def shadow(self, offset, spread, blur, color):
+        out = Image(self.width, self.height)
+        _check(lib().pxh_shadow(out._h, self._h, float(offset[0]), float(offset[1]),
+                                float(spread), float(blur), pack(color)))
+        return out
+
+    def applyOpacity(self, opacity):
+        _check(lib().pxh_apply_opacity(self._h, float(opacity)))
+
+
+def blend_pixels(mode, backdrop, source):
+    """blender(mode) over pixel arrays on the device (uint8 [..., 4])."""
+    b = np.ascontiguousarray(backdrop, dtype=np.uint8)
+    s = np.ascontiguousarray(source, dtype=np.uint8)
+    out = np.empty_like(b)
+    _check(lib().pxh_blend_pixels(mode, b.ctypes.data, s.ctypes.data, out.ctypes.data,
+                                  b.size // 4))
+    return out
+
+
+def debug_fill_geometry(path, w, h, transform=None, stroke=False, width=1.0, cap=0, join=0,
+                        miter=4.0):
+    """Host geometry of one fill: (info[16], parts[n,4], entries[m,4], windings[m])."""
+    c = _cmds(path)
+    m = _mat(transform)
+    info = (ctypes.c_int32 * 16)()
+    P = ctypes.POINTER
+    parts, ents, wind = P(ctypes.c_int32)(), P(ctypes.c_float)(), P(ctypes.c_int8)()
+    np_, ne = ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().pxh_debug_fill_geometry(_fp(c), len(c), _fp(m), w, h, int(stroke), width, cap,
+                                       join, miter, info, ctypes.byref(parts), ctypes.byref(np_),
+                                       ctypes.byref(ents), ctypes.byref(wind), ctypes.byref(ne))
+    pa = np.ctypeslib.as_array(parts, shape=(max(np_.value, 1) * 4,)).copy()[:np_.value * 4]
+    en = np.ctypeslib.as_array(ents, shape=(max(ne.value, 1) * 4,)).copy()[:ne.value * 4]
+    wd = np.ctypeslib.as_array(wind, shape=(max(ne.value, 1),)).copy()[:ne.value]
+    for p in (parts, ents, wind):
+        lib().pxh_free(p)
+    _check(rc)
+    return np.array(info[:], np.int32), pa.reshape(-1, 4), en.reshape(-1, 4), wd

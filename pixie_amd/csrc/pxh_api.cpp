@@ -1,0 +1,597 @@
+// pxh_api.cpp -- the C-ABI (include/pixie_hip.h): device context, images and
+// the host side of each entry point (geometry, uploads, kernel launches).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/pixie_hip.h"
+#include "pxh_common.h"
+#include "pxh_host_geom.h"
+#include "pxh_kernels.h"
+#include "pxh_pixel.h"
+
+struct pxh_image {
+  int32_t w, h, fmt;
+  void* dev;
+  size_t bytes;
+};
+
+namespace pxh {
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define PXH_HIP(call)                                                            \
+  do {                                                                           \
+    hipError_t e_ = (call);                                                      \
+    if (e_ != hipSuccess) return fail(kErrDevice, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Growable device buffer.
+struct DevBuf {
+  char* p = nullptr;
+  size_t cap = 0;
+  int ensure(size_t n) {
+    if (n <= cap) return 0;
+    if (p) {
+      if (hipFree(p) != hipSuccess) return kErrDevice;
+      p = nullptr;
+      cap = 0;
+    }
+    size_t c = n < (1u << 20) ? (1u << 20) : n + n / 2;
+    if (hipMalloc((void**)&p, c) != hipSuccess) return kErrNoMem;
+    cap = c;
+    return 0;
+  }
+};
+
+struct Ctx {
+  bool inited = false;
+  hipStream_t stream = nullptr;
+  size_t maxLds = 65536;
+  int numCUs = 256;
+  // geometry upload: pinned host staging + device copy, reused per call once
+  // the previous consumer kernel has finished (stageFree)
+  char* hostStage = nullptr;
+  size_t hostCap = 0;
+  DevBuf devStage;
+  hipEvent_t stageFree = nullptr;
+  DevBuf gscratch;   // per-workgroup row scratch when it does not fit in LDS
+  DevBuf tmp;        // blur / shadow temporaries
+  DevBuf tmp2;
+  DevBuf plane;      // alpha plane
+  DevBuf blendBuf;   // pxh_blend_pixels
+  int32_t* errFlag = nullptr;
+  std::map<int, uint32_t*> luts;  // gaussian LUT per radius (device, uint32 taps)
+};
+Ctx g;
+
+int init() {
+  if (g.inited) return 0;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(kErrDevice, "no HIP device");
+  PXH_HIP(hipSetDevice(0));
+  int lds = 0, cus = 0;
+  PXH_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
+  PXH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
+  g.maxLds = (size_t)lds;
+  g.numCUs = cus;
+  PXH_HIP(hipStreamCreate(&g.stream));
+  PXH_HIP(hipEventCreateWithFlags(&g.stageFree, hipEventDisableTiming));
+  PXH_HIP(hipEventRecord(g.stageFree, g.stream));
+  PXH_HIP(hipMalloc((void**)&g.errFlag, sizeof(int32_t)));
+  PXH_HIP(hipMemsetAsync(g.errFlag, 0, sizeof(int32_t), g.stream));
+  if (set_fill_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(fill)");
+  if (set_blur_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(blur)");
+  g.inited = true;
+  return 0;
+}
+
+#define PXH_INIT()          \
+  do {                      \
+    int r_ = init();        \
+    if (r_ != 0) return r_; \
+  } while (0)
+
+// Stages `parts` contiguous host blobs into device memory; returns device
+// pointers.  Waits for the previous consumer of the staging buffers.
+int stage(const std::vector<std::pair<const void*, size_t>>& parts, std::vector<char*>* out) {
+  size_t total = 0;
+  std::vector<size_t> offs;
+  for (auto& p : parts) {
+    offs.push_back(total);
+    total += (p.second + 255) & ~(size_t)255;
+  }
+  PXH_HIP(hipEventSynchronize(g.stageFree));
+  if (total > g.hostCap) {
+    if (g.hostStage) PXH_HIP(hipHostFree(g.hostStage));
+    g.hostStage = nullptr;
+    size_t c = total + total / 2 + (1 << 20);
+    PXH_HIP(hipHostMalloc((void**)&g.hostStage, c, hipHostMallocDefault));
+    g.hostCap = c;
+  }
+  if (int r = g.devStage.ensure(total)) return fail(r, "device staging allocation failed");
+  for (size_t i = 0; i < parts.size(); i++)
+    if (parts[i].second) memcpy(g.hostStage + offs[i], parts[i].first, parts[i].second);
+  PXH_HIP(hipMemcpyAsync(g.devStage.p, g.hostStage, total, hipMemcpyHostToDevice, g.stream));
+  out->clear();
+  for (size_t i = 0; i < parts.size(); i++) out->push_back(g.devStage.p + offs[i]);
+  return 0;
+}
+
+int next_pow2(int v) {
+  int p = 64;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+// Launches the raster of one built fill onto img.
+int raster_fill(pxh_image* img, FillGeom& geo) {
+  FillInfo& f = geo.info;
+  if (f.pathWidth == 0) return 0;
+  const bool mask = f.mode == kMask;
+  int32_t rowBase, nRows;
+  if (mask) {
+    rowBase = 0;
+    nRows = img->h;
+  } else {
+    rowBase = f.startY;
+    nRows = f.pathHeight - f.startY;
+  }
+  if (nRows <= 0) return 0;
+  std::vector<char*> dp;
+  int r = stage({{&f, sizeof(FillInfo)},
+                 {geo.parts.data(), geo.parts.size() * sizeof(PartInfo)},
+                 {geo.entries.data(), geo.entries.size() * sizeof(Entry)},
+                 {geo.windings.data(), geo.windings.size()}},
+                &dp);
+  if (r) return r;
+  const int ww = 1024;
+  const int cap = next_pow2(geo.maxCount > 2 ? geo.maxCount : 2);
+  const size_t fixedBytes = 8 * (size_t)ww + 16;
+  const size_t capBytes = (size_t)cap * 24;
+  FillArgs a;
+  a.F = (const FillInfo*)dp[0];
+  a.parts = (const PartInfo*)dp[1];
+  a.E = (const Entry*)dp[2];
+  a.EW = (const int8_t*)dp[3];
+  a.canvas = img->dev;
+  a.rowBase = rowBase;
+  a.nRows = nRows;
+  a.cap = cap;
+  a.ww = ww;
+  a.err = g.errFlag;
+  int grid = nRows < 8192 ? nRows : 8192;
+  size_t lds = fixedBytes;
+  if (fixedBytes + capBytes <= g.maxLds) {
+    a.capInLds = 1;
+    a.gscratch = nullptr;
+    lds += capBytes;
+  } else {
+    a.capInLds = 0;
+    size_t maxScratch = (size_t)1 << 30;
+    if ((size_t)grid * capBytes > maxScratch) grid = (int)(maxScratch / capBytes);
+    if (grid < 1) return fail(kErrCapacity, "a band holds too many segments");
+    if (int e = g.gscratch.ensure((size_t)grid * capBytes)) return fail(e, "scratch allocation failed");
+    a.gscratch = g.gscratch.p;
+  }
+  if (launch_fill_rows(a, grid, lds, img->fmt == PXH_FORMAT_A8, g.stream) != 0)
+    return fail(kErrDevice, std::string("fill launch: ") + hipGetErrorString(hipGetLastError()));
+  PXH_HIP(hipEventRecord(g.stageFree, g.stream));
+  return 0;
+}
+
+int check_image(const pxh_image* img) {
+  if (!img || !img->dev) return fail(kErrArg, "invalid image handle");
+  return 0;
+}
+
+int fill_shapes_impl(pxh_image* img, const Shapes& sh, uint32_t rgbx, int32_t rule, int32_t mode) {
+  if (mode < 0 || mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (rule != kNonZero && rule != kEvenOdd) return fail(kErrArg, "invalid winding rule");
+  FillGeom geo;
+  std::string err;
+  int32_t r = build_fill(sh, img->w, img->h, rgbx, mode, rule, &geo, &err);
+  if (r != kOk) return fail(r, err);
+  return raster_fill(img, geo);
+}
+
+const float kIdentity[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+
+// gaussianKernel (internal.nim:17-34): s and the exp() argument in float32,
+// the 1/sqrt(2*PI*s^2) factor in float64 (2*PI is a float64 constant), taps
+// summed in float64, stored to float32, normalised in float64.
+void gaussian_lut(int radius, std::vector<uint32_t>* out) {
+  int n = 2 * radius + 1;
+  std::vector<float> f(n);
+  double total = 0.0;
+  for (int step = -radius; step <= radius; step++) {
+    float s = (float)radius / 2.2f;
+    float s2 = s * s;
+    float st = (float)step;
+    double a = (1.0 / sqrt(6.283185307179586 * (double)s2)) *
+               (double)expf((-1.0f * (st * st)) / (2.0f * s2));
+    f[step + radius] = (float)a;
+    total += a;
+  }
+  out->resize(n);
+  for (int i = 0; i < n; i++) {
+    float v = (float)((double)f[i] / total);
+    (*out)[i] = (uint16_t)roundf(v * 255.0f * 256.0f);
+  }
+}
+
+int get_lut(int radius, uint32_t** dev) {
+  auto it = g.luts.find(radius);
+  if (it != g.luts.end()) {
+    *dev = it->second;
+    return 0;
+  }
+  std::vector<uint32_t> lut;
+  gaussian_lut(radius, &lut);
+  uint32_t* p = nullptr;
+  PXH_HIP(hipMalloc((void**)&p, lut.size() * 4));
+  PXH_HIP(hipMemcpy(p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice));
+  g.luts[radius] = p;
+  *dev = p;
+  return 0;
+}
+
+int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius, uint32_t oob,
+              bool shadowColor, uint32_t color) {
+  if (blur_x_lds(radius) > g.maxLds || blur_y_lds(radius) > g.maxLds)
+    return fail(kErrCapacity, "blur radius too large for the LDS tiles");
+  uint32_t* lut = nullptr;
+  if (int r = get_lut(radius, &lut)) return r;
+  if (int e = g.tmp.ensure((size_t)w * h * 4)) return fail(e, "blur temporary allocation failed");
+  BlurArgs ax{src_dst, (uint32_t*)g.tmp.p, lut, w, h, radius, oob, 0, 0};
+  BlurArgs ay{(uint32_t*)g.tmp.p, out, lut, w, h, radius, oob, shadowColor ? 1 : 0, color};
+  if (launch_blur(ax, ay, g.stream) != 0) return fail(kErrDevice, "blur launch failed");
+  return 0;
+}
+
+}  // namespace
+}  // namespace pxh
+
+using namespace pxh;
+
+extern "C" {
+
+int pxh_init(void) { return init(); }
+
+const char* pxh_last_error(void) { return g_err.c_str(); }
+
+const char* pxh_version(void) { return "pixie_hip 0.1 (gfx950)"; }
+
+int pxh_synchronize(void) {
+  PXH_INIT();
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  int32_t e = 0;
+  PXH_HIP(hipMemcpy(&e, g.errFlag, sizeof(e), hipMemcpyDeviceToHost));
+  if (e) {
+    PXH_HIP(hipMemset(g.errFlag, 0, sizeof(int32_t)));
+    return fail(kErrCapacity, "a band held more segments than the row scratch capacity");
+  }
+  return 0;
+}
+
+static pxh_image* image_new(int32_t w, int32_t h, int32_t fmt) {
+  if (init() != 0) return nullptr;
+  if (w <= 0 || h <= 0) {
+    fail(kErrArg, "Image width and height must be > 0");
+    return nullptr;
+  }
+  pxh_image* img = new pxh_image;
+  img->w = w;
+  img->h = h;
+  img->fmt = fmt;
+  img->bytes = (size_t)w * h * (fmt == PXH_FORMAT_A8 ? 1 : 4);
+  img->dev = nullptr;
+  if (hipMalloc(&img->dev, img->bytes) != hipSuccess) {
+    delete img;
+    fail(kErrNoMem, "device allocation failed");
+    return nullptr;
+  }
+  if (hipMemsetAsync(img->dev, 0, img->bytes, g.stream) != hipSuccess ||
+      hipStreamSynchronize(g.stream) != hipSuccess) {
+    (void)hipFree(img->dev);
+    delete img;
+    fail(kErrDevice, "device memset failed");
+    return nullptr;
+  }
+  return img;
+}
+
+pxh_image* pxh_image_new(int32_t w, int32_t h) { return image_new(w, h, PXH_FORMAT_RGBA8); }
+pxh_image* pxh_image_new_a8(int32_t w, int32_t h) { return image_new(w, h, PXH_FORMAT_A8); }
+
+void pxh_image_free(pxh_image* img) {
+  if (!img) return;
+  (void)hipStreamSynchronize(g.stream);
+  if (img->dev) (void)hipFree(img->dev);
+  delete img;
+}
+
+int32_t pxh_image_width(const pxh_image* img) { return img ? img->w : 0; }
+int32_t pxh_image_height(const pxh_image* img) { return img ? img->h : 0; }
+int32_t pxh_image_format(const pxh_image* img) { return img ? img->fmt : -1; }
+void* pxh_image_device_ptr(pxh_image* img) { return img ? img->dev : nullptr; }
+
+int pxh_image_upload(pxh_image* img, const void* data) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  PXH_HIP(hipMemcpyAsync(img->dev, data, img->bytes, hipMemcpyHostToDevice, g.stream));
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+int pxh_image_download(const pxh_image* img, void* data) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  PXH_HIP(hipMemcpyAsync(data, img->dev, img->bytes, hipMemcpyDeviceToHost, g.stream));
+  return pxh_synchronize();
+}
+
+int pxh_image_copy(pxh_image* dst, const pxh_image* src) {
+  PXH_INIT();
+  if (int r = check_image(dst)) return r;
+  if (int r = check_image(src)) return r;
+  if (dst->bytes != src->bytes || dst->fmt != src->fmt) return fail(kErrArg, "image size mismatch");
+  PXH_HIP(hipMemcpyAsync(dst->dev, src->dev, src->bytes, hipMemcpyDeviceToDevice, g.stream));
+  return 0;
+}
+
+int pxh_image_fill(pxh_image* img, uint32_t rgbx) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt == PXH_FORMAT_A8)
+    PXH_HIP(hipMemsetAsync(img->dev, (int)(rgbx >> 24), img->bytes, g.stream));
+  else
+    PXH_HIP(hipMemsetD32Async((hipDeviceptr_t)img->dev, (int)rgbx, img->bytes / 4, g.stream));
+  return 0;
+}
+
+int pxh_parse_path(const char* path, float** cmds, int32_t* ncmds) {
+  std::vector<float> out;
+  std::string err;
+  *cmds = nullptr;
+  *ncmds = 0;
+  int32_t r = parse_path(path, &out, &err);
+  if (r != kOk) return fail(r, err);
+  if (!out.empty()) {
+    *cmds = (float*)malloc(out.size() * sizeof(float));
+    if (!*cmds) return fail(kErrNoMem, "allocation failed");
+    memcpy(*cmds, out.data(), out.size() * sizeof(float));
+  }
+  *ncmds = (int32_t)out.size();
+  return 0;
+}
+
+void pxh_free(void* p) { free(p); }
+
+int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9, uint32_t rgbx,
+                  int32_t winding_rule, int32_t blend_mode) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (!mat9) mat9 = kIdentity;
+  if (!(ch_a(rgbx) > 0 || blend_mode == kOverwrite)) return 0;  // paths.nim:2107
+  Shapes sh;
+  std::string err;
+  int32_t r = commands_to_shapes(cmds, ncmds, true, pixel_scale(mat9), &sh, &err);
+  if (r != kOk) return fail(r, err);
+  transform_shapes(&sh, mat9);
+  return fill_shapes_impl(img, sh, rgbx, winding_rule, blend_mode);
+}
+
+int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                    uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
+                    float miter_limit, const float* dashes, int32_t ndashes, int32_t blend_mode) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (!mat9) mat9 = kIdentity;
+  if (!(ch_a(rgbx) > 0 || blend_mode == kOverwrite)) return 0;  // paths.nim:2162
+  float ps = pixel_scale(mat9);
+  Shapes sh, st;
+  std::string err;
+  int32_t r = commands_to_shapes(cmds, ncmds, false, ps, &sh, &err);
+  if (r != kOk) return fail(r, err);
+  StrokeParams sp;
+  sp.width = stroke_width;
+  sp.cap = line_cap;
+  sp.join = line_join;
+  sp.miterLimit = miter_limit;
+  sp.dashes = dashes;
+  sp.ndashes = ndashes;
+  r = stroke_shapes(sh, sp, ps, &st, &err);
+  if (r != kOk) return fail(r, err);
+  transform_shapes(&st, mat9);
+  return fill_shapes_impl(img, st, rgbx, kNonZero, blend_mode);
+}
+
+int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, int32_t npolys,
+                    uint32_t rgbx, int32_t winding_rule, int32_t blend_mode) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  Shapes sh;
+  for (int32_t k = 0; k < npolys; k++) {
+    int32_t a = poly_ofs[k], b = poly_ofs[k + 1];
+    std::vector<V2> p;
+    for (int32_t i = a; i < b; i++) p.push_back(v2(xy[2 * i], xy[2 * i + 1]));
+    sh.add(p.data(), (int32_t)p.size());
+  }
+  return fill_shapes_impl(img, sh, rgbx, winding_rule, blend_mode);
+}
+
+int pxh_draw(pxh_image* a, const pxh_image* b, int32_t dx, int32_t dy, int32_t blend_mode) {
+  PXH_INIT();
+  if (int r = check_image(a)) return r;
+  if (int r = check_image(b)) return r;
+  if (a->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "draw target must be RGBA8");
+  if (blend_mode < 0 || blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  // blendRect, images.nim:468-483
+  if (dx >= a->w || dx + b->w <= 0 || dy >= a->h || dy + b->h <= 0) {
+    if (blend_mode == kMask) return pxh_image_fill(a, 0);
+    return 0;
+  }
+  CompositeArgs c;
+  c.dst = (uint32_t*)a->dev;
+  c.src = b->dev;
+  c.aw = a->w;
+  c.ah = a->h;
+  c.bw = b->w;
+  c.bh = b->h;
+  c.px = dx;
+  c.py = dy;
+  c.xStart = -dx > 0 ? -dx : 0;
+  c.yStart = -dy > 0 ? -dy : 0;
+  c.xEnd = b->w < a->w - dx ? b->w : a->w - dx;
+  c.yEnd = b->h < a->h - dy ? b->h : a->h - dy;
+  c.mode = blend_mode;
+  if (launch_composite(c, b->fmt == PXH_FORMAT_A8, g.stream) != 0)
+    return fail(kErrDevice, "composite launch failed");
+  return 0;
+}
+
+int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "blur needs an RGBA8 image");
+  int64_t r = nim_f2i(roundf(radius));  // images.nim:308
+  if (r == 0) return 0;
+  if (r < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
+  if (r > 1 << 20) return fail(kErrCapacity, "blur radius too large");
+  uint32_t* p = (uint32_t*)img->dev;
+  return blur_impl(p, p, img->w, img->h, (int)r, oob_rgbx, false, 0);
+}
+
+int pxh_spread(pxh_image* img, float spread) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "spread needs an RGBA8 image");
+  int64_t s = nim_f2i(roundf(spread));  // images.nim:702
+  if (s == 0) return 0;
+  if (int e = g.plane.ensure((size_t)img->w * img->h)) return fail(e, "plane allocation failed");
+  SpreadArgs a{(const uint32_t*)img->dev, (uint8_t*)g.plane.p, (uint32_t*)img->dev, img->w, img->h,
+               (int32_t)(s > 0 ? s : -s), s > 0 ? 1 : 0, 0, 0};
+  if (launch_spread(a, g.stream) != 0) return fail(kErrDevice, "spread launch failed");
+  return 0;
+}
+
+int pxh_shadow(pxh_image* out, const pxh_image* src, float offset_x, float offset_y, float spread,
+               float blur, uint32_t rgbx) {
+  PXH_INIT();
+  if (int r = check_image(out)) return r;
+  if (int r = check_image(src)) return r;
+  if (out->w != src->w || out->h != src->h || out->fmt != PXH_FORMAT_RGBA8 ||
+      src->fmt != PXH_FORMAT_RGBA8)
+    return fail(kErrArg, "shadow needs RGBA8 images of the same size");
+  if (offset_x != floorf(offset_x) || offset_y != floorf(offset_y))
+    return fail(kErrArg, "fractional shadow offsets need drawSmooth");
+  int64_t br = nim_f2i(roundf(blur));
+  if (br < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
+  const int32_t w = src->w, h = src->h;
+  const int32_t dx = (int32_t)offset_x, dy = (int32_t)offset_y;
+  if (int e = g.tmp2.ensure((size_t)w * h * 4)) return fail(e, "shadow temporary allocation failed");
+  uint32_t* mask = (uint32_t*)g.tmp2.p;
+  int64_t s = nim_f2i(roundf(spread));
+  SpreadArgs sa{(const uint32_t*)src->dev, nullptr, mask, w, h, (int32_t)(s > 0 ? s : -s),
+                s > 0 ? 1 : 0, dx, dy};
+  if (s != 0) {
+    // mask = draw(image, translate(offset), OverwriteBlend); spread(mask)
+    if (int e = g.plane.ensure((size_t)w * h)) return fail(e, "plane allocation failed");
+    sa.plane = (uint8_t*)g.plane.p;
+    if (launch_spread(sa, g.stream) != 0) return fail(kErrDevice, "spread launch failed");
+  } else {
+    if (launch_shift(sa, g.stream) != 0) return fail(kErrDevice, "shift launch failed");
+  }
+  if (br != 0) {
+    // blur(mask); result = fill(color).draw(mask, MaskBlend), fused into the Y pass
+    return blur_impl(mask, (uint32_t*)out->dev, w, h, (int)br, 0u, true, rgbx);
+  }
+  if (int r = pxh_image_fill(out, rgbx)) return r;
+  CompositeArgs c{(uint32_t*)out->dev, mask, w, h, w, h, 0, 0, 0, w, 0, h, kMask};
+  if (launch_composite(c, false, g.stream) != 0) return fail(kErrDevice, "composite launch failed");
+  return 0;
+}
+
+int pxh_apply_opacity(pxh_image* img, float opacity) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "applyOpacity needs an RGBA8 image");
+  uint32_t o = (uint16_t)nim_f2u32(roundf(255.0f * opacity));  // images.nim:263
+  if (o == 255) return 0;
+  if (o == 0) return pxh_image_fill(img, 0);
+  if (launch_opacity((uint32_t*)img->dev, (int64_t)img->w * img->h, o, g.stream) != 0)
+    return fail(kErrDevice, "opacity launch failed");
+  return 0;
+}
+
+int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_t* source,
+                     uint32_t* out, int64_t n) {
+  PXH_INIT();
+  if (blend_mode < 0 || blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (n <= 0) return 0;
+  size_t bytes = (size_t)n * 4;
+  if (int e = g.blendBuf.ensure(bytes * 3)) return fail(e, "allocation failed");
+  uint32_t* d = (uint32_t*)g.blendBuf.p;
+  PXH_HIP(hipMemcpyAsync(d, backdrop, bytes, hipMemcpyHostToDevice, g.stream));
+  PXH_HIP(hipMemcpyAsync(d + n, source, bytes, hipMemcpyHostToDevice, g.stream));
+  if (launch_blend_pixels(d, d + n, d + 2 * n, n, blend_mode, g.stream) != 0)
+    return fail(kErrDevice, "blend launch failed");
+  PXH_HIP(hipMemcpyAsync(out, d + 2 * n, bytes, hipMemcpyDeviceToHost, g.stream));
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+// Host geometry of one fillPath / strokePath call without touching the
+// device: the FillInfo, bands, entries and windings the raster kernel would
+// receive.  Arrays are malloc'd (release with pxh_free).  stroke_width <= 0
+// means fill.  For tests of the geometry against the reference semantics.
+int pxh_debug_fill_geometry(const float* cmds, int32_t ncmds, const float* mat9, int32_t w,
+                            int32_t h, int32_t stroke, float stroke_width, int32_t line_cap,
+                            int32_t line_join, float miter_limit, int32_t* info16,
+                            int32_t** parts4, int32_t* nparts, float** entries4,
+                            int8_t** windings, int32_t* nentries) {
+  if (!mat9) mat9 = kIdentity;
+  float ps = pixel_scale(mat9);
+  Shapes sh, st;
+  std::string err;
+  int32_t r = commands_to_shapes(cmds, ncmds, !stroke, ps, &sh, &err);
+  if (r != kOk) return fail(r, err);
+  if (stroke) {
+    StrokeParams sp;
+    sp.width = stroke_width;
+    sp.cap = line_cap;
+    sp.join = line_join;
+    sp.miterLimit = miter_limit;
+    r = stroke_shapes(sh, sp, ps, &st, &err);
+    if (r != kOk) return fail(r, err);
+    sh = st;
+  }
+  transform_shapes(&sh, mat9);
+  FillGeom geo;
+  r = build_fill(sh, w, h, 0xFFFFFFFFu, 0, 0, &geo, &err);
+  memcpy(info16, &geo.info, sizeof(FillInfo));
+  *nparts = (int32_t)geo.parts.size();
+  *nentries = (int32_t)geo.entries.size();
+  *parts4 = (int32_t*)malloc(sizeof(PartInfo) * (geo.parts.size() + 1));
+  *entries4 = (float*)malloc(sizeof(Entry) * (geo.entries.size() + 1));
+  *windings = (int8_t*)malloc(geo.windings.size() + 1);
+  if (!geo.parts.empty()) memcpy(*parts4, geo.parts.data(), sizeof(PartInfo) * geo.parts.size());
+  if (!geo.entries.empty()) memcpy(*entries4, geo.entries.data(), sizeof(Entry) * geo.entries.size());
+  if (!geo.windings.empty()) memcpy(*windings, geo.windings.data(), geo.windings.size());
+  if (r != kOk) return fail(r, err);
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,66 @@
+// pxh_kernels.h -- argument blocks and launchers of the HIP kernels
+// (pxh_kernels.hip).  Host code includes this; device pointers only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "pxh_common.h"
+
+namespace pxh {
+
+struct FillArgs {
+  const FillInfo* F;     // one fill
+  const PartInfo* parts; // its bands
+  const Entry* E;        // its entries
+  const int8_t* EW;      // entry windings
+  void* canvas;          // RGBA8 (uint32 per pixel) or A8 target
+  int32_t rowBase, nRows;
+  int32_t cap, ww;       // row scratch capacity, window width
+  int32_t capInLds;      // cap arrays in LDS (else gscratch)
+  char* gscratch;        // per-workgroup cap arrays when not in LDS
+  int32_t* err;          // set to 1 when a band exceeds cap
+};
+
+struct CompositeArgs {
+  uint32_t* dst;
+  const void* src;       // RGBA8 or A8 source
+  int32_t aw, ah, bw, bh;
+  int32_t px, py;        // integer translation of src
+  int32_t xStart, xEnd, yStart, yEnd;  // src rect that lands inside dst
+  int32_t mode;
+};
+
+struct BlurArgs {
+  const uint32_t* src;
+  uint32_t* dst;
+  const uint32_t* lut;   // 2r+1 taps (device)
+  int32_t w, h, radius;
+  uint32_t oob;
+  int32_t shadowColor;   // Y pass only: write blendMask(color, result)
+  uint32_t color;
+};
+
+struct SpreadArgs {
+  const uint32_t* src;
+  uint8_t* plane;        // alpha plane scratch (w*h)
+  uint32_t* dst;
+  int32_t w, h, s, grow;
+  int32_t dx, dy;        // read src translated by (dx, dy)
+};
+
+int launch_fill_rows(const FillArgs& a, int grid, size_t ldsBytes, bool a8, hipStream_t s);
+int set_fill_lds_limit(size_t bytes);
+int launch_composite(const CompositeArgs& a, bool srcA8, hipStream_t s);
+int launch_blur(const BlurArgs& ax, const BlurArgs& ay, hipStream_t s);
+int set_blur_lds_limit(size_t bytes);
+size_t blur_x_lds(int r);
+size_t blur_y_lds(int r);
+int launch_spread(const SpreadArgs& a, hipStream_t s);
+int launch_shift(const SpreadArgs& a, hipStream_t s);
+int launch_opacity(uint32_t* img, int64_t n, uint32_t o, hipStream_t s);
+int launch_blend_pixels(const uint32_t* b, const uint32_t* s_, uint32_t* out, int64_t n,
+                        int32_t mode, hipStream_t s);
+
+}  // namespace pxh

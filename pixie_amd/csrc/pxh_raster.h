@@ -1,0 +1,698 @@
+// pxh_raster.h -- the per-row raster program of fillShapes (paths.nim:1631-1908)
+// as wave-level (64-lane) device code, and its fused scan -> paint -> blend ->
+// write-back epilogue.  Included by the HIP translation units only.
+//
+// One wavefront owns one (fill, row, window) item.  The row program decides
+// pixie's per-row path (two-spanning fill, trapezoid, 5-sample coverage or
+// 1-sample fill), computes the row's pixel operations as runs of codes and
+// adds them to a difference array D in LDS.  The epilogue prefix-scans D
+// across the window (one wave scan per 256 pixels) and applies each pixel's
+// operation to the canvas in HBM with 16-byte accesses.
+//
+// Pixel codes (prefix sums of D):
+//   coverage rows: 0..255 = accumulated coverage; kClear = cleared (mask mode)
+//   other rows:    0 = untouched, kFill = blend with paint, kArea = blend with
+//                  paint * AX[x] (trapezoid sliver), kClear = cleared
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pxh_common.h"
+#include "pxh_geom.h"
+#include "pxh_pixel.h"
+
+namespace pxh {
+
+constexpr int kWave = 64;
+constexpr int32_t kFill = 256;
+constexpr int32_t kArea = 512;
+constexpr int32_t kClear = 1024;
+
+enum RowKind : int32_t { kRowNone = 0, kRowCoverage = 1, kRowOps = 2 };
+
+// Per-wave row scratch.  cap = capacity (power of two >= 64) for selected
+// entries / hits of one row; ww = window width in pixels (multiple of 256).
+// D and AX always live in LDS; the cap-sized arrays live in LDS when they fit
+// and otherwise in a per-wave global scratch slot (same code either way).
+struct RowLds {
+  int32_t* D;        // ww + 4: difference array of pixel codes
+  uint32_t* AX;      // ww: coverage factor of trapezoid sliver pixels
+  uint64_t* keys;    // cap: sort keys (value << 32 | position)
+  int32_t* A;        // cap: selected entries (band-local indices)
+  int32_t* B;        // cap: reordered selection / hit windings by position
+  float* tx0;        // cap: trapezoid x at scanTop (by selection position)
+  float* tx1;        // cap: trapezoid x at scanBottom
+  int cap, ww;
+};
+
+PXH_HD size_t row_lds_fixed_bytes(int ww) { return (size_t)(ww + 4) * 4 + (size_t)ww * 4; }
+PXH_HD size_t row_cap_bytes(int cap) { return (size_t)cap * 24; }
+
+// D/AX from `fixed` (LDS), cap arrays from `arr` (LDS or global).
+__device__ __forceinline__ RowLds row_lds_carve(char* fixed, char* arr, int cap, int ww) {
+  RowLds L;
+  L.cap = cap;
+  L.ww = ww;
+  L.D = (int32_t*)fixed;
+  L.AX = (uint32_t*)(L.D + ww + 4);
+  L.keys = (uint64_t*)arr;
+  L.A = (int32_t*)(L.keys + cap);
+  L.B = L.A + cap;
+  L.tx0 = (float*)(L.B + cap);
+  L.tx1 = L.tx0 + cap;
+  return L;
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+__device__ __forceinline__ int32_t wave_excl_scan(int32_t v, int32_t* total) {
+  int lane = lane_id();
+  int32_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int32_t y = __shfl_up(x, o, kWave);
+    if (lane >= o) x += y;
+  }
+  *total = __shfl(x, kWave - 1, kWave);
+  return x - v;
+}
+
+__device__ __forceinline__ int32_t wave_excl_max(int32_t v) {
+  int lane = lane_id();
+  int32_t x = v;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    int32_t y = __shfl_up(x, o, kWave);
+    if (lane >= o) x = x > y ? x : y;
+  }
+  int32_t e = __shfl_up(x, 1, kWave);
+  return lane == 0 ? INT32_MIN : e;
+}
+
+__device__ __forceinline__ int32_t wave_max(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    int32_t y = __shfl_xor(v, o, kWave);
+    v = v > y ? v : y;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void wave_sync() { __syncthreads(); }
+
+// Bitonic sort of n keys in L.keys (n <= cap, cap a power of two).  Keys
+// beyond n are padded with UINT64_MAX.  The whole wave participates.
+__device__ void wave_sort_keys(uint64_t* keys, int n, int cap) {
+  int lane = lane_id();
+  int np = 64;
+  while (np < n) np <<= 1;
+  if (np > cap) np = cap;  // n <= cap guaranteed by the caller
+  for (int i = n + lane; i < np; i += kWave) keys[i] = ~0ull;
+  wave_sync();
+  if (np == 64) {
+    uint64_t k = keys[lane];
+#pragma unroll
+    for (int size = 2; size <= 64; size <<= 1) {
+#pragma unroll
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        uint64_t o = __shfl_xor(k, stride, kWave);
+        bool up = ((lane & size) == 0);
+        bool lower = (lane & stride) == 0;
+        uint64_t mn = k < o ? k : o, mx = k < o ? o : k;
+        k = (lower == up) ? mn : mx;
+      }
+    }
+    keys[lane] = k;
+    wave_sync();
+    return;
+  }
+  for (int size = 2; size <= np; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = lane; t < np / 2; t += kWave) {
+        int i = 2 * t - (t & (stride - 1));
+        int j = i + stride;
+        bool up = ((i & size) == 0);
+        uint64_t a = keys[i], b = keys[j];
+        if ((a > b) == up) {
+          keys[i] = b;
+          keys[j] = a;
+        }
+      }
+      wave_sync();
+    }
+  }
+}
+
+// Order-preserving map of a float to an unsigned key (finite values).
+__device__ __forceinline__ uint32_t float_key(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+struct RowCtx {
+  const FillInfo* F;
+  const PartInfo* parts;  // this fill's partitions
+  const Entry* E;         // this fill's entries
+  const int8_t* EW;       // windings
+  int32_t y;
+  int32_t wx0, wx1;       // window [wx0, wx1)
+  int32_t cx0, cx1;       // clip of this row's writes
+};
+
+// Adds value v to pixels [a, b) of the window (clipped), via D.
+__device__ __forceinline__ void add_run(const RowCtx& C, RowLds& L, int64_t a, int64_t b, int32_t v) {
+  int64_t lo = a > C.cx0 ? a : C.cx0;
+  int64_t hi = b < C.cx1 ? b : C.cx1;
+  if (lo < C.wx0) lo = C.wx0;
+  if (hi > C.wx1) hi = C.wx1;
+  if (lo >= hi) return;
+  atomicAdd(&L.D[lo - C.wx0], v);
+  atomicAdd(&L.D[hi - C.wx0], -v);
+}
+
+// The walk iterator (paths.nim:1298-1330) over the n sorted hits in L.keys
+// (at in the high word, hit position in the low word; windings in L.B by
+// position), executed in parallel.  Calls span(prevAt, at) per yielded span.
+__device__ __forceinline__ int32_t key_at(uint64_t k) {
+  return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+
+template <class SpanFn>
+__device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
+  int lane = lane_id();
+  int32_t carryCount = 0;
+  int32_t carryPrev = 0;      // prevAt so far (0 before any event)
+  bool carryConsumed = false; // first hit of this chunk consumed by a skip
+  for (int base = 0; base < n; base += kWave) {
+    int i = base + lane;
+    bool valid = i < n;
+    uint64_t k = valid ? L.keys[i] : 0ull;
+    int32_t at = valid ? key_at(k) : 0;
+    int32_t w = valid ? L.B[(uint32_t)k] : 0;
+    int32_t tot;
+    int32_t countBefore = carryCount + wave_excl_scan(w, &tot);
+    bool filling = should_fill(rule, countBefore);
+    bool hasNext = valid && i < n - 1;
+    uint64_t kn = hasNext ? L.keys[i + 1] : 0ull;
+    int32_t nextAt = hasNext ? key_at(kn) : 0;
+    int32_t nextW = hasNext ? L.B[(uint32_t)kn] : 0;
+    bool canSkip = valid && at > 0 && filling && hasNext && nextAt == at && w + nextW == 0;
+    uint64_t cm = ballot(canSkip);
+    uint64_t skipMask = 0, consMask = carryConsumed ? 1ull : 0ull;
+    bool carryOut = false;
+    while (cm) {
+      int b = __builtin_ctzll(cm);
+      cm &= cm - 1;
+      if (!((consMask >> b) & 1ull)) {
+        skipMask |= 1ull << b;
+        if (b + 1 < kWave) consMask |= 1ull << (b + 1);
+        else carryOut = true;
+      }
+    }
+    bool consumed = (consMask >> lane) & 1ull;
+    bool skipStart = (skipMask >> lane) & 1ull;
+    // 0 = none, 1 = start (prevAt = at), 2 = end (yield, prevAt = at)
+    int ev = 0;
+    if (valid && !consumed && !skipStart && at > 0) {
+      if (filling) {
+        bool cont = (rule == kNonZero) && hasNext && (countBefore + w != 0);
+        ev = cont ? 0 : 2;
+      } else {
+        ev = 1;
+      }
+    }
+    int32_t prevHere = wave_excl_max(ev ? at : INT32_MIN);
+    int32_t prevAt = prevHere > carryPrev ? prevHere : carryPrev;
+    if (ev == 2) span(prevAt, at);
+    int32_t lastEv = wave_max(ev ? at : INT32_MIN);
+    if (lastEv > carryPrev) carryPrev = lastEv;
+    carryCount += tot;
+    carryConsumed = carryOut;
+  }
+}
+
+// Coverage contributions of one span (computeCoverage 1390-1431).
+struct CovSpan {
+  const RowCtx* C;
+  RowLds* L;
+  int32_t sc;        // sampleCoverage
+  int32_t covLo, covHi;  // coverage buffer span [startX, startX + pathWidth)
+  __device__ void add_px(int64_t x, int32_t v) {
+    if (x < covLo || x >= covHi) return;
+    if (x < C->wx0 || x >= C->wx1) return;
+    atomicAdd(&L->D[x - C->wx0], v);
+    atomicAdd(&L->D[x - C->wx0 + 1], -v);
+  }
+  __device__ void operator()(int32_t prevAt, int32_t at) {
+    int64_t fillStart = fx_int(prevAt);
+    bool pixelCrossed = fx_int(at) != fx_int(prevAt);
+    int32_t leftCover = pixelCrossed ? fx_trunc(prevAt) + 256 - prevAt : at - prevAt;
+    if (leftCover != 0) {
+      fillStart++;
+      add_px(fx_int(prevAt), (int32_t)(uint8_t)fx_int(leftCover * sc));
+    }
+    if (pixelCrossed) {
+      int32_t rightCover = at - fx_trunc(at);
+      if (rightCover > 0) add_px(fx_int(at), (int32_t)(uint8_t)fx_int(rightCover * sc));
+    }
+    int64_t fillEnd = fx_int(at);
+    if (fillEnd > fillStart) {
+      int64_t lo = fillStart > covLo ? fillStart : covLo;
+      int64_t hi = fillEnd < covHi ? fillEnd : covHi;
+      if (lo < C->wx0) lo = C->wx0;
+      if (hi > C->wx1) hi = C->wx1;
+      if (lo < hi) {
+        atomicAdd(&L->D[lo - C->wx0], sc);
+        atomicAdd(&L->D[hi - C->wx0], -sc);
+      }
+    }
+  }
+};
+
+// Fill runs of a walkInteger span (fillHits 1540-1591).
+struct FillSpan {
+  const RowCtx* C;
+  RowLds* L;
+  int32_t code;   // value added over the span
+  int64_t minX;   // spans are cut below minX (mask mode: left of startX is cleared)
+  __device__ void operator()(int32_t prevAt, int32_t at) {
+    int64_t s = fx_int(prevAt), e = fx_int(at);
+    if (e - s <= 0) return;
+    if (s < minX) s = minX;
+    add_run(*C, *L, s, e, code);
+  }
+};
+
+// Collects the hits of one sample line (paths.nim:1373-1388) from the
+// selection L.A[0..ne) into L.keys sorted stably by x; windings go to L.B by
+// hit position.  Returns the number of hits.
+__device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine) {
+  int lane = lane_id();
+  const float Wf = (float)C.F->canvasW;
+  int nh = 0;
+  for (int base = 0; base < ne; base += kWave) {
+    int i = base + lane;
+    bool hit = false;
+    int32_t at = 0, w = 0;
+    if (i < ne) {
+      int k = L.A[i];
+      Entry e = C.E[k];
+      if (e.aty <= yLine && e.toy >= yLine) {
+        float x = e.m == 0.0f ? e.b : (yLine - e.b) / e.m;
+        at = fixed32(nim_minf(x, Wf));
+        w = C.EW[k];
+        hit = true;
+      }
+    }
+    uint64_t m = ballot(hit);
+    int pos = nh + __popcll(m & ((1ull << lane) - 1ull));
+    if (hit) {
+      L.keys[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | (uint32_t)pos;
+      L.B[pos] = w;
+    }
+    nh += __popcll(m);
+  }
+  wave_sync();
+  if (nh > 1) wave_sort_keys(L.keys, nh, L.cap);
+  return nh;
+}
+
+// Runs the row program for (fill, row y, window) and leaves the row's codes
+// in L.D (and L.AX).  Returns the row kind; kRowNone means nothing to do and
+// -1 means the band holds more entries than L.cap (nothing was written).
+__device__ int32_t row_program(const RowCtx& C, RowLds& L) {
+  const FillInfo& F = *C.F;
+  const int lane = lane_id();
+  const int32_t W = F.canvasW;
+  const int32_t y = C.y;
+  const bool mask = F.mode == kMask;
+
+  if (y < F.startY || y >= F.pathHeight) {
+    if (mask) {  // rows outside the path are cleared (paths.nim:1910-1912)
+      if (lane == 0) add_run(C, L, 0, W, kClear);
+      wave_sync();
+      return kRowOps;
+    }
+    return kRowNone;
+  }
+  int32_t pi = (y - F.startY) / F.bandH;
+  if (pi > F.nParts - 1) pi = F.nParts - 1;
+  const PartInfo P = C.parts[pi];
+  if (P.count > L.cap) return -1;
+  const bool aa = P.flags & 1, two = P.flags & 2;
+  RowCtx Cp = C;
+  Cp.E = C.E + P.entryBase;
+  Cp.EW = C.EW + P.entryBase;
+  const Entry* E = Cp.E;
+  const int8_t* EW = Cp.EW;
+
+  if (mask && lane == 0) add_run(C, L, 0, W, kClear);  // default for mask rows
+
+  if (two && !aa) {  // paths.nim:1644-1668
+    if (lane == 0) {
+      int64_t left = nim_f2i(E[0].b), right = nim_f2i(E[1].b);
+      int64_t minX = left < 0 ? 0 : (left > W ? W : left);
+      int64_t maxX = right < 0 ? 0 : (right > W ? W : right);
+      if (maxX > minX) add_run(C, L, minX, maxX, mask ? kFill - kClear : kFill);
+    }
+    wave_sync();
+    return kRowOps;
+  }
+
+  const float scanTop = (float)y, scanBottom = (float)(y + 1);
+  int ne = 0;
+  bool allSpan = true;
+  if (two) {
+    if (lane == 0) {
+      L.A[0] = 0;
+      L.A[1] = 1;
+    }
+    ne = 2;
+  } else {
+    for (int base = 0; base < P.count; base += kWave) {
+      int i = base + lane;
+      bool s = false, notSpan = false;
+      if (i < P.count) {
+        Entry e = E[i];
+        s = !(e.toy <= scanTop || e.aty >= scanBottom);
+        notSpan = s && (e.aty > scanTop || e.toy < scanBottom);
+      }
+      uint64_t m = ballot(s);
+      int pos = ne + __popcll(m & ((1ull << lane) - 1ull));
+      if (s) L.A[pos] = i;
+      ne += __popcll(m);
+      if (ballot(notSpan)) allSpan = false;
+    }
+  }
+  wave_sync();
+
+  if (allSpan && (ne % 2) == 0) {  // trapezoid fast path, 1691-1872
+    for (int i = lane; i < ne; i += kWave) {
+      Entry e = E[L.A[i]];
+      float a = solve_x(e.m, e.b, scanTop), b = solve_x(e.m, e.b, scanBottom);
+      L.tx0[i] = a;
+      L.tx1[i] = b;
+      L.keys[i] = ((uint64_t)float_key(midpoint_x(v2(a, scanTop), v2(b, scanBottom))) << 32) |
+                  (uint32_t)i;
+    }
+    wave_sync();
+    if (ne > 1) wave_sort_keys(L.keys, ne, L.cap);
+    // the partial coverage areas must not overlap (1720-1728) and the pairs
+    // must be simple fills (1731-1745)
+    bool bad = false;
+    int32_t carry = 0;
+    for (int base = 0; base < ne; base += kWave) {
+      int i = base + lane;
+      bool fail = false;
+      int32_t w = 0;
+      if (i < ne) {
+        int pi0 = (int)(uint32_t)L.keys[i];
+        w = EW[L.A[pi0]];
+        if (i < ne - 1) {
+          int pi1 = (int)(uint32_t)L.keys[i + 1];
+          float lmax = nim_maxf(L.tx0[pi0], L.tx1[pi0]);
+          float rmin = nim_minf(L.tx0[pi1], L.tx1[pi1]);
+          if (nim_f2i(ceilf(lmax)) > nim_f2i(rmin)) fail = true;
+        }
+      }
+      int32_t tot;
+      int32_t incl = carry + wave_excl_scan(w, &tot) + w;
+      if (i < ne) {
+        bool f = should_fill(F.rule, incl);
+        if ((i % 2 == 0) != f) fail = true;
+      }
+      carry += tot;
+      if (ballot(fail)) bad = true;
+    }
+    if (!bad) {
+      const int32_t areaCode = mask ? kArea - kClear : kArea;
+      for (int p = lane; p < ne / 2; p += kWave) {  // one lane per pair
+        int pl = (int)(uint32_t)L.keys[2 * p], pr = (int)(uint32_t)L.keys[2 * p + 1];
+        Entry le = E[L.A[pl]], re = E[L.A[pr]];
+        float la = L.tx0[pl], lt = L.tx1[pl];
+        float ra = L.tx0[pr], rt = L.tx1[pr];
+        float leftMaxX = nim_maxf(la, lt);
+        float rightMinX = nim_minf(ra, rt);
+        int64_t leftCoverEnd = nim_f2i(ceilf(leftMaxX));
+        int64_t rightCoverBegin = nim_f2i(truncf(rightMinX));
+        {  // left sliver 1772-1809
+          bool inverted = la < lt;
+          float sliverStart = nim_minf(la, lt);
+          float rectStart = leftMaxX;
+          float pen = sliverStart, prevPen = pen;
+          float penY = inverted ? (float)y : (float)(y + 1);
+          float prevPenY = penY;
+          int64_t x0 = nim_f2i(sliverStart), x1 = nim_f2i(ceilf(rectStart));
+          for (int64_t x = x0; x < x1; x++) {
+            prevPen = pen;
+            pen = (float)(x + 1);
+            float rightRectArea = 0.0f;
+            if (pen > rectStart) {
+              rightRectArea = pen - rectStart;
+              pen = rectStart;
+            }
+            prevPenY = penY;
+            penY = solve_y(le.m, le.b, pen);
+            if (x < 0 || x >= W) continue;
+            float run = pen - prevPen;
+            float triangleArea = 0.5f * run * fabsf(penY - prevPenY);
+            float rectArea = inverted ? (prevPenY - (float)y) * run : ((float)(y + 1) - prevPenY) * run;
+            float area = triangleArea + rectArea + rightRectArea;
+            if (x >= C.cx0 && x < C.cx1 && x >= C.wx0 && x < C.wx1) {
+              L.AX[x - C.wx0] = area_factor(area);
+              atomicAdd(&L.D[x - C.wx0], areaCode);
+              atomicAdd(&L.D[x - C.wx0 + 1], -areaCode);
+            }
+          }
+        }
+        {  // right sliver 1811-1847
+          bool inverted = ra > rt;
+          float rectEnd = rightMinX;
+          float sliverEnd = nim_maxf(ra, rt);
+          float pen = rectEnd, prevPen = pen;
+          float penY = inverted ? (float)(y + 1) : (float)y;
+          float prevPenY = penY;
+          int64_t x0 = nim_f2i(rectEnd), x1 = nim_f2i(ceilf(sliverEnd));
+          for (int64_t x = x0; x < x1; x++) {
+            prevPen = pen;
+            pen = (float)(x + 1);
+            float ap = fabsf(prevPen);
+            float leftRectArea = ap - floorf(ap);
+            if (pen > sliverEnd) pen = sliverEnd;
+            prevPenY = penY;
+            penY = solve_y(re.m, re.b, pen);
+            if (x < 0 || x >= W) continue;
+            float run = pen - prevPen;
+            float triangleArea = 0.5f * run * fabsf(penY - prevPenY);
+            float rectArea = inverted ? (penY - (float)y) * run : ((float)(y + 1) - penY) * run;
+            float area = leftRectArea + triangleArea + rectArea;
+            if (x >= C.cx0 && x < C.cx1 && x >= C.wx0 && x < C.wx1) {
+              L.AX[x - C.wx0] = area_factor(area);
+              atomicAdd(&L.D[x - C.wx0], areaCode);
+              atomicAdd(&L.D[x - C.wx0 + 1], -areaCode);
+            }
+          }
+        }
+        int64_t fb = leftCoverEnd < 0 ? 0 : (leftCoverEnd > W ? W : leftCoverEnd);
+        int64_t fe = rightCoverBegin < 0 ? 0 : (rightCoverBegin > W ? W : rightCoverBegin);
+        // interior: fillHits on (fixed32(fb), +1), (fixed32(fe), -1)
+        if (fe > fb) add_run(C, L, fb, fe, mask ? kFill - kClear : kFill);
+      }
+      wave_sync();
+      return kRowOps;
+    }
+    // Not a simple trapezoid row: pixie keeps the midpoint order of the
+    // selection for the coverage pass below.
+    for (int i = lane; i < ne; i += kWave) L.B[i] = L.A[(int)(uint32_t)L.keys[i]];
+    wave_sync();
+    for (int i = lane; i < ne; i += kWave) L.A[i] = L.B[i];
+    wave_sync();
+  }
+
+  // computeCoverage 1350-1431 / fillHits 1898
+  const int quality = aa ? 5 : 1;
+  const int32_t sampleCoverage = 255 / quality;
+  const float offset = 1.0f / (float)quality;
+  const float initialOffset = offset / 2.0f + 0.00031415926535897936f;
+  float yLine = (float)y + initialOffset - offset;
+  const int64_t covLo = F.startX, covHi = (int64_t)F.startX + F.pathWidth;
+  if (aa) {
+    if (mask && lane == 0) add_run(C, L, covLo, covHi, -kClear);  // coverage range
+    CovSpan cs{&C, &L, sampleCoverage, (int32_t)covLo, (int32_t)covHi};
+    for (int m = 0; m < quality; m++) {
+      yLine += offset;
+      int nh = row_hits(Cp, L, ne, yLine);
+      wave_walk(L, nh, F.rule, cs);
+      wave_sync();
+    }
+    return kRowCoverage;
+  }
+  yLine += offset;
+  int nh = row_hits(Cp, L, ne, yLine);
+  FillSpan fs{&C, &L, mask ? kFill - kClear : kFill, mask ? (int64_t)F.startX : (int64_t)INT64_MIN};
+  wave_walk(L, nh, F.rule, fs);
+  wave_sync();
+  return kRowOps;
+}
+
+// Applies one pixel's code.  Returns the new pixel value.
+__device__ __forceinline__ uint32_t apply_code(int32_t kind, int32_t code, uint32_t ax,
+                                               uint32_t dst, uint32_t rgbx, int32_t mode,
+                                               bool* touched) {
+  if (kind == kRowCoverage) {
+    if (code >= kClear) {
+      *touched = true;
+      return 0;
+    }
+    uint32_t c = (uint32_t)code & 255u;
+    bool doit = (mode == kMask) ? (c != 255u) : (c != 0u);
+    if (!doit) return dst;
+    *touched = true;
+    uint32_t src = mul_cov(rgbx, c);
+    return mode == kOverwrite ? src : blend(mode, dst, src);
+  }
+  if (code == 0) return dst;
+  *touched = true;
+  if (code >= kClear) return 0;
+  uint32_t src = code >= kArea ? mul_cov(rgbx, ax) : rgbx;
+  return blend(mode, dst, src);
+}
+
+// Scans L.D over the window and applies the codes to canvas row `row`.
+// Leaves L.D zeroed for the next row.  Canvas rows are W pixels wide.
+__device__ void row_epilogue(const RowCtx& C, RowLds& L, int32_t kind, uint32_t* canvas,
+                             int32_t W, uint32_t rgbx, int32_t mode) {
+  const int lane = lane_id();
+  const int32_t wlen = C.wx1 - C.wx0;
+  uint32_t* rowp = canvas + (size_t)C.y * (size_t)W;
+  const bool vec_ok = ((W & 3) == 0) && ((C.wx0 & 3) == 0);
+  int32_t carry = 0;
+  for (int base = 0; base < wlen; base += 4 * kWave) {
+    int i0 = base + 4 * lane;
+    int32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+    if (i0 < wlen + 1) {  // D has ww + 4 entries; indices up to ww + 3 valid
+      d0 = L.D[i0];
+      d1 = L.D[i0 + 1];
+      d2 = L.D[i0 + 2];
+      d3 = L.D[i0 + 3];
+      L.D[i0] = 0;
+      L.D[i0 + 1] = 0;
+      L.D[i0 + 2] = 0;
+      L.D[i0 + 3] = 0;
+    }
+    int32_t c0 = d0, c1 = c0 + d1, c2 = c1 + d2, c3 = c2 + d3;
+    int32_t tot;
+    int32_t off = carry + wave_excl_scan(c3, &tot);
+    carry += tot;
+    int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (i0 + k < wlen && code[k] != 0) any = true;
+    if (kind == kRowCoverage && mode == kMask) any = true;  // mask coverage rows write every pixel
+    if (!any) continue;
+    int32_t x0 = C.wx0 + i0;
+    uint32_t px[4];
+    bool full = vec_ok && (i0 + 3 < wlen);
+    if (full) {
+      uint4 v = *(const uint4*)(rowp + x0);
+      px[0] = v.x; px[1] = v.y; px[2] = v.z; px[3] = v.w;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) px[k] = (i0 + k < wlen) ? rowp[x0 + k] : 0u;
+    }
+    bool touched = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < wlen) {
+        uint32_t ax = (code[k] >= kArea && code[k] < kClear && kind == kRowOps) ? L.AX[i0 + k] : 0u;
+        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &touched);
+      }
+    }
+    if (!touched) continue;
+    if (full) {
+      *(uint4*)(rowp + x0) = make_uint4(px[0], px[1], px[2], px[3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (i0 + k < wlen) rowp[x0 + k] = px[k];
+    }
+  }
+  // clear the tail entries the loop did not visit
+  for (int i = lane; i < 4; i += kWave) {
+    int idx = ((wlen + 4 * kWave - 1) / (4 * kWave)) * 4 * kWave + i;
+    if (idx < L.ww + 4) L.D[idx] = 0;
+  }
+  wave_sync();
+}
+
+// Epilogue for an A8 target (masks, glyph atlas): pixel v reads as
+// rgbx(v,v,v,v) and the alpha of the result is stored.
+__device__ void row_epilogue_a8(const RowCtx& C, RowLds& L, int32_t kind, uint8_t* canvas,
+                                int32_t W, uint32_t rgbx, int32_t mode) {
+  const int lane = lane_id();
+  const int32_t wlen = C.wx1 - C.wx0;
+  uint8_t* rowp = canvas + (size_t)C.y * (size_t)W;
+  const bool vec_ok = ((W & 3) == 0) && ((C.wx0 & 3) == 0);
+  int32_t carry = 0;
+  for (int base = 0; base < wlen; base += 4 * kWave) {
+    int i0 = base + 4 * lane;
+    int32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+    if (i0 < wlen + 1) {
+      d0 = L.D[i0];
+      d1 = L.D[i0 + 1];
+      d2 = L.D[i0 + 2];
+      d3 = L.D[i0 + 3];
+      L.D[i0] = 0;
+      L.D[i0 + 1] = 0;
+      L.D[i0 + 2] = 0;
+      L.D[i0 + 3] = 0;
+    }
+    int32_t c0 = d0, c1 = c0 + d1, c2 = c1 + d2, c3 = c2 + d3;
+    int32_t tot;
+    int32_t off = carry + wave_excl_scan(c3, &tot);
+    carry += tot;
+    int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+    bool any = kind == kRowCoverage && mode == kMask;
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (i0 + k < wlen && code[k] != 0) any = true;
+    if (!any) continue;
+    int32_t x0 = C.wx0 + i0;
+    uint32_t px[4];
+    bool full = vec_ok && (i0 + 3 < wlen);
+    if (full) {
+      uint32_t v = *(const uint32_t*)(rowp + x0);
+#pragma unroll
+      for (int k = 0; k < 4; k++) px[k] = ((v >> (8 * k)) & 255u) * 0x01010101u;
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++) px[k] = (i0 + k < wlen) ? rowp[x0 + k] * 0x01010101u : 0u;
+    }
+    bool touched = false;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < wlen) {
+        uint32_t ax = (code[k] >= kArea && code[k] < kClear && kind == kRowOps) ? L.AX[i0 + k] : 0u;
+        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &touched);
+      }
+    }
+    if (!touched) continue;
+    if (full) {
+      *(uint32_t*)(rowp + x0) = ch_a(px[0]) | (ch_a(px[1]) << 8) | (ch_a(px[2]) << 16) | (ch_a(px[3]) << 24);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        if (i0 + k < wlen) rowp[x0 + k] = (uint8_t)ch_a(px[k]);
+    }
+  }
+  for (int i = lane; i < 4; i += kWave) {
+    int idx = ((wlen + 4 * kWave - 1) / (4 * kWave)) * 4 * kWave + i;
+    if (idx < L.ww + 4) L.D[idx] = 0;
+  }
+  wave_sync();
+}
+
+}  // namespace pxh
