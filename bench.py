@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""bench.py -- throughput of pixie's fillPath + blend raster path on one MI355X.
+
+Workload (one step): the mixed fill+stroke scene -- 2,000 random closed paths
+of 4-32 cubic Beziers (bbox side lognormal(ln 64, 1) clipped to [4, 2048]),
+50/50 NonZero / EvenOdd, paint alpha in [0.2, 1], the 16 blend modes cycled,
+30% of paths also stroked (width 1-8, random joins and caps) -- rendered onto
+a 4096x4096 premultiplied RGBA8 canvas that is resident in HBM.
+
+value = N_touch / step time, in Mpix/s, where N_touch counts the pixels the
+fills write (the unit pixie's fillShapes does its per-pixel work in).
+Algorithmic bytes = 8 B per touched pixel (RGBA8 read + write).
+
+The CPU baseline is the oracle (a C restatement of pixie's scalar path) on a
+bounded prefix of the same scene, one core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain]
+"""
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mpix/s fillPath+blend 4K RGBA8 (%HBM peak); glyphs/s batched; serve p99 ms"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_plain(px, img, cmds):
+    for c in cmds:
+        if c.stroke:
+            img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                           miterLimit=c.miter, blendMode=c.mode)
+        else:
+            img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+
+
+def cpu_baseline(cmds, canvas, seconds):
+    import oracle
+    img = canvas.copy()
+    touched = 0
+    n = 0
+    t0 = time.perf_counter()
+    for c in cmds:
+        if c.stroke:
+            st = oracle.stroke_path(img, c.cmds, c.color, width=c.width, cap=c.cap, join=c.join,
+                                    miter_limit=c.miter, mode=c.mode)
+        else:
+            st = oracle.fill_path(img, c.cmds, c.color, winding=c.winding, mode=c.mode)
+        touched += st["touched"]
+        n += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": touched / dt / 1e6, "unit": "Mpix/s", "cores": 1, "kind": "port",
+            "sample": "first %d of %d fill/stroke calls of the same scene (N_touch %d), %.1f s; "
+                      "CPU %s" % (n, len(cmds), touched, dt, platform.processor() or platform.machine())}
+
+
+def star_extra(px, WL, reps=50):
+    import oracle
+    cmd, w, h = WL.star512()
+    img = px.Image(w, h)
+    img.fillPath(cmd.cmds, cmd.color)
+    px.synchronize()
+    px.touched_pixels(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        img.fillPath(cmd.cmds, cmd.color)
+    px.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    touched = px.touched_pixels(reset=True) / reps
+    want = oracle.new_image(w, h)
+    t1 = time.perf_counter()
+    st = oracle.fill_path(want, cmd.cmds, cmd.color)
+    cdt = time.perf_counter() - t1
+    return {"workload": "star 64-vertex 512x512 NonZero Normal (one fillPath call)",
+            "gpu_ms_per_call": dt * 1e3, "gpu_mpix_s": touched / dt / 1e6,
+            "cpu_ms_per_call": cdt * 1e3, "cpu_mpix_s": st["touched"] / cdt / 1e6}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--form", default="plain", choices=["plain"])
+    ap.add_argument("--paths", type=int, default=2000)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has one MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import pixie_amd as px
+    from pixie_amd import workloads as WL
+
+    px.init()
+    t = time.perf_counter()
+    cmds, canvas = WL.mixed_scene(n_paths=args.paths)
+    log("scene: %d calls generated in %.1fs" % (len(cmds), time.perf_counter() - t))
+    img = px.Image.from_array(canvas)
+
+    for _ in range(args.warmup):
+        run_plain(px, img, cmds)
+    px.synchronize()
+    px.touched_pixels(reset=True)
+    px.profile_begin()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        run_plain(px, img, cmds)
+        log("step %d done" % k)
+    px.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms, launches = px.profile_end()
+    touched = px.touched_pixels(reset=True)
+
+    per_step_touch = touched / args.steps
+    ms_per_step = wall / args.steps * 1e3
+    value = per_step_touch / (wall / args.steps) / 1e6
+    bytes_per_step = 8.0 * per_step_touch
+    kernel_s_per_step = kernel_ms / args.steps / 1e3
+    achieved = bytes_per_step / kernel_s_per_step / 1e9 if kernel_s_per_step > 0 else 0.0
+    result = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": "mixed fill+stroke scene 4096x4096 RGBA8: %d cubic paths, "
+                               "%d fill/stroke calls, seed 2" % (args.paths, len(cmds)),
+                   "form": args.form, "n_touch_per_step": int(per_step_touch)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "fill_rows_kernel", "launches_per_step": launches // args.steps,
+                     "kernel_ms_per_step": round(kernel_ms / args.steps, 3)},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(cmds, canvas, args.cpu_seconds)
+    if not args.no_extra:
+        result["extra"] = {"star512": star_extra(px, WL)}
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,16 @@ int pxh_apply_opacity(pxh_image* img, float opacity);
 int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_t* source,
                      uint32_t* out, int64_t n);
 
+/* Measurement (no reference counterpart: pixie has no device timeline). */
+/* Pixels written by fills since the last reset (N_touch); reset != 0 zeroes. */
+int pxh_touched_pixels(int64_t* out, int32_t reset);
+/* Records event slot 0..15 on the library's stream / elapsed ms a -> b. */
+int pxh_event_record(int32_t slot);
+int pxh_event_elapsed(int32_t a, int32_t b, float* ms);
+/* Per-launch timing of raster kernels between begin and end (summed ms). */
+int pxh_profile_begin(void);
+int pxh_profile_end(double* kernel_ms, int64_t* launches);
+
 /* Host geometry of one fillPath (stroke = 0) or strokePath (stroke = 1) call
  * without device work: info16 receives the 16-int fill header (startX,
  * startY, pathWidth, pathHeight, nParts, bandH, ...), parts4 4 ints per band

@@ -82,6 +82,10 @@ def lib():
         L.pxh_shadow.argtypes = [vp, vp, f, f, f, f, u32]
         L.pxh_apply_opacity.argtypes = [vp, f]
         L.pxh_blend_pixels.argtypes = [i32, vp, vp, vp, ctypes.c_int64]
+        L.pxh_touched_pixels.argtypes = [P(ctypes.c_int64), i32]
+        L.pxh_event_record.argtypes = [i32]
+        L.pxh_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int64)]
+        L.pxh_event_elapsed.argtypes = [i32, i32, P(ctypes.c_float)]
         L.pxh_debug_fill_geometry.argtypes = [f32p, i32, f32p, i32, i32, i32, f, i32, i32, f,
                                               i32p, P(i32p), i32p, P(f32p),
                                               P(P(ctypes.c_int8)), i32p]
@@ -230,6 +234,34 @@ class Image:
 
     def applyOpacity(self, opacity):
         _check(lib().pxh_apply_opacity(self._h, float(opacity)))
+
+
+def touched_pixels(reset=True):
+    """Pixels written by fills since the last reset (N_touch)."""
+    v = ctypes.c_int64()
+    _check(lib().pxh_touched_pixels(ctypes.byref(v), int(reset)))
+    return v.value
+
+
+def profile_begin():
+    _check(lib().pxh_profile_begin())
+
+
+def profile_end():
+    """(summed kernel ms, launches) since profile_begin()."""
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _check(lib().pxh_profile_end(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def event_record(slot):
+    _check(lib().pxh_event_record(slot))
+
+
+def event_elapsed_ms(a, b):
+    ms = ctypes.c_float()
+    _check(lib().pxh_event_elapsed(a, b, ctypes.byref(ms)))
+    return ms.value
 
 
 def blend_pixels(mode, backdrop, source):

@@ -72,6 +72,12 @@ struct Ctx {
   DevBuf plane;      // alpha plane
   DevBuf blendBuf;   // pxh_blend_pixels
   int32_t* errFlag = nullptr;
+  unsigned long long* touched = nullptr;  // N_touch counter of fills
+  hipEvent_t ev[16] = {};
+  // per-launch kernel timing (pxh_profile_begin / _end)
+  bool profiling = false;
+  std::vector<hipEvent_t> evPool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> evPairs;
   std::map<int, uint32_t*> luts;  // gaussian LUT per radius (device, uint32 taps)
 };
 Ctx g;
@@ -91,6 +97,9 @@ int init() {
   PXH_HIP(hipEventRecord(g.stageFree, g.stream));
   PXH_HIP(hipMalloc((void**)&g.errFlag, sizeof(int32_t)));
   PXH_HIP(hipMemsetAsync(g.errFlag, 0, sizeof(int32_t), g.stream));
+  PXH_HIP(hipMalloc((void**)&g.touched, sizeof(unsigned long long)));
+  PXH_HIP(hipMemsetAsync(g.touched, 0, sizeof(unsigned long long), g.stream));
+  for (auto& e : g.ev) PXH_HIP(hipEventCreate(&e));
   if (set_fill_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(fill)");
   if (set_blur_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(blur)");
   g.inited = true;
@@ -128,6 +137,35 @@ int stage(const std::vector<std::pair<const void*, size_t>>& parts, std::vector<
   for (size_t i = 0; i < parts.size(); i++) out->push_back(g.devStage.p + offs[i]);
   return 0;
 }
+
+hipEvent_t pool_event() {
+  if (!g.evPool.empty()) {
+    hipEvent_t e = g.evPool.back();
+    g.evPool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Brackets one kernel launch with events when profiling is on.
+struct LaunchTimer {
+  hipEvent_t a = nullptr, b = nullptr;
+  LaunchTimer() {
+    if (g.profiling) {
+      a = pool_event();
+      b = pool_event();
+      (void)hipEventRecord(a, g.stream);
+    }
+  }
+  ~LaunchTimer() {
+    if (a) {
+      (void)hipEventRecord(b, g.stream);
+      g.evPairs.push_back({a, b});
+    }
+  }
+};
 
 int next_pow2(int v) {
   int p = 64;
@@ -171,6 +209,7 @@ int raster_fill(pxh_image* img, FillGeom& geo) {
   a.cap = cap;
   a.ww = ww;
   a.err = g.errFlag;
+  a.touched = g.touched;
   int grid = nRows < 8192 ? nRows : 8192;
   size_t lds = fixedBytes;
   if (fixedBytes + capBytes <= g.maxLds) {
@@ -185,8 +224,11 @@ int raster_fill(pxh_image* img, FillGeom& geo) {
     if (int e = g.gscratch.ensure((size_t)grid * capBytes)) return fail(e, "scratch allocation failed");
     a.gscratch = g.gscratch.p;
   }
-  if (launch_fill_rows(a, grid, lds, img->fmt == PXH_FORMAT_A8, g.stream) != 0)
-    return fail(kErrDevice, std::string("fill launch: ") + hipGetErrorString(hipGetLastError()));
+  {
+    LaunchTimer t;
+    if (launch_fill_rows(a, grid, lds, img->fmt == PXH_FORMAT_A8, g.stream) != 0)
+      return fail(kErrDevice, std::string("fill launch: ") + hipGetErrorString(hipGetLastError()));
+  }
   PXH_HIP(hipEventRecord(g.stageFree, g.stream));
   return 0;
 }
@@ -550,6 +592,61 @@ int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_
     return fail(kErrDevice, "blend launch failed");
   PXH_HIP(hipMemcpyAsync(out, d + 2 * n, bytes, hipMemcpyDeviceToHost, g.stream));
   PXH_HIP(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+// Reads (and optionally resets) the count of pixels written by fills since
+// the last reset (N_touch, the unit of the fill throughput metric).
+int pxh_touched_pixels(int64_t* out, int32_t reset) {
+  PXH_INIT();
+  unsigned long long v = 0;
+  PXH_HIP(hipMemcpyAsync(&v, g.touched, sizeof(v), hipMemcpyDeviceToHost, g.stream));
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  *out = (int64_t)v;
+  if (reset) PXH_HIP(hipMemsetAsync(g.touched, 0, sizeof(unsigned long long), g.stream));
+  return 0;
+}
+
+// Per-launch timing of the raster kernels: between begin and end every fill
+// kernel launch is bracketed by events; end returns the summed kernel time.
+int pxh_profile_begin(void) {
+  PXH_INIT();
+  g.profiling = true;
+  return 0;
+}
+
+int pxh_profile_end(double* kernel_ms, int64_t* launches) {
+  PXH_INIT();
+  g.profiling = false;
+  double tot = 0;
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  for (auto& pr : g.evPairs) {
+    float ms = 0;
+    PXH_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    tot += ms;
+    g.evPool.push_back(pr.first);
+    g.evPool.push_back(pr.second);
+  }
+  *kernel_ms = tot;
+  *launches = (int64_t)g.evPairs.size();
+  g.evPairs.clear();
+  return 0;
+}
+
+// Records event slot (0..15) on the library stream.
+int pxh_event_record(int32_t slot) {
+  PXH_INIT();
+  if (slot < 0 || slot >= 16) return fail(kErrArg, "event slot out of range");
+  PXH_HIP(hipEventRecord(g.ev[slot], g.stream));
+  return 0;
+}
+
+// Milliseconds between two recorded event slots (waits for the second).
+int pxh_event_elapsed(int32_t a, int32_t b, float* ms) {
+  PXH_INIT();
+  if (a < 0 || a >= 16 || b < 0 || b >= 16) return fail(kErrArg, "event slot out of range");
+  PXH_HIP(hipEventSynchronize(g.ev[b]));
+  PXH_HIP(hipEventElapsedTime(ms, g.ev[a], g.ev[b]));
   return 0;
 }
 

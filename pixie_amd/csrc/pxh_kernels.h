@@ -21,6 +21,7 @@ struct FillArgs {
   int32_t capInLds;      // cap arrays in LDS (else gscratch)
   char* gscratch;        // per-workgroup cap arrays when not in LDS
   int32_t* err;          // set to 1 when a band exceeds cap
+  unsigned long long* touched;  // pixels written (N_touch), may be null
 };
 
 struct CompositeArgs {

@@ -32,6 +32,7 @@ __global__ void __launch_bounds__(64) fill_rows_kernel(FillArgs a) {
   const bool mask = F.mode == kMask;
   const int32_t W = F.canvasW;
   const int32_t lo = mask ? 0 : F.xlo, hi = mask ? W : F.xhi;
+  uint32_t ntouch = 0;
   for (int32_t r = blockIdx.x; r < a.nRows; r += gridDim.x) {
     const int32_t y = a.rowBase + r;
     for (int32_t wx = lo & ~3; wx < hi; wx += a.ww) {
@@ -52,10 +53,15 @@ __global__ void __launch_bounds__(64) fill_rows_kernel(FillArgs a) {
       }
       if (kind == kRowNone) break;
       if (kA8)
-        row_epilogue_a8(C, L, kind, (uint8_t*)a.canvas, W, F.rgbx, F.mode);
+        ntouch += row_epilogue_a8(C, L, kind, (uint8_t*)a.canvas, W, F.rgbx, F.mode);
       else
-        row_epilogue(C, L, kind, (uint32_t*)a.canvas, W, F.rgbx, F.mode);
+        ntouch += row_epilogue(C, L, kind, (uint32_t*)a.canvas, W, F.rgbx, F.mode);
     }
+  }
+  if (a.touched) {
+    // wave reduction, one atomic per workgroup
+    for (int o = 32; o > 0; o >>= 1) ntouch += __shfl_xor(ntouch, o, kWave);
+    if (threadIdx.x == 0 && ntouch) atomicAdd(a.touched, (unsigned long long)ntouch);
   }
 }
 

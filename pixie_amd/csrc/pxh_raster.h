@@ -562,13 +562,14 @@ __device__ __forceinline__ uint32_t apply_code(int32_t kind, int32_t code, uint3
 
 // Scans L.D over the window and applies the codes to canvas row `row`.
 // Leaves L.D zeroed for the next row.  Canvas rows are W pixels wide.
-__device__ void row_epilogue(const RowCtx& C, RowLds& L, int32_t kind, uint32_t* canvas,
-                             int32_t W, uint32_t rgbx, int32_t mode) {
+__device__ uint32_t row_epilogue(const RowCtx& C, RowLds& L, int32_t kind, uint32_t* canvas,
+                                 int32_t W, uint32_t rgbx, int32_t mode) {
   const int lane = lane_id();
   const int32_t wlen = C.wx1 - C.wx0;
   uint32_t* rowp = canvas + (size_t)C.y * (size_t)W;
   const bool vec_ok = ((W & 3) == 0) && ((C.wx0 & 3) == 0);
   int32_t carry = 0;
+  uint32_t ntouch = 0;
   for (int base = 0; base < wlen; base += 4 * kWave) {
     int i0 = base + 4 * lane;
     int32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
@@ -608,7 +609,10 @@ __device__ void row_epilogue(const RowCtx& C, RowLds& L, int32_t kind, uint32_t*
     for (int k = 0; k < 4; k++) {
       if (i0 + k < wlen) {
         uint32_t ax = (code[k] >= kArea && code[k] < kClear && kind == kRowOps) ? L.AX[i0 + k] : 0u;
-        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &touched);
+        bool t = false;
+        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &t);
+        ntouch += t ? 1u : 0u;
+        touched |= t;
       }
     }
     if (!touched) continue;
@@ -626,17 +630,19 @@ __device__ void row_epilogue(const RowCtx& C, RowLds& L, int32_t kind, uint32_t*
     if (idx < L.ww + 4) L.D[idx] = 0;
   }
   wave_sync();
+  return ntouch;
 }
 
 // Epilogue for an A8 target (masks, glyph atlas): pixel v reads as
 // rgbx(v,v,v,v) and the alpha of the result is stored.
-__device__ void row_epilogue_a8(const RowCtx& C, RowLds& L, int32_t kind, uint8_t* canvas,
-                                int32_t W, uint32_t rgbx, int32_t mode) {
+__device__ uint32_t row_epilogue_a8(const RowCtx& C, RowLds& L, int32_t kind, uint8_t* canvas,
+                                    int32_t W, uint32_t rgbx, int32_t mode) {
   const int lane = lane_id();
   const int32_t wlen = C.wx1 - C.wx0;
   uint8_t* rowp = canvas + (size_t)C.y * (size_t)W;
   const bool vec_ok = ((W & 3) == 0) && ((C.wx0 & 3) == 0);
   int32_t carry = 0;
+  uint32_t ntouch = 0;
   for (int base = 0; base < wlen; base += 4 * kWave) {
     int i0 = base + 4 * lane;
     int32_t d0 = 0, d1 = 0, d2 = 0, d3 = 0;
@@ -676,7 +682,10 @@ __device__ void row_epilogue_a8(const RowCtx& C, RowLds& L, int32_t kind, uint8_
     for (int k = 0; k < 4; k++) {
       if (i0 + k < wlen) {
         uint32_t ax = (code[k] >= kArea && code[k] < kClear && kind == kRowOps) ? L.AX[i0 + k] : 0u;
-        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &touched);
+        bool t = false;
+        px[k] = apply_code(kind, code[k], ax, px[k], rgbx, mode, &t);
+        ntouch += t ? 1u : 0u;
+        touched |= t;
       }
     }
     if (!touched) continue;
@@ -693,6 +702,7 @@ __device__ void row_epilogue_a8(const RowCtx& C, RowLds& L, int32_t kind, uint8_
     if (idx < L.ww + 4) L.D[idx] = 0;
   }
   wave_sync();
+  return ntouch;
 }
 
 }  // namespace pxh

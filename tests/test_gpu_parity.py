@@ -159,16 +159,20 @@ def test_mixed_scene_small(gpu, orc, seed):
     cmds, canvas = WL.mixed_scene(seed=seed, n_paths=60, W=512, H=512)
     want = canvas.copy()
     img = px.Image.from_array(canvas)
+    px.touched_pixels(reset=True)
+    touched = 0
     for c in cmds:
         if c.stroke:
-            orc.stroke_path(want, c.cmds, c.color, width=c.width, cap=c.cap, join=c.join,
-                            miter_limit=c.miter, mode=c.mode)
+            touched += orc.stroke_path(want, c.cmds, c.color, width=c.width, cap=c.cap,
+                                       join=c.join, miter_limit=c.miter, mode=c.mode)["touched"]
             img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
                            miterLimit=c.miter, blendMode=c.mode)
         else:
-            orc.fill_path(want, c.cmds, c.color, winding=c.winding, mode=c.mode)
+            touched += orc.fill_path(want, c.cmds, c.color, winding=c.winding, mode=c.mode)["touched"]
             img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
     _cov_check(img.download(), want, "mixed seed %d" % seed)
+    # the GPU's N_touch counter (the bench's algorithmic-byte unit) matches
+    assert px.touched_pixels(reset=True) == touched
 
 
 def test_glyphs_into_a8(gpu, orc):
@@ -189,7 +193,7 @@ def test_glyphs_into_a8(gpu, orc):
 @pytest.mark.parametrize("mode", range(20))
 @pytest.mark.parametrize("dx,dy", [(0, 0), (13, -7), (-40, 25), (90, 90), (-200, 0)])
 def test_draw_blend_rect(gpu, orc, mode, dx, dy):
-    rng = np.random.default_rng(mode * 7 + dx)
+    rng = np.random.default_rng(mode * 7 + abs(dx) * 3 + abs(dy))
     a = WL.random_premultiplied_image(rng, 100, 80)
     b = WL.random_premultiplied_image(rng, 60, 70)
     want = a.copy()
