@@ -91,6 +91,38 @@ int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const floa
 int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, int32_t npolys,
                     uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
 
+/* Scenes (batched form) ------------------------------------------------- */
+/* A scene records an ordered sequence of fillPath / strokePath calls
+ * (paths.nim:2093-2113 / 2144-2176; solid paints, no dashes, no elliptical
+ * arc commands) and renders them in one device pass.  The result is the image
+ * the same calls leave when issued one by one in order.  Pixie has no batched
+ * entry point; this one replaces a loop over fillPath / strokePath on one
+ * image.  Calls that draw nothing (transparent paint that is not
+ * OverwriteBlend, stroke width <= 0) are not recorded.  The recording stays
+ * resident on the device across renders; scene pools are sized on the first
+ * render at a given image size (which waits for the device once) and reused
+ * afterwards without host synchronisation. */
+typedef struct pxh_scene pxh_scene;
+pxh_scene* pxh_scene_new(void);
+void pxh_scene_free(pxh_scene* scene);
+int pxh_scene_clear(pxh_scene* scene);
+int32_t pxh_scene_count(const pxh_scene* scene); /* recorded calls */
+int pxh_scene_fill_path(pxh_scene* scene, const float* cmds, int32_t ncmds, const float* mat9,
+                        uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
+int pxh_scene_stroke_path(pxh_scene* scene, const float* cmds, int32_t ncmds, const float* mat9,
+                          uint32_t rgbx, float stroke_width, int32_t line_cap,
+                          int32_t line_join, float miter_limit, int32_t blend_mode);
+/* Renders the recorded calls onto img (queued on the library's stream). */
+int pxh_scene_render(pxh_scene* scene, pxh_image* img);
+/* Per-call status of the last render: 0, or the PXH_ERR_* code the call
+ * would have raised (e.g. PXH_ERR_INT_OVERFLOW); such calls draw nothing. */
+int pxh_scene_status(pxh_scene* scene, int32_t* status, int32_t n);
+/* Device geometry of recorded call i after a render, in the layout of
+ * pxh_debug_fill_geometry (test access). */
+int pxh_scene_debug_geometry(pxh_scene* scene, int32_t i, int32_t* info16, int32_t** parts4,
+                             int32_t* nparts, float** entries4, int8_t** windings,
+                             int32_t* nentries);
+
 /* Image operations ------------------------------------------------------ */
 /* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.
  * blendRect (images.nim:468-529, reached from draw images.nim:636-678). */

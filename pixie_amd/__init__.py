@@ -89,6 +89,16 @@ def lib():
         L.pxh_debug_fill_geometry.argtypes = [f32p, i32, f32p, i32, i32, i32, f, i32, i32, f,
                                               i32p, P(i32p), i32p, P(f32p),
                                               P(P(ctypes.c_int8)), i32p]
+        L.pxh_scene_new.restype = vp
+        L.pxh_scene_free.argtypes = [vp]
+        L.pxh_scene_clear.argtypes = [vp]
+        L.pxh_scene_count.argtypes = [vp]
+        L.pxh_scene_fill_path.argtypes = [vp, f32p, i32, f32p, u32, i32, i32]
+        L.pxh_scene_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, i32]
+        L.pxh_scene_render.argtypes = [vp, vp]
+        L.pxh_scene_status.argtypes = [vp, i32p, i32]
+        L.pxh_scene_debug_geometry.argtypes = [vp, i32, i32p, P(i32p), i32p, P(f32p),
+                                               P(P(ctypes.c_int8)), i32p]
         _lib = L
     return _lib
 
@@ -234,6 +244,71 @@ class Image:
 
     def applyOpacity(self, opacity):
         _check(lib().pxh_apply_opacity(self._h, float(opacity)))
+
+
+class Scene:
+    """An ordered recording of fillPath / strokePath calls rendered in one
+    device pass (the batched form of paths.nim:2093-2176).  Rendering a scene
+    onto an image leaves the pixels the same calls leave when issued one by
+    one on that image."""
+
+    def __init__(self):
+        h = lib().pxh_scene_new()
+        if not h:
+            raise PixieError(7, "scene allocation failed")
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.pxh_scene_free(h)
+            self._h = None
+
+    def __len__(self):
+        return lib().pxh_scene_count(self._h)
+
+    def clear(self):
+        _check(lib().pxh_scene_clear(self._h))
+
+    def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
+        c = _cmds(path)
+        m = _mat(transform)
+        _check(lib().pxh_scene_fill_path(self._h, _fp(c), len(c), _fp(m), pack(color),
+                                         windingRule, blendMode))
+
+    def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
+                   lineJoin=MiterJoin, miterLimit=4.0, blendMode=0):
+        c = _cmds(path)
+        m = _mat(transform)
+        _check(lib().pxh_scene_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color),
+                                           strokeWidth, lineCap, lineJoin, miterLimit,
+                                           blendMode))
+
+    def render(self, image):
+        _check(lib().pxh_scene_render(self._h, image.handle))
+
+    def status(self):
+        n = len(self)
+        out = np.zeros(max(n, 1), np.int32)
+        _check(lib().pxh_scene_status(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      n))
+        return out[:n]
+
+    def debug_geometry(self, i):
+        """Device geometry of call i: (info[16], parts[n,4], entries[m,4], windings[m])."""
+        info = (ctypes.c_int32 * 16)()
+        P = ctypes.POINTER
+        parts, ents, wind = P(ctypes.c_int32)(), P(ctypes.c_float)(), P(ctypes.c_int8)()
+        np_, ne = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib().pxh_scene_debug_geometry(self._h, i, info, ctypes.byref(parts),
+                                              ctypes.byref(np_), ctypes.byref(ents),
+                                              ctypes.byref(wind), ctypes.byref(ne)))
+        pa = np.ctypeslib.as_array(parts, shape=(max(np_.value, 1) * 4,)).copy()[:np_.value * 4]
+        en = np.ctypeslib.as_array(ents, shape=(max(ne.value, 1) * 4,)).copy()[:ne.value * 4]
+        wd = np.ctypeslib.as_array(wind, shape=(max(ne.value, 1),)).copy()[:ne.value]
+        for p in (parts, ents, wind):
+            lib().pxh_free(p)
+        return np.array(info[:], np.int32), pa.reshape(-1, 4), en.reshape(-1, 4), wd
 
 
 def touched_pixels(reset=True):

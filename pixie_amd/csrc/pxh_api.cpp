@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <string>
 #include <vector>
@@ -14,11 +15,29 @@
 #include "pxh_host_geom.h"
 #include "pxh_kernels.h"
 #include "pxh_pixel.h"
+#include "pxh_scene.h"
 
 struct pxh_image {
   int32_t w, h, fmt;
   void* dev;
   size_t bytes;
+};
+
+struct pxh_scene {
+  std::vector<pxh::SceneItem> items;
+  std::vector<pxh::SceneCmd> cmds;
+  std::vector<pxh::ScenePoly> polys;
+  bool uploaded = false;
+  int32_t validW = -1, validH = -1;  // target size the pools were validated for
+  // device copies of the recording
+  pxh::SceneItem* dItems = nullptr;
+  pxh::SceneCmd* dCmds = nullptr;
+  pxh::ScenePoly* dPolys = nullptr;
+  // per-render buffers
+  char* dMem = nullptr;
+  size_t dMemBytes = 0;
+  pxh::ScenePools pools{};
+  int32_t ptCap = 0, segCap = 0, bandCap = 0, entryCap = 0, taskCap = 0, runCap = 0;
 };
 
 namespace pxh {
@@ -102,6 +121,7 @@ int init() {
   for (auto& e : g.ev) PXH_HIP(hipEventCreate(&e));
   if (set_fill_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(fill)");
   if (set_blur_lds_limit(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(blur)");
+  if (scene_set_lds_limits(g.maxLds) != 0) return fail(kErrDevice, "hipFuncSetAttribute(scene)");
   g.inited = true;
   return 0;
 }
@@ -299,6 +319,276 @@ int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius
   BlurArgs ax{src_dst, (uint32_t*)g.tmp.p, lut, w, h, radius, oob, 0, 0};
   BlurArgs ay{(uint32_t*)g.tmp.p, out, lut, w, h, radius, oob, shadowColor ? 1 : 0, color};
   if (launch_blur(ax, ay, g.stream) != 0) return fail(kErrDevice, "blur launch failed");
+  return 0;
+}
+
+
+// ---- scene (batched form) ----
+
+const int kSceneGcap = 8192;  // band entries a wave can hold in global scratch
+const int kSceneLdsCap = 256; // band entries held in LDS
+
+// Converts pixie commands to the device's canonical absolute form
+// (commandsToShapes' command semantics, paths.nim:886-1057): relative,
+// horizontal / vertical and smooth commands are resolved, every subpath
+// starts with an explicit Move, and each command carries its subpath start.
+int canonicalize(const float* c, int32_t n, std::vector<SceneCmd>* out,
+                 std::vector<ScenePoly>* polys) {
+  const int32_t polyBase0 = (int32_t)polys->size();  // first subpath of this call
+  V2 start = v2(0, 0), at = v2(0, 0), prevCtrl = v2(0, 0), prevCtrl2 = v2(0, 0);
+  int32_t prevKind = 1;  // Move
+  bool open = false;     // a subpath (with its Move) has been started
+  auto push = [&](int kind, std::initializer_list<float> p) {
+    SceneCmd sc = {};
+    sc.kind = (float)kind;
+    int k = 0;
+    for (float v : p) sc.p[k++] = v;
+    sc.sx = start.x;
+    sc.sy = start.y;
+    sc.pad[0] = (int32_t)polys->size() - 1 - polyBase0;  // subpath index within the call
+    out->push_back(sc);
+    polys->back().numCmds++;
+  };
+  auto begin = [&](V2 s, bool prevNoClose) {
+    if (open && prevNoClose) polys->back().noClose = 1;
+    start = s;
+    ScenePoly sp = {(int32_t)out->size(), 0, 0, 0};
+    polys->push_back(sp);
+    open = true;
+    push(1, {s.x, s.y});
+  };
+  auto ensure_open = [&]() {
+    if (!open) begin(at, false);
+  };
+  auto is_cubic = [](int k) { return k == 5 || k == 6 || k == 14 || k == 15; };
+  auto is_quad = [](int k) { return k == 7 || k == 8 || k == 16 || k == 17; };
+  int32_t i = 0;
+  while (i < n) {
+    int32_t kind = (int32_t)c[i];
+    i++;
+    static const int pc[19] = {0, 2, 2, 1, 1, 6, 4, 4, 2, 7, 2, 2, 1, 1, 6, 4, 4, 2, 7};
+    if (kind < 0 || kind > 18 || i + pc[kind] > n) return fail(kErrArg, "Invalid path command");
+    const float* a = c + i;
+    switch (kind) {
+      case 1:  // Move
+        at = v2(a[0], a[1]);
+        begin(at, false);
+        break;
+      case 10:  // RMove: ends the subpath without closing it (paths.nim:968-974)
+        at = v2(at.x + a[0], at.y + a[1]);
+        begin(at, true);
+        break;
+      case 2: case 3: case 4: case 11: case 12: case 13: {
+        V2 to;
+        if (kind == 2) to = v2(a[0], a[1]);
+        else if (kind == 3) to = v2(a[0], at.y);
+        else if (kind == 4) to = v2(at.x, a[0]);
+        else if (kind == 11) to = v2(at.x + a[0], at.y + a[1]);
+        else if (kind == 12) to = v2(at.x + a[0], at.y);
+        else to = v2(at.x, at.y + a[0]);
+        ensure_open();
+        push(2, {to.x, to.y});
+        at = to;
+      } break;
+      case 5: case 6: case 14: case 15: {
+        V2 c1, c2, to;
+        if (kind == 5) {
+          c1 = v2(a[0], a[1]); c2 = v2(a[2], a[3]); to = v2(a[4], a[5]);
+        } else if (kind == 14) {
+          c1 = v2(at.x + a[0], at.y + a[1]); c2 = v2(at.x + a[2], at.y + a[3]);
+          to = v2(at.x + a[4], at.y + a[5]);
+        } else if (kind == 6) {
+          c2 = v2(a[0], a[1]); to = v2(a[2], a[3]);
+          c1 = is_cubic(prevKind) ? at * 2.0f - prevCtrl2 : at;
+        } else {
+          c2 = v2(at.x + a[0], at.y + a[1]); to = v2(at.x + a[2], at.y + a[3]);
+          c1 = is_cubic(prevKind) ? at * 2.0f - prevCtrl2 : at;
+        }
+        ensure_open();
+        push(5, {c1.x, c1.y, c2.x, c2.y, to.x, to.y});
+        at = to;
+        prevCtrl2 = c2;
+      } break;
+      case 7: case 8: case 16: case 17: {
+        V2 cc, to;
+        if (kind == 7) {
+          cc = v2(a[0], a[1]); to = v2(a[2], a[3]);
+        } else if (kind == 16) {
+          cc = v2(at.x + a[0], at.y + a[1]); to = v2(at.x + a[2], at.y + a[3]);
+        } else if (kind == 8) {
+          to = v2(a[0], a[1]);
+          cc = is_quad(prevKind) ? at * 2.0f - prevCtrl : at;
+        } else {
+          to = v2(at.x + a[0], at.y + a[1]);
+          cc = is_quad(prevKind) ? at * 2.0f - prevCtrl : at;
+        }
+        ensure_open();
+        push(7, {cc.x, cc.y, to.x, to.y});
+        at = to;
+        prevCtrl = cc;
+      } break;
+      case 0:  // Close
+        ensure_open();
+        push(0, {});
+        at = start;
+        open = false;
+        break;
+      default:
+        return fail(kErrArg, "elliptical arc commands are only supported by the per-call form");
+    }
+    i += pc[kind];
+    prevKind = kind;
+  }
+  return 0;
+}
+
+int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9, uint32_t rgbx,
+                 int32_t rule, int32_t mode, bool stroke, float width, int32_t cap, int32_t join,
+                 float miterLimit) {
+  if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (mode < 0 || mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (rule != kNonZero && rule != kEvenOdd) return fail(kErrArg, "invalid winding rule");
+  if (!mat9) mat9 = kIdentity;
+  if (!(ch_a(rgbx) > 0 || mode == kOverwrite)) return 0;  // paths.nim:2107 / 2162
+  if (stroke && width <= 0.0f) return 0;  // strokeShapes yields nothing (paths.nim:1931-1932)
+  SceneItem it = {};
+  it.cmdBase = (int32_t)sc->cmds.size();
+  it.polyBase = (int32_t)sc->polys.size();
+  size_t polys0 = sc->polys.size(), cmds0 = sc->cmds.size();
+  if (int r = canonicalize(cmds, n, &sc->cmds, &sc->polys)) {
+    sc->cmds.resize(cmds0);
+    sc->polys.resize(polys0);
+    return r;
+  }
+  it.cmdCount = (int32_t)sc->cmds.size() - it.cmdBase;
+  it.polyCount = (int32_t)sc->polys.size() - it.polyBase;
+  it.stroke = stroke ? 1 : 0;
+  it.rgbx = rgbx;
+  it.mode = mode;
+  it.rule = rule;
+  it.cap = cap;
+  it.join = join;
+  float ps = pixel_scale(mat9);
+  it.halfStroke = width / 2.0f;
+  it.miterAngleLimit = asinf(1.0f / miterLimit) * 2.0f;
+  it.minArea = 0.2f / ps;
+  it.errSq = powf(0.2f / ps, 2.0f);
+  memcpy(it.mat, mat9, sizeof(it.mat));
+  it.identity = mat_is_identity(mat9) ? 1 : 0;
+  it.strokeNone = (stroke && width <= 0.0f) ? 1 : 0;
+  sc->items.push_back(it);
+  sc->uploaded = false;
+  sc->validW = sc->validH = -1;
+  return 0;
+}
+
+void scene_free_device(pxh_scene* sc) {
+  if (sc->dItems) (void)hipFree(sc->dItems);
+  if (sc->dCmds) (void)hipFree(sc->dCmds);
+  if (sc->dPolys) (void)hipFree(sc->dPolys);
+  if (sc->dMem) (void)hipFree(sc->dMem);
+  sc->dItems = nullptr;
+  sc->dCmds = nullptr;
+  sc->dPolys = nullptr;
+  sc->dMem = nullptr;
+  sc->dMemBytes = 0;
+}
+
+// (Re)allocates the per-render pools of a scene in one device block.
+int scene_alloc(pxh_scene* sc) {
+  const size_t nI = sc->items.size(), nC = sc->cmds.size(), nP = sc->polys.size();
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  size_t oGeo = carve(nI * sizeof(ItemGeom)), oFill = carve(nI * sizeof(FillInfo));
+  size_t oCC = carve(nC * 4), oCO = carve(nC * 4), oPP = carve(nP * 4), oPN = carve(nP * 4);
+  size_t oPts = carve((size_t)sc->ptCap * 8), oPtP = carve((size_t)sc->ptCap * 4);
+  size_t oSeg = carve((size_t)sc->segCap * 16), oSegW = carve((size_t)sc->segCap);
+  size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
+  size_t oEnt = carve((size_t)sc->entryCap * sizeof(Entry)), oEW = carve((size_t)sc->entryCap);
+  size_t oTask = carve((size_t)sc->taskCap * sizeof(RowRuns)), oRun = carve((size_t)sc->runCap * 8);
+  size_t oCnt = carve(kNumCounters * 4);
+  const int gridRows = g.numCUs * 8;
+  size_t oScr = carve(scene_gscratch_bytes(gridRows, kSceneGcap));
+  if (off > sc->dMemBytes) {
+    PXH_HIP(hipStreamSynchronize(g.stream));
+    if (sc->dMem) PXH_HIP(hipFree(sc->dMem));
+    sc->dMem = nullptr;
+    if (hipMalloc((void**)&sc->dMem, off) != hipSuccess) return fail(kErrNoMem, "scene pool allocation failed");
+    sc->dMemBytes = off;
+  }
+  char* b = sc->dMem;
+  ScenePools& p = sc->pools;
+  p.items = sc->dItems;
+  p.cmds = sc->dCmds;
+  p.polys = sc->dPolys;
+  p.nItems = (int32_t)nI;
+  p.geo = (ItemGeom*)(b + oGeo);
+  p.fills = (FillInfo*)(b + oFill);
+  p.cmdCount = (int32_t*)(b + oCC);
+  p.cmdOff = (int32_t*)(b + oCO);
+  p.polyPt = (int32_t*)(b + oPP);
+  p.polyN = (int32_t*)(b + oPN);
+  p.pts = (float2*)(b + oPts);
+  p.ptPoly = (int32_t*)(b + oPtP);
+  p.ptCap = sc->ptCap;
+  p.segs = (float4*)(b + oSeg);
+  p.segW = (int8_t*)(b + oSegW);
+  p.segCap = sc->segCap;
+  p.parts = (PartInfo*)(b + oParts);
+  p.bandItem = (int32_t*)(b + oBI);
+  p.bandCap = sc->bandCap;
+  p.entries = (Entry*)(b + oEnt);
+  p.entW = (int8_t*)(b + oEW);
+  p.entryCap = sc->entryCap;
+  p.tasks = (RowRuns*)(b + oTask);
+  p.taskCap = sc->taskCap;
+  p.runs = (int2*)(b + oRun);
+  p.runCap = sc->runCap;
+  p.counters = (int32_t*)(b + oCnt);
+  p.gscratch = b + oScr;
+  p.gcap = kSceneGcap;
+  p.ldsCap = kSceneLdsCap;
+  return 0;
+}
+
+int scene_upload(pxh_scene* sc) {
+  if (sc->uploaded) return 0;
+  PXH_HIP(hipStreamSynchronize(g.stream));  // a previous render may still read the buffers
+  scene_free_device(sc);
+  auto up = [&](void** d, const void* h, size_t bytes) -> int {
+    if (hipMalloc(d, bytes ? bytes : 16) != hipSuccess) return kErrNoMem;
+    if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return kErrDevice;
+    return 0;
+  };
+  if (up((void**)&sc->dItems, sc->items.data(), sc->items.size() * sizeof(SceneItem)) ||
+      up((void**)&sc->dCmds, sc->cmds.data(), sc->cmds.size() * sizeof(SceneCmd)) ||
+      up((void**)&sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly)))
+    return fail(kErrNoMem, "scene upload failed");
+  // first estimates of the pools; a render that overflows grows them
+  size_t nC = sc->cmds.size();
+  sc->ptCap = (int32_t)std::min<size_t>(64 * nC + 4096, 1u << 28);
+  sc->segCap = sc->ptCap * 2;
+  sc->bandCap = (int32_t)std::min<size_t>(64 * sc->items.size() + 4096, 1u << 26);
+  sc->entryCap = sc->segCap * 2;
+  sc->taskCap = (int32_t)std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28);
+  sc->runCap = sc->taskCap * 16;
+  sc->uploaded = true;
+  return scene_alloc(sc);
+}
+
+int scene_launch(pxh_scene* sc, pxh_image* img) {
+  ScenePools& p = sc->pools;
+  p.W = img->w;
+  p.H = img->h;
+  PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, g.stream));
+  if (scene_launch_geometry(p, g.stream) || scene_launch_bands(p, g.numCUs * 16, g.stream) ||
+      scene_launch_rows(p, g.numCUs * 8, g.stream) ||
+      scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched, g.numCUs * 16, g.stream))
+    return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
@@ -592,6 +882,128 @@ int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_
     return fail(kErrDevice, "blend launch failed");
   PXH_HIP(hipMemcpyAsync(out, d + 2 * n, bytes, hipMemcpyDeviceToHost, g.stream));
   PXH_HIP(hipStreamSynchronize(g.stream));
+  return 0;
+}
+
+pxh_scene* pxh_scene_new(void) { return new pxh_scene(); }
+
+void pxh_scene_free(pxh_scene* sc) {
+  if (!sc) return;
+  if (g.inited) (void)hipStreamSynchronize(g.stream);
+  scene_free_device(sc);
+  delete sc;
+}
+
+int pxh_scene_clear(pxh_scene* sc) {
+  if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (g.inited) PXH_HIP(hipStreamSynchronize(g.stream));
+  sc->items.clear();
+  sc->cmds.clear();
+  sc->polys.clear();
+  sc->uploaded = false;
+  sc->validW = sc->validH = -1;
+  return 0;
+}
+
+int32_t pxh_scene_count(const pxh_scene* sc) { return sc ? (int32_t)sc->items.size() : 0; }
+
+int pxh_scene_fill_path(pxh_scene* sc, const float* cmds, int32_t ncmds, const float* mat9,
+                        uint32_t rgbx, int32_t winding_rule, int32_t blend_mode) {
+  return scene_record(sc, cmds, ncmds, mat9, rgbx, winding_rule, blend_mode, false, 0, 0, 0, 4.0f);
+}
+
+int pxh_scene_stroke_path(pxh_scene* sc, const float* cmds, int32_t ncmds, const float* mat9,
+                          uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
+                          float miter_limit, int32_t blend_mode) {
+  return scene_record(sc, cmds, ncmds, mat9, rgbx, kNonZero, blend_mode, true, stroke_width,
+                      line_cap, line_join, miter_limit);
+}
+
+int pxh_scene_render(pxh_scene* sc, pxh_image* img) {
+  PXH_INIT();
+  if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (int r = check_image(img)) return r;
+  if (sc->items.empty()) return 0;
+  if (int r = scene_upload(sc)) return r;
+  if (sc->validW == img->w && sc->validH == img->h) return scene_launch(sc, img);
+  // First render at this size: check the pools and grow them until the render
+  // fits.  A render that overflows a pool stops before touching the canvas.
+  for (int attempt = 0; attempt < 8; attempt++) {
+    if (int r = scene_launch(sc, img)) return r;
+    int32_t cnt[kNumCounters];
+    PXH_HIP(hipMemcpyAsync(cnt, sc->pools.counters, sizeof(cnt), hipMemcpyDeviceToHost, g.stream));
+    PXH_HIP(hipStreamSynchronize(g.stream));
+    if (!cnt[kCntError]) {
+      sc->validW = img->w;
+      sc->validH = img->h;
+      return 0;
+    }
+    if (cnt[kCntError] & 8) return fail(kErrCapacity, "a band holds more segments than the row scratch");
+    auto grow = [](int32_t cap, int64_t need) {
+      int64_t c = std::max<int64_t>((int64_t)cap * 2, need + need / 4 + 1024);
+      return (int32_t)std::min<int64_t>(c, (int64_t)1 << 30);
+    };
+    sc->ptCap = grow(sc->ptCap, cnt[kCntPoints]);
+    sc->segCap = grow(sc->segCap, cnt[kCntSegs]);
+    sc->bandCap = grow(sc->bandCap, cnt[kCntBands]);
+    sc->entryCap = grow(sc->entryCap, cnt[kCntEntries]);
+    sc->taskCap = grow(sc->taskCap, cnt[kCntTasks]);
+    sc->runCap = grow(sc->runCap, cnt[kCntRuns]);
+    if (int r = scene_alloc(sc)) return r;
+  }
+  return fail(kErrCapacity, "scene pools could not be sized");
+}
+
+// Per-call status of the last render (0 ok, else the PixieError code of that
+// call, e.g. PXH_ERR_INT_OVERFLOW); waits for the render.
+int pxh_scene_status(pxh_scene* sc, int32_t* status, int32_t n) {
+  PXH_INIT();
+  if (!sc || !sc->uploaded) return fail(kErrArg, "scene not rendered");
+  int32_t m = std::min<int32_t>(n, (int32_t)sc->items.size());
+  std::vector<ItemGeom> geo(m);
+  PXH_HIP(hipMemcpyAsync(geo.data(), sc->pools.geo, m * sizeof(ItemGeom), hipMemcpyDeviceToHost, g.stream));
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  for (int32_t i = 0; i < m; i++) status[i] = geo[i].status;
+  return 0;
+}
+
+// Device geometry of call i of the last render, in the layout of
+// pxh_debug_fill_geometry (for tests against the host geometry).
+int pxh_scene_debug_geometry(pxh_scene* sc, int32_t i, int32_t* info16, int32_t** parts4,
+                             int32_t* nparts, float** entries4, int8_t** windings,
+                             int32_t* nentries) {
+  PXH_INIT();
+  if (!sc || !sc->uploaded || i < 0 || i >= (int32_t)sc->items.size())
+    return fail(kErrArg, "invalid scene call");
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  FillInfo f;
+  ItemGeom geo;
+  PXH_HIP(hipMemcpy(&f, sc->pools.fills + i, sizeof(f), hipMemcpyDeviceToHost));
+  PXH_HIP(hipMemcpy(&geo, sc->pools.geo + i, sizeof(geo), hipMemcpyDeviceToHost));
+  memcpy(info16, &f, sizeof(FillInfo));
+  int32_t np = (f.status == kOk && f.pathWidth > 0) ? f.nParts : 0;
+  std::vector<PartInfo> parts(np);
+  if (np) PXH_HIP(hipMemcpy(parts.data(), sc->pools.parts + geo.bandBase, np * sizeof(PartInfo),
+                            hipMemcpyDeviceToHost));
+  int32_t ne = 0;
+  for (auto& p : parts) ne += p.count;
+  *parts4 = (int32_t*)malloc(sizeof(PartInfo) * (np + 1));
+  *entries4 = (float*)malloc(sizeof(Entry) * (ne + 1));
+  *windings = (int8_t*)malloc(ne + 1);
+  int32_t k = 0;
+  for (int32_t b = 0; b < np; b++) {
+    PartInfo q = parts[b];
+    if (q.count) {
+      PXH_HIP(hipMemcpy(*entries4 + 4 * k, sc->pools.entries + q.entryBase, q.count * sizeof(Entry),
+                        hipMemcpyDeviceToHost));
+      PXH_HIP(hipMemcpy(*windings + k, sc->pools.entW + q.entryBase, q.count, hipMemcpyDeviceToHost));
+    }
+    parts[b].entryBase = k;
+    k += q.count;
+  }
+  if (np) memcpy(*parts4, parts.data(), np * sizeof(PartInfo));
+  *nparts = np;
+  *nentries = ne;
   return 0;
 }
 

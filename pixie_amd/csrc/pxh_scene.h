@@ -1,0 +1,130 @@
+// pxh_scene.h -- device layout of a recorded scene (batched form) and the
+// launchers of its kernels (pxh_scene.hip).
+//
+// A scene is an ordered list of fillPath / strokePath calls whose canonical
+// commands stay resident in HBM.  Each render runs on the device:
+//   geometry  one workgroup per call: adaptive flattening (count -> scan ->
+//             emit), stroke expansion, shapesToSegments, bounds, band layout
+//   bands     one wave per y-band: ordered entry lists (partitionSegments)
+//   rows      phase A: every (call, row, window) row program in parallel,
+//             each leaving its pixel codes as sparse (x, delta) runs
+//   compose   phase B: each wave owns a canvas row window in registers and
+//             applies the runs of every call covering it, in call order
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pxh_common.h"
+
+namespace pxh {
+
+// Canonical command: absolute Move / Line / Cubic / Quad / Close.  Every
+// subpath starts with a Move (the host inserts the implicit ones).
+struct SceneCmd {
+  float kind;          // pixie command ordinal: Close 0, Move 1, Line 2, Cubic 5, Quad 7
+  float p[6];          // absolute parameters
+  float sx, sy;        // start point of the command's subpath
+  int32_t pad[3];
+};
+
+struct ScenePoly {     // one subpath of a call
+  int32_t firstCmd, numCmds;  // absolute SceneCmd indices; firstCmd is its Move
+  int32_t noClose;            // ended by a relative move: no implicit closing point
+  int32_t pad;
+};
+
+struct SceneItem {
+  int32_t cmdBase, cmdCount;   // SceneCmd range
+  int32_t polyBase, polyCount; // ScenePoly range
+  int32_t stroke;              // 0 fill, 1 stroke
+  uint32_t rgbx;
+  int32_t mode, rule;
+  int32_t cap, join;
+  float halfStroke;            // strokeWidth / 2
+  float miterAngleLimit;       // arcsin(1 / miterLimit) * 2
+  float minArea;               // pixelErrorMargin / pixelScale
+  float errSq;                 // (pixelErrorMargin / pixelScale)^2
+  float mat[9];
+  int32_t identity;
+  int32_t strokeNone;          // strokeWidth <= 0: strokeShapes yields no shapes
+  int32_t pad;
+};
+
+// Per-render derived data of one call.
+struct ItemGeom {
+  int32_t ptBase, ptCount;
+  int32_t segBase, segCount;
+  int32_t bandBase;
+  int32_t rowStart, rowCount;  // rows that run a row program
+  int32_t winBase, nWin;       // first column window, windows per row
+  int32_t taskBase;            // first phase-A task
+  int32_t status;
+  int32_t pad;
+};
+
+// Phase-A output of one (call, row, window).
+struct RowRuns {
+  int32_t off, n;    // run entries in the run pool
+  int32_t kind;      // RowKind
+  int32_t pad;
+};
+
+// Run entry: x in the low 28 bits (tag in bit 30 = area factor entry),
+// value = code delta at x, or the pixel's area factor.
+constexpr int32_t kRunArea = 1 << 30;
+constexpr int32_t kRunXMask = (1 << 28) - 1;
+
+enum SceneCounter {
+  kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks, kCntRuns, kCntUnitsA,
+  kCntUnitsB, kCntError, kCntMaxBand, kNumCounters
+};
+
+struct ScenePools {
+  // recorded inputs
+  const SceneItem* items;
+  const SceneCmd* cmds;
+  const ScenePoly* polys;
+  int32_t nItems;
+  int32_t W, H;              // target size
+  // per-call outputs
+  ItemGeom* geo;
+  FillInfo* fills;
+  // per-command / per-subpath scratch
+  int32_t* cmdCount;
+  int32_t* cmdOff;
+  int32_t* polyPt;
+  int32_t* polyN;
+  // pools
+  float2* pts;
+  int32_t* ptPoly;
+  int32_t ptCap;
+  float4* segs;
+  int8_t* segW;
+  int32_t segCap;
+  PartInfo* parts;
+  int32_t* bandItem;
+  int32_t bandCap;
+  Entry* entries;
+  int8_t* entW;
+  int32_t entryCap;
+  RowRuns* tasks;
+  int32_t taskCap;
+  int2* runs;
+  int32_t runCap;
+  int32_t* counters;         // SceneCounter
+  // row scratch for bands larger than the LDS capacity
+  char* gscratch;
+  int32_t gcap;              // entries per wave slot
+  int32_t ldsCap;            // entries that fit in LDS
+};
+
+int scene_set_lds_limits(size_t bytes);
+size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows
+int scene_launch_geometry(const ScenePools& p, hipStream_t s);
+int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);
+int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s);
+int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
+                         int grid, hipStream_t s);
+
+}  // namespace pxh

@@ -1,0 +1,1039 @@
+// pxh_scene.hip -- the batched form: device geometry, band building and the
+// two raster phases of a recorded scene (see pxh_scene.h).
+//
+// Every stage is exact with respect to the per-call form: flattening,
+// stroking, segments, bounds and bands use the same pxh_geom.h functions in
+// the same float32 order, and the row program is the same device code.
+#include <hip/hip_runtime.h>
+
+#include "pxh_raster.h"
+#include "pxh_scene.h"
+
+namespace pxh {
+
+constexpr int kBlock = 256;
+constexpr int kWinA = 1024;   // phase-A column window
+constexpr int kWinB = 1024;   // phase-B column window (4 chunks of 256 px)
+constexpr int kChunksB = kWinB / 256;
+
+// ---------------------------------------------------------------------------
+// block helpers (256 threads = 4 waves)
+// ---------------------------------------------------------------------------
+__device__ int block_excl_scan(int v, int* total, int* sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int wt;
+  int x = wave_excl_scan(v, &wt);
+  __syncthreads();
+  if (lane == 0) sh[w] = wt;
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kBlock / 64; i++) {
+    int t = sh[i];
+    if (i < w) off += t;
+    tot += t;
+  }
+  *total = tot;
+  return off + x;
+}
+
+__device__ int block_sum(int v, int* sh) {
+  int t;
+  block_excl_scan(v, &t, sh);
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// commands (SceneCmd) -> points, commandsToShapes (paths.nim:654-1057)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V2 cmd_end(const SceneCmd& c) {
+  switch ((int)c.kind) {
+    case 1: case 2: return v2(c.p[0], c.p[1]);  // Move, Line
+    case 5: return v2(c.p[4], c.p[5]);          // Cubic
+    case 7: return v2(c.p[2], c.p[3]);          // Quad
+    default: return v2(c.sx, c.sy);             // Close
+  }
+}
+
+struct CountEmit {
+  int n = 0;
+  __device__ void operator()(V2 a, V2 b) {
+    V2 d = a - b;
+    if (d.x != 0.0f || d.y != 0.0f) n++;  // addSegment skips zero-length pieces
+  }
+};
+
+struct StoreEmit {
+  float2* out;
+  int32_t* outPoly;
+  int32_t poly;
+  const float* mat;
+  bool xf;
+  int n = 0;
+  __device__ void put(V2 p) {
+    if (xf) p = mat_apply(mat, p);
+    out[n] = make_float2(p.x, p.y);
+    outPoly[n] = poly;
+    n++;
+  }
+  __device__ void operator()(V2 a, V2 b) {
+    V2 d = a - b;
+    if (d.x != 0.0f || d.y != 0.0f) put(b);
+  }
+};
+
+// Points command i emits (its non-degenerate pieces' end points).
+template <class Emit>
+__device__ int32_t cmd_emit(const SceneCmd* cmds, int i, float errSq, Emit& e) {
+  const SceneCmd& c = cmds[i];
+  int k = (int)c.kind;
+  if (k == 1) return kOk;  // Move
+  V2 at = cmd_end(cmds[i - 1]);  // a subpath always starts with a Move
+  switch (k) {
+    case 2: e(at, v2(c.p[0], c.p[1])); return kOk;
+    case 5: return flatten_cubic(at, v2(c.p[0], c.p[1]), v2(c.p[2], c.p[3]), v2(c.p[4], c.p[5]), errSq, e);
+    case 7: return flatten_quad(at, v2(c.p[0], c.p[1]), v2(c.p[2], c.p[3]), errSq, e);
+    case 0: {  // Close: addSegment(at, start) when at != start
+      V2 st = v2(c.sx, c.sy);
+      if (!veq(at, st)) e(at, st);
+      return kOk;
+    }
+    default: return kErrArg;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// stroke polygons (strokeShapes, paths.nim:1922-2082) -> segments
+// ---------------------------------------------------------------------------
+struct SegSink {
+  // counting when out == nullptr
+  float4* out = nullptr;
+  int8_t* outW = nullptr;
+  int n = 0;
+  Bounds bd;
+  __device__ void seg(V2 v1, V2 v2_) {
+    Seg s;
+    if (!make_segment(v1, v2_, &s)) return;
+    if (out) {
+      out[n] = make_float4(s.at.x, s.at.y, s.to.x, s.to.y);
+      outW[n] = (int8_t)s.w;
+      bounds_add(&bd, s);
+    }
+    n++;
+  }
+  // shapesToSegments (paths.nim:1069-1090) of one polygon q[0..m)
+  __device__ void poly(const V2* q, int m, const SceneItem& I) {
+    if (m <= 0) return;
+    V2 last = I.identity ? q[m - 1] : mat_apply(I.mat, q[m - 1]);
+    V2 vec1 = v2(last.x, quantize_y(last.y));
+    for (int i = 0; i < m; i++) {
+      V2 p = I.identity ? q[i] : mat_apply(I.mat, q[i]);
+      V2 vec2 = v2(p.x, quantize_y(p.y));
+      if (i == 0 && veq(vec1, vec2)) continue;
+      seg(vec1, vec2);
+      vec1 = vec2;
+    }
+  }
+};
+
+// Streams the polygon of makeCircle (paths.nim:1938-1941): the ellipse
+// path's first shape.  Its last point equals its first bit for bit (the last
+// cubic ends on the Move point), so the closing candidate is always skipped.
+struct CircleEmit {
+  SegSink* sink;
+  const SceneItem* I;
+  V2 prevq;
+  __device__ void add(V2 p) {
+    V2 t = I->identity ? p : mat_apply(I->mat, p);
+    V2 q = v2(t.x, quantize_y(t.y));
+    sink->seg(prevq, q);
+    prevq = q;
+  }
+  __device__ void operator()(V2 a, V2 b) {
+    V2 d = a - b;
+    if (d.x != 0.0f || d.y != 0.0f) add(b);
+  }
+};
+
+__device__ void circle_segments(V2 at, const SceneItem& I, SegSink& sink) {
+  const double K = 4.0 * (-1.0 + 1.4142135623730951) / 3.0;  // splineCircleK (float64)
+  float cx = at.x, cy = at.y, rx = I.halfStroke, ry = I.halfStroke;
+  double mx = K * (double)rx, my = K * (double)ry;
+  V2 s = v2(cx + rx, cy);
+  V2 st = I.identity ? s : mat_apply(I.mat, s);
+  CircleEmit ce{&sink, &I, v2(st.x, quantize_y(st.y))};
+  V2 a = s;
+  V2 c1 = v2(cx + rx, (float)((double)cy + my)), c2 = v2((float)((double)cx + mx), cy + ry), to = v2(cx, cy + ry);
+  flatten_cubic(a, c1, c2, to, I.errSq, ce);
+  a = to;
+  c1 = v2((float)((double)cx - mx), cy + ry); c2 = v2(cx - rx, (float)((double)cy + my)); to = v2(cx - rx, cy);
+  flatten_cubic(a, c1, c2, to, I.errSq, ce);
+  a = to;
+  c1 = v2(cx - rx, (float)((double)cy - my)); c2 = v2((float)((double)cx - mx), cy - ry); to = v2(cx, cy - ry);
+  flatten_cubic(a, c1, c2, to, I.errSq, ce);
+  a = to;
+  c1 = v2((float)((double)cx + mx), cy - ry); c2 = v2(cx + rx, (float)((double)cy - my)); to = v2(cx + rx, cy);
+  flatten_cubic(a, c1, c2, to, I.errSq, ce);
+}
+
+__device__ void rect_segments(V2 at, V2 to, const SceneItem& I, SegSink& sink) {  // makeRect
+  float hs = I.halfStroke;
+  V2 tangent = normalize(to - at);
+  V2 normal = v2(tangent.y, tangent.x);
+  V2 q[5];
+  q[0] = v2(at.x + normal.x * hs, at.y - normal.y * hs);
+  q[1] = v2(to.x + normal.x * hs, to.y - normal.y * hs);
+  q[2] = v2(to.x - normal.x * hs, to.y + normal.y * hs);
+  q[3] = v2(at.x - normal.x * hs, at.y + normal.y * hs);
+  q[4] = q[0];
+  sink.poly(q, 5, I);
+}
+
+__device__ __forceinline__ float fix_angle_d(float a) {  // vmath fixAngle
+  const float PI = 3.14159265358979323846f;
+  while (a > PI) a -= PI * 2.0f;
+  while (a < -PI) a += PI * 2.0f;
+  return a;
+}
+
+__device__ void join_segments(V2 prevPos, V2 pos, V2 nextPos, const SceneItem& I, SegSink& sink) {
+  const float PI = 3.14159265358979323846f;
+  float minArea = I.minArea, hs = I.halfStroke;
+  if (I.join == 1) {  // RoundJoin
+    float area = PI * hs * hs;
+    if (area > minArea) circle_segments(pos, I, sink);
+    return;
+  }
+  V2 dn = nextPos - pos, dp = prevPos - pos;
+  float angle = fix_angle_d(atan2f(dn.y, dn.x) - atan2f(dp.y, dp.x));
+  if (fabsf(fabsf(angle) - PI) > 0.00031415926535897936f) {
+    V2 a = normalize(pos - prevPos) * hs;
+    V2 b = normalize(pos - nextPos) * hs;
+    if (angle >= 0.0f) {
+      a = v2(-a.y, a.x);
+      b = v2(b.y, -b.x);
+    } else {
+      a = v2(a.y, -a.x);
+      b = v2(-b.y, b.x);
+    }
+    int j = I.join;
+    if (j == 0 && fabsf(angle) < I.miterAngleLimit) j = 2;
+    if (j == 0) {
+      V2 la0 = prevPos + a, la1 = pos + a, lb0 = nextPos + b, lb1 = pos + b;
+      V2 s1 = la1 - la0, s2 = lb1 - lb0;
+      float den = (-s2.x * s1.y + s1.x * s2.y);
+      if (den != 0.0f) {
+        float t = (s2.x * (la0.y - lb0.y) - s2.y * (la0.x - lb0.x)) / den;
+        V2 at = la0 + s1 * t;
+        float bl = lensq(at - pos);
+        float areaSq = 0.25f * (lensq(a) * bl + lensq(b) * bl);
+        if (areaSq > minArea * minArea) {
+          V2 q[5] = {pos + a, at, pos + b, pos, pos + a};
+          sink.poly(q, 5, I);
+        }
+      }
+    } else if (j == 2) {
+      float areaSq = 0.25f * lensq(a) * lensq(b);
+      if (areaSq > minArea * minArea) {
+        V2 q[4] = {a + pos, b + pos, pos, a + pos};
+        sink.poly(q, 4, I);
+      }
+    }
+  }
+}
+
+// Stroke polygons attributed to point k of a shape with n points, in
+// strokeShapes order.
+__device__ void stroke_point(const float2* P, int k, int n, const SceneItem& I, SegSink& sink) {
+  auto pt = [&](int i) { return v2(P[i].x, P[i].y); };
+  bool closed = veq(pt(0), pt(n - 1));
+  if (k == 0) {
+    if (!closed) {
+      if (I.cap == 1) circle_segments(pt(0), I, sink);
+      else if (I.cap == 2) {
+        V2 tangent = normalize(pt(1) - pt(0));
+        rect_segments(pt(0) - tangent * I.halfStroke, pt(0), I, sink);
+      }
+    }
+    return;
+  }
+  rect_segments(pt(k - 1), pt(k), I, sink);
+  if (k < n - 1) join_segments(pt(k - 1), pt(k), pt(k + 1), I, sink);
+  if (k == n - 1) {
+    if (closed) {
+      if (n >= 2) join_segments(pt(n - 2), pt(n - 1), pt(1), I, sink);
+    } else {
+      if (I.cap == 1) circle_segments(pt(n - 1), I, sink);
+      else if (I.cap == 2) {
+        V2 tangent = normalize(pt(n - 1) - pt(n - 2));
+        rect_segments(pt(n - 1) + tangent * I.halfStroke, pt(n - 1), I, sink);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// geometry: one workgroup per call
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
+  __shared__ int sh[8];
+  __shared__ int sBase;
+  __shared__ Bounds sBd[kBlock / 64];
+  __shared__ int sErr;
+  const int it = blockIdx.x;
+  const int tid = threadIdx.x;
+  const SceneItem I = P.items[it];
+  if (tid == 0) sErr = kOk;
+  __syncthreads();
+
+  // 1. points per command
+  for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
+    int i = I.cmdBase + c0 + tid;
+    if (c0 + tid < I.cmdCount) {
+      CountEmit ce;
+      int32_t r = cmd_emit(P.cmds, i, I.errSq, ce);
+      if (r != kOk) atomicMax(&sErr, r);
+      P.cmdCount[i] = ce.n;
+    }
+  }
+  __syncthreads();
+  // 2. points per subpath: start + pieces (+ the implicit closing point of a fill)
+  int total = 0;
+  for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
+    int q = I.polyBase + q0 + tid;
+    int np = 0;
+    if (q0 + tid < I.polyCount) {
+      ScenePoly sp = P.polys[q];
+      int n = 0;
+      for (int c = 0; c < sp.numCmds; c++) n += P.cmdCount[sp.firstCmd + c];
+      if (n > 0) {
+        np = n + 1;
+        const SceneCmd& last = P.cmds[sp.firstCmd + sp.numCmds - 1];
+        if (!I.stroke && (int)last.kind != 0 && !sp.noClose) {
+          V2 d = cmd_end(last) - v2(last.sx, last.sy);
+          if (d.x != 0.0f || d.y != 0.0f) np++;
+        }
+      }
+    }
+    int t;
+    int off = block_excl_scan(np, &t, sh);
+    if (q0 + tid < I.polyCount) {
+      P.polyPt[q] = total + off;
+      P.polyN[q] = np;
+    }
+    total += t;
+  }
+  if (tid == 0) {
+    int base = atomicAdd(&P.counters[kCntPoints], total);
+    if (base + total > P.ptCap) {
+      atomicOr(&P.counters[kCntError], 1);
+      sErr = kErrCapacity;
+    }
+    sBase = base;
+  }
+  __syncthreads();
+  const int ptBase = sBase;
+  if (sErr != kOk) {
+    if (tid == 0) {
+      P.geo[it].status = sErr;
+      P.geo[it].segCount = 0;
+      P.fills[it].status = sErr;
+      P.fills[it].pathWidth = 0;
+    }
+    return;
+  }
+  float2* pts = P.pts + ptBase;
+  int32_t* ptPoly = P.ptPoly + ptBase;
+  const bool xf = !I.stroke && !I.identity;  // fills transform points here
+
+  // 3. per-command point offsets, start and closing points (thread per subpath)
+  for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
+    int ql = q0 + tid;
+    if (ql < I.polyCount) {
+      int q = I.polyBase + ql;
+      ScenePoly sp = P.polys[q];
+      int np = P.polyN[q], pb = P.polyPt[q];
+      int off = pb + 1;
+      for (int c = 0; c < sp.numCmds; c++) {
+        P.cmdOff[sp.firstCmd + c] = off;
+        off += P.cmdCount[sp.firstCmd + c];
+      }
+      if (np > 0) {
+        const SceneCmd& mv = P.cmds[sp.firstCmd];
+        V2 s = v2(mv.sx, mv.sy);
+        V2 w = xf ? mat_apply(I.mat, s) : s;
+        pts[pb] = make_float2(w.x, w.y);
+        ptPoly[pb] = ql;
+        if (off < pb + np) {  // implicit closing point of a fill
+          pts[off] = make_float2(w.x, w.y);
+          ptPoly[off] = ql;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // 4. emit points (thread per command)
+  for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
+    int i = I.cmdBase + c0 + tid;
+    if (c0 + tid < I.cmdCount && P.cmdCount[i] > 0) {
+      // subpath of command i: the host stored it in the Move's pad slot
+      int ql = P.cmds[i].pad[0];
+      StoreEmit se{pts + P.cmdOff[i], ptPoly + P.cmdOff[i], ql, I.mat, xf};
+      cmd_emit(P.cmds, i, I.errSq, se);
+    }
+  }
+  __syncthreads();
+
+  // 5. segments: count, allocate, emit (thread per point)
+  int nseg = 0;
+  for (int pass = 0; pass < 2; pass++) {
+    int segOff = 0;
+    Bounds bd = bounds_init();
+    for (int j0 = 0; j0 < total; j0 += kBlock) {
+      int j = j0 + tid;
+      SegSink sink;
+      int cnt = 0;
+      if (j < total) {
+        int ql = ptPoly[j];
+        int q = I.polyBase + ql;
+        int pb = P.polyPt[q], np = P.polyN[q];
+        int k = j - pb;
+        if (!I.stroke) {
+          // shapesToSegments: candidate (prev point -> this point); the first
+          // point pairs with the polygon's last point
+          V2 a = k == 0 ? v2(pts[pb + np - 1].x, pts[pb + np - 1].y) : v2(pts[j - 1].x, pts[j - 1].y);
+          V2 b = v2(pts[j].x, pts[j].y);
+          V2 v1 = v2(a.x, quantize_y(a.y)), v2_ = v2(b.x, quantize_y(b.y));
+          if (!(k == 0 && veq(v1, v2_))) {
+            Seg s;
+            if (make_segment(v1, v2_, &s)) cnt = 1;
+          }
+        } else if (!I.strokeNone) {
+          stroke_point(pts + pb, k, np, I, sink);
+          cnt = sink.n;
+        }
+      }
+      int t;
+      int off = block_excl_scan(cnt, &t, sh);
+      if (pass == 1 && j < total && cnt > 0) {
+        float4* so = P.segs + sBase + segOff + off;
+        int8_t* sw = P.segW + sBase + segOff + off;
+        if (!I.stroke) {
+          int ql = ptPoly[j];
+          int q = I.polyBase + ql;
+          int pb = P.polyPt[q], np = P.polyN[q];
+          int k = j - pb;
+          V2 a = k == 0 ? v2(pts[pb + np - 1].x, pts[pb + np - 1].y) : v2(pts[j - 1].x, pts[j - 1].y);
+          V2 b = v2(pts[j].x, pts[j].y);
+          Seg s;
+          make_segment(v2(a.x, quantize_y(a.y)), v2(b.x, quantize_y(b.y)), &s);
+          so[0] = make_float4(s.at.x, s.at.y, s.to.x, s.to.y);
+          sw[0] = (int8_t)s.w;
+          bounds_add(&bd, s);
+        } else {
+          SegSink out;
+          out.out = so;
+          out.outW = sw;
+          out.bd = bd;
+          int ql = ptPoly[j];
+          int q = I.polyBase + ql;
+          int pb = P.polyPt[q], np = P.polyN[q];
+          stroke_point(pts + pb, j - pb, np, I, out);
+          bd = out.bd;
+        }
+      }
+      segOff += t;
+    }
+    if (pass == 0) {
+      nseg = segOff;
+      __syncthreads();
+      if (tid == 0) {
+        int base = atomicAdd(&P.counters[kCntSegs], nseg);
+        if (base + nseg > P.segCap) {
+          atomicOr(&P.counters[kCntError], 1);
+          sErr = kErrCapacity;
+        }
+        sBase = base;
+      }
+      __syncthreads();
+      if (sErr != kOk) break;
+    } else {
+      // 6. bounds -> fill locals, band layout
+      for (int o = 32; o > 0; o >>= 1) {
+        Bounds ob;
+        ob.xMin = __shfl_xor(bd.xMin, o, 64);
+        ob.xMax = __shfl_xor(bd.xMax, o, 64);
+        ob.yMin = __shfl_xor(bd.yMin, o, 64);
+        ob.yMax = __shfl_xor(bd.yMax, o, 64);
+        bd = bounds_merge(bd, ob);
+      }
+      if ((tid & 63) == 0) sBd[tid >> 6] = bd;
+      __syncthreads();
+      if (tid == 0) {
+        Bounds b = sBd[0];
+        for (int w = 1; w < kBlock / 64; w++) b = bounds_merge(b, sBd[w]);
+        FillInfo f = {};
+        f.rgbx = I.rgbx;
+        f.mode = I.mode;
+        f.rule = I.stroke ? kNonZero : I.rule;
+        int32_t r = fill_locals(b, false, P.W, P.H, &f);
+        f.status = r;
+        if (r == kOk && f.pathWidth > 0) band_layout(&f, nseg);
+        P.fills[it] = f;
+      }
+    }
+  }
+  if (tid == 0) {
+    ItemGeom g = {};
+    g.ptBase = ptBase;
+    g.ptCount = total;
+    g.segBase = sErr == kOk ? sBase : 0;
+    g.segCount = sErr == kOk ? nseg : 0;
+    g.status = sErr != kOk ? sErr : P.fills[it].status;
+    if (sErr != kOk) {
+      P.fills[it].status = sErr;
+      P.fills[it].pathWidth = 0;
+    }
+    P.geo[it] = g;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// planning (one block): bands, phase-A tasks, band -> call map
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) scene_plan_kernel(ScenePools P) {
+  __shared__ int sh[8];
+  const int tid = threadIdx.x;
+  int bandTot = 0, taskTot = 0;
+  for (int i0 = 0; i0 < P.nItems; i0 += kBlock) {
+    int it = i0 + tid;
+    int nb = 0, nt = 0;
+    ItemGeom g;
+    FillInfo f;
+    if (it < P.nItems) {
+      g = P.geo[it];
+      f = P.fills[it];
+      g.rowStart = 0;
+      g.rowCount = 0;
+      g.nWin = 0;
+      if (g.status == kOk && f.pathWidth > 0) {
+        nb = f.nParts;
+        int rs = f.startY, re = f.pathHeight;
+        if (re > P.H) re = P.H;
+        g.rowStart = rs;
+        g.rowCount = re > rs ? re - rs : 0;
+        int lo = f.mode == kMask ? 0 : (f.xlo & ~3), hi = f.mode == kMask ? P.W : f.xhi;
+        g.winBase = lo;
+        g.nWin = hi > lo ? (hi - lo + kWinA - 1) / kWinA : 0;
+        nt = g.rowCount * g.nWin;
+      }
+    }
+    int tb, tt;
+    int bOff = block_excl_scan(nb, &tb, sh);
+    int tOff = block_excl_scan(nt, &tt, sh);
+    if (it < P.nItems) {
+      g.bandBase = bandTot + bOff;
+      g.taskBase = taskTot + tOff;
+      P.geo[it] = g;
+      if (g.bandBase + nb <= P.bandCap)
+        for (int b = 0; b < nb; b++) P.bandItem[g.bandBase + b] = it;
+    }
+    bandTot += tb;
+    taskTot += tt;
+  }
+  if (tid == 0) {
+    P.counters[kCntBands] = bandTot;
+    P.counters[kCntTasks] = taskTot;
+    if (bandTot > P.bandCap || taskTot > P.taskCap) atomicOr(&P.counters[kCntError], 2);
+  }
+}
+
+// Band b of its call: top / bottom rows (partitionSegments 1183-1192).
+__device__ __forceinline__ void band_rows(const FillInfo& f, int k, float* top, float* bot) {
+  *top = (float)(f.startY + k * f.bandH);
+  *bot = (k == f.nParts - 1) ? (float)f.pathHeight : (float)(f.startY + (k + 1) * f.bandH);
+}
+
+__device__ __forceinline__ bool seg_in_band(const float4& s, const FillInfo& f, int k) {
+  if (f.nParts == 1) return true;
+  uint32_t lo, hi;
+  partition_range(s.y, s.w, (uint32_t)f.startY, (uint32_t)f.bandH, (uint32_t)f.nParts, &lo, &hi);
+  return (uint32_t)k >= lo && (uint32_t)k <= hi;
+}
+
+// One wave per band: entry count and requiresAntiAliasing.
+__global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
+  const int lane = threadIdx.x;
+  const int nb = P.counters[kCntBands];
+  if (nb > P.bandCap) return;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int it = P.bandItem[b];
+    const ItemGeom g = P.geo[it];
+    const FillInfo f = P.fills[it];
+    const int k = b - g.bandBase;
+    int cnt = 0;
+    bool aa = false;
+    for (int j0 = 0; j0 < g.segCount; j0 += 64) {
+      int j = j0 + lane;
+      bool in = false, a = false;
+      if (j < g.segCount) {
+        float4 s = P.segs[g.segBase + j];
+        in = seg_in_band(s, f, k);
+        if (in) a = seg_requires_aa(v2(s.x, s.y), v2(s.z, s.w));
+      }
+      cnt += __popcll(__ballot(in));
+      if (__ballot(a)) aa = true;
+    }
+    if (lane == 0) {
+      PartInfo pi;
+      pi.entryBase = 0;
+      pi.count = cnt;
+      pi.flags = aa ? 1 : 0;
+      pi.pad = 0;
+      P.parts[b] = pi;
+    }
+  }
+}
+
+// One block: entry offsets of all bands, largest band.
+__global__ void __launch_bounds__(kBlock) scene_entry_scan_kernel(ScenePools P) {
+  __shared__ int sh[8];
+  __shared__ int sMax;
+  const int nb = P.counters[kCntBands];
+  if (threadIdx.x == 0) sMax = 0;
+  __syncthreads();
+  if (nb > P.bandCap) return;
+  int tot = 0, mx = 0;
+  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+    int b = b0 + threadIdx.x;
+    int c = b < nb ? P.parts[b].count : 0;
+    mx = c > mx ? c : mx;
+    int t;
+    int off = block_excl_scan(c, &t, sh);
+    if (b < nb) P.parts[b].entryBase = tot + off;
+    tot += t;
+  }
+  atomicMax(&sMax, mx);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    P.counters[kCntEntries] = tot;
+    P.counters[kCntMaxBand] = sMax;
+    if (tot > P.entryCap) atomicOr(&P.counters[kCntError], 4);
+  }
+}
+
+// One wave per band: ordered entries, clipping, two-spanning flag
+// (partitionSegments 1194-1262).
+__global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
+  __shared__ float4 sTwo[2];
+  const int lane = threadIdx.x;
+  const int nb = P.counters[kCntBands];
+  if (nb > P.bandCap || P.counters[kCntEntries] > P.entryCap) return;
+  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+    const int it = P.bandItem[b];
+    const ItemGeom g = P.geo[it];
+    const FillInfo f = P.fills[it];
+    const int k = b - g.bandBase;
+    PartInfo pi = P.parts[b];
+    float top, bot;
+    band_rows(f, k, &top, &bot);
+    int pos = 0;
+    for (int j0 = 0; j0 < g.segCount; j0 += 64) {
+      int j = j0 + lane;
+      bool in = false;
+      float4 s = make_float4(0, 0, 0, 0);
+      if (j < g.segCount) {
+        s = P.segs[g.segBase + j];
+        in = seg_in_band(s, f, k);
+      }
+      uint64_t m = __ballot(in);
+      int p = pos + __popcll(m & ((1ull << lane) - 1ull));
+      if (in) {
+        V2 at = v2(s.x, s.y), to = v2(s.z, s.w);
+        Entry e = init_entry(at, to);
+        if (at.y <= top && to.y >= bot) {
+          V2 c = v2(0, 0);
+          clip_to_line(at, to, top, &c);
+          at = c;
+          clip_to_line(at, to, bot, &c);
+          to = c;
+          e.aty = at.y;
+          e.toy = to.y;
+        }
+        P.entries[pi.entryBase + p] = e;
+        P.entW[pi.entryBase + p] = P.segW[g.segBase + j];
+        if (p < 2) sTwo[p] = make_float4(at.x, at.y, to.x, to.y);
+      }
+      pos += __popcll(m);
+    }
+    __syncthreads();
+    if (lane == 0 && pi.count == 2) {
+      float4 a = sTwo[0], c = sTwo[1];
+      V2 a0 = v2(a.x, a.y), a1 = v2(a.z, a.w), c0 = v2(c.x, c.y), c1 = v2(c.z, c.w);
+      if (!intersects_inside(a0, a1, c0, c1)) {
+        if (a0.y <= top && a1.y >= bot && c0.y <= top && c1.y >= bot) {
+          pi.flags |= 2;
+          if (midpoint_x(a0, a1) > midpoint_x(c0, c1)) {
+            Entry t = P.entries[pi.entryBase];
+            P.entries[pi.entryBase] = P.entries[pi.entryBase + 1];
+            P.entries[pi.entryBase + 1] = t;
+            int8_t w = P.entW[pi.entryBase];
+            P.entW[pi.entryBase] = P.entW[pi.entryBase + 1];
+            P.entW[pi.entryBase + 1] = w;
+          }
+        }
+      }
+      P.parts[b].flags = pi.flags;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phase A: row programs of every (call, row, window) -> sparse runs
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_item(const ItemGeom* geo, int n, int task) {
+  int lo = 0, hi = n - 1;  // last item with taskBase <= task
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (geo[mid].taskBase <= task) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
+  extern __shared__ __align__(16) char smem[];
+  __shared__ int sTask;
+  const int lane = threadIdx.x;
+  const size_t fixedBytes = row_lds_fixed_bytes(kWinA);
+  RowLds Ll = row_lds_carve(smem, smem + fixedBytes, P.ldsCap, kWinA);
+  RowLds Lg = row_lds_carve(smem, P.gscratch + (size_t)blockIdx.x * row_cap_bytes(P.gcap), P.gcap, kWinA);
+  for (int i = lane; i < kWinA + 4; i += 64) Ll.D[i] = 0;
+  __syncthreads();
+  const int ntasks = P.counters[kCntTasks];
+  if (P.counters[kCntError]) return;
+  for (;;) {
+    if (lane == 0) sTask = atomicAdd(&P.counters[kCntUnitsA], 1);
+    __syncthreads();
+    const int task = sTask;
+    __syncthreads();
+    if (task >= ntasks) break;
+    const int it = find_item(P.geo, P.nItems, task);
+    const ItemGeom g = P.geo[it];
+    const FillInfo& F = P.fills[it];
+    const int local = task - g.taskBase;
+    const int y = g.rowStart + local / g.nWin;
+    const int win = local % g.nWin;
+    const bool mask = F.mode == kMask;
+    RowCtx C;
+    C.F = &P.fills[it];
+    C.parts = P.parts + g.bandBase;
+    C.E = P.entries;
+    C.EW = P.entW;
+    C.y = y;
+    C.cx0 = mask ? 0 : F.xlo;
+    C.cx1 = mask ? P.W : F.xhi;
+    C.wx0 = g.winBase + win * kWinA;
+    C.wx1 = C.wx0 + kWinA < C.cx1 ? C.wx0 + kWinA : C.cx1;
+    // band of this row decides where the row scratch lives
+    int pi = (y - F.startY) / F.bandH;
+    if (pi > F.nParts - 1) pi = F.nParts - 1;
+    const int cnt = C.parts[pi].count;
+    RowLds& L = cnt <= P.ldsCap ? Ll : Lg;
+    int32_t kind = cnt <= P.gcap ? row_program(C, L) : -1;
+    RowRuns rr;
+    rr.off = 0;
+    rr.n = 0;
+    rr.kind = kind;
+    rr.pad = 0;
+    if (kind < 0) {
+      if (lane == 0) atomicOr(&P.counters[kCntError], 8);
+    } else if (kind != kRowNone) {
+      // compact D[0..wlen] (nonzero deltas) and the area factors
+      const int wlen = C.wx1 - C.wx0;
+      int total = 0;
+      for (int pass = 0; pass < 2; pass++) {
+        int carry = 0, written = 0;
+        for (int base = 0; base <= wlen; base += 4 * kWave) {
+          int i0 = base + 4 * lane;
+          int32_t d[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int q = 0; q < 4; q++)
+            if (i0 + q <= wlen) d[q] = L.D[i0 + q];
+          int32_t c0 = d[0], c1 = c0 + d[1], c2 = c1 + d[2], c3 = c2 + d[3];
+          int32_t tot;
+          int32_t off = carry + wave_excl_scan(c3, &tot);
+          carry += tot;
+          int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+          int n = 0;
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            if (i0 + q > wlen) continue;
+            if (d[q] != 0) n++;
+            if (kind == kRowOps && i0 + q < wlen && code[q] >= kArea && code[q] < kClear) n++;
+          }
+          int ntot;
+          int p = wave_excl_scan(n, &ntot);
+          if (pass == 1) {
+            int2* out = P.runs + rr.off + written + p;
+            int w = 0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              if (i0 + q > wlen) continue;
+              int x = C.wx0 + i0 + q;
+              if (d[q] != 0) out[w++] = make_int2(x, d[q]);
+              if (kind == kRowOps && i0 + q < wlen && code[q] >= kArea && code[q] < kClear)
+                out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
+              L.D[i0 + q] = 0;
+            }
+          }
+          written += ntot;
+        }
+        if (pass == 0) {
+          total = written;
+          if (lane == 0) sTask = atomicAdd(&P.counters[kCntRuns], total);
+          __syncthreads();
+          rr.off = sTask;
+          __syncthreads();
+          if (rr.off + total > P.runCap) {
+            if (lane == 0) atomicOr(&P.counters[kCntError], 16);
+            for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
+            total = 0;
+            break;
+          }
+        }
+      }
+      rr.n = total;
+      __syncthreads();
+    }
+    if (lane == 0) P.tasks[task] = rr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phase B: canvas row windows in registers, calls applied in order
+// ---------------------------------------------------------------------------
+template <bool kA8>
+__global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* canvas,
+                                                          unsigned long long* touchedOut) {
+  __shared__ int32_t D[kWinB + 4];
+  __shared__ uint32_t AX[kWinB];
+  __shared__ int sUnit;
+  const int lane = threadIdx.x;
+  for (int i = lane; i < kWinB + 4; i += 64) D[i] = 0;
+  const int W = P.W, H = P.H;
+  const int winsPerRow = (W + kWinB - 1) / kWinB;
+  const int nUnits = H * winsPerRow;
+  uint32_t ntouch = 0;
+  if (P.counters[kCntError]) return;
+  for (;;) {
+    __syncthreads();
+    if (lane == 0) sUnit = atomicAdd(&P.counters[kCntUnitsB], 1);
+    __syncthreads();
+    const int unit = sUnit;
+    if (unit >= nUnits) break;
+    const int y = unit / winsPerRow;
+    const int b0 = (unit % winsPerRow) * kWinB;
+    const int b1 = b0 + kWinB < W ? b0 + kWinB : W;
+    const bool vec = ((W & 3) == 0);
+    uint32_t px[kChunksB][4];
+    bool loaded = false, dirty = false;
+    for (int i0 = 0; i0 < P.nItems; i0 += 64) {
+      int it = i0 + lane;
+      bool cov = false;
+      if (it < P.nItems) {
+        const FillInfo& f = P.fills[it];
+        const ItemGeom& g = P.geo[it];
+        if (f.status == kOk && f.pathWidth > 0) {
+          if (f.mode == kMask) cov = true;
+          else cov = y >= g.rowStart && y < g.rowStart + g.rowCount && f.xhi > b0 && f.xlo < b1;
+        }
+      }
+      uint64_t m = __ballot(cov);
+      while (m) {
+        const int it2 = i0 + __builtin_ctzll(m);
+        m &= m - 1;
+        const FillInfo f = P.fills[it2];
+        const ItemGeom g = P.geo[it2];
+        if (!loaded) {
+#pragma unroll
+          for (int c = 0; c < kChunksB; c++) {
+            int x0 = b0 + c * 256 + 4 * lane;
+            if (kA8) {
+              const uint8_t* row = (const uint8_t*)canvas + (size_t)y * W;
+              if (vec && x0 + 3 < b1) {
+                uint32_t v = *(const uint32_t*)(row + x0);
+#pragma unroll
+                for (int q = 0; q < 4; q++) px[c][q] = ((v >> (8 * q)) & 255u) * 0x01010101u;
+              } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) px[c][q] = (x0 + q < b1) ? row[x0 + q] * 0x01010101u : 0u;
+              }
+            } else {
+              const uint32_t* row = (const uint32_t*)canvas + (size_t)y * W;
+              if (vec && x0 + 3 < b1) {
+                uint4 v = *(const uint4*)(row + x0);
+                px[c][0] = v.x; px[c][1] = v.y; px[c][2] = v.z; px[c][3] = v.w;
+              } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) px[c][q] = (x0 + q < b1) ? row[x0 + q] : 0u;
+              }
+            }
+          }
+          loaded = true;
+        }
+        const bool mask = f.mode == kMask;
+        if (mask && !(y >= g.rowStart && y < g.rowStart + g.rowCount)) {
+          // rows outside a mask fill's path are cleared (paths.nim:1910-1912)
+#pragma unroll
+          for (int c = 0; c < kChunksB; c++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              int x = b0 + c * 256 + 4 * lane + q;
+              if (x < b1) {
+                px[c][q] = 0;
+                ntouch++;
+              }
+            }
+          dirty = true;
+          continue;
+        }
+        // gather this call's runs for row y into D / AX
+        const int tb = g.taskBase + (y - g.rowStart) * g.nWin;
+        int carry = 0;
+        int kind = kRowNone;
+        for (int w = 0; w < g.nWin; w++) {
+          RowRuns rr = P.tasks[tb + w];
+          if (rr.kind > kind) kind = rr.kind;
+          for (int e0 = 0; e0 < rr.n; e0 += 64) {
+            int e = e0 + lane;
+            if (e < rr.n) {
+              int2 r = P.runs[rr.off + e];
+              int x = r.x & kRunXMask;
+              if (r.x & kRunArea) {
+                if (x >= b0 && x < b1) AX[x - b0] = (uint32_t)r.y;
+              } else if (x < b0) {
+                carry += r.y;
+              } else if (x < b1) {
+                atomicAdd(&D[x - b0], r.y);
+              }
+            }
+          }
+        }
+        for (int o = 32; o > 0; o >>= 1) carry += __shfl_xor(carry, o, 64);
+        __syncthreads();
+        if (kind == kRowNone) continue;
+        const int lo = mask ? b0 : (f.xlo > b0 ? f.xlo : b0);
+        const int hi = mask ? b1 : (f.xhi < b1 ? f.xhi : b1);
+        const int cLo = (lo - b0) / 256;
+        int cHi = (hi - b0) / 256;
+        if (cHi > kChunksB - 1) cHi = kChunksB - 1;
+#pragma unroll
+        for (int c = 0; c < kChunksB; c++) {
+          if (c < cLo || c > cHi) continue;
+          int i0 = c * 256 + 4 * lane;
+          int32_t d0 = D[i0], d1 = D[i0 + 1], d2 = D[i0 + 2], d3 = D[i0 + 3];
+          D[i0] = 0;
+          D[i0 + 1] = 0;
+          D[i0 + 2] = 0;
+          D[i0 + 3] = 0;
+          int32_t c0 = d0, c1 = c0 + d1, c2 = c1 + d2, c3 = c2 + d3;
+          int32_t tot;
+          int32_t off = carry + wave_excl_scan(c3, &tot);
+          carry += tot;
+          int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+#pragma unroll
+          for (int q = 0; q < 4; q++) {
+            int x = b0 + i0 + q;
+            if (x < lo || x >= hi) continue;
+            uint32_t ax = (kind == kRowOps && code[q] >= kArea && code[q] < kClear) ? AX[i0 + q] : 0u;
+            bool t = false;
+            px[c][q] = apply_code(kind, code[q], ax, px[c][q], f.rgbx, f.mode, &t);
+            if (t) {
+              ntouch++;
+              dirty = true;
+            }
+          }
+        }
+        // end markers of runs that stop at the window edge live at index b1-b0
+        if (lane == 0) D[kWinB] = 0;
+        __syncthreads();
+      }
+    }
+    if (__ballot(dirty) != 0) {
+#pragma unroll
+      for (int c = 0; c < kChunksB; c++) {
+        int x0 = b0 + c * 256 + 4 * lane;
+        if (kA8) {
+          uint8_t* row = (uint8_t*)canvas + (size_t)y * W;
+          if (vec && x0 + 3 < b1) {
+            *(uint32_t*)(row + x0) = ch_a(px[c][0]) | (ch_a(px[c][1]) << 8) | (ch_a(px[c][2]) << 16) |
+                                     (ch_a(px[c][3]) << 24);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (x0 + q < b1) row[x0 + q] = (uint8_t)ch_a(px[c][q]);
+          }
+        } else {
+          uint32_t* row = (uint32_t*)canvas + (size_t)y * W;
+          if (vec && x0 + 3 < b1) {
+            *(uint4*)(row + x0) = make_uint4(px[c][0], px[c][1], px[c][2], px[c][3]);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+              if (x0 + q < b1) row[x0 + q] = px[c][q];
+          }
+        }
+      }
+    }
+  }
+  if (touchedOut) {
+    for (int o = 32; o > 0; o >>= 1) ntouch += __shfl_xor(ntouch, o, 64);
+    if (lane == 0 && ntouch) atomicAdd(touchedOut, (unsigned long long)ntouch);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launcher
+// ---------------------------------------------------------------------------
+int scene_set_lds_limits(size_t bytes) {
+  // the kernel's static LDS (its task slot) counts against the same limit
+  if (hipFuncSetAttribute((const void*)scene_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)(bytes - 1024)) != hipSuccess)
+    return kErrDevice;
+  return 0;
+}
+
+size_t scene_gscratch_bytes(int rowGrid, int gcap) { return (size_t)rowGrid * row_cap_bytes(gcap); }
+
+int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
+  if (p.nItems <= 0) return 0;
+  hipLaunchKernelGGL(scene_geom_kernel, dim3(p.nItems), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_plan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(scene_band_count_kernel, dim3(grid), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_band_write_kernel, dim3(grid), dim3(64), 0, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s) {
+  size_t lds = row_lds_fixed_bytes(kWinA) + row_cap_bytes(p.ldsCap);
+  hipLaunchKernelGGL(scene_rows_kernel, dim3(grid), dim3(64), lds, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
+                         int grid, hipStream_t s) {
+  if (a8)
+    hipLaunchKernelGGL(scene_compose_kernel<true>, dim3(grid), dim3(64), 0, s, p, canvas, touched);
+  else
+    hipLaunchKernelGGL(scene_compose_kernel<false>, dim3(grid), dim3(64), 0, s, p, canvas, touched);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+}  // namespace pxh

@@ -1,0 +1,325 @@
+"""GPU parity of the batched form (pxh_scene_*) against the per-call form and
+the oracle.
+
+A scene replays pixie's fillPath / strokePath calls (paths.nim:2093-2176) in
+order with device geometry and two raster phases.  Its device geometry
+(flattening, stroking, segments, bounds, bands) must equal the host geometry
+of the per-call form bit for bit, its image must equal the image the same
+calls leave through the per-call form bit for bit, with the same N_touch, and
+both stay within the coverage bar of the oracle.
+"""
+import numpy as np
+import pytest
+
+import pixie_amd as px
+from pixie_amd import workloads as WL
+from pixie_amd.paths import Path, translate, scale, mat_mul
+
+pytestmark = pytest.mark.gpu
+
+COV_MAX_DIFF = 1        # 1/255 per channel against the oracle
+COV_MAX_FRAC = 1e-4     # 0.01 % of pixels
+
+
+def _cov_check(got, want, what=""):
+    d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+    px_diff = d.max(axis=-1) if d.ndim == 3 else d
+    frac = float(np.mean(px_diff > 0))
+    print("%s: max diff %d, %.6f%% of pixels differ" % (what, int(d.max()), 100 * frac))
+    assert d.max() <= COV_MAX_DIFF, what
+    assert frac <= COV_MAX_FRAC, what
+
+
+def _exact(got, want, what=""):
+    if not np.array_equal(got, want):
+        d = np.abs(got.astype(np.int32) - want.astype(np.int32))
+        idx = np.argwhere(d.max(axis=-1) > 0 if d.ndim == 3 else d > 0)
+        raise AssertionError("%s: %d pixels differ, first %s got %s want %s" % (
+            what, len(idx), idx[0], got[tuple(idx[0])], want[tuple(idx[0])]))
+
+
+def rotate(a):
+    c, s = np.float32(np.cos(a)), np.float32(np.sin(a))
+    return np.array([c, s, 0, -s, c, 0, 0, 0, 1], np.float32)
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    px.init()
+    return px
+
+
+# Commands without elliptical arcs, covering every canonicalisation rule:
+# relative / H / V forms, smooth cubics and quads (reflection and its absence),
+# implicit moves after Close, the relative-move no-close quirk, degenerate
+# pieces, paths leaving the canvas.
+PATHS = [
+    "M 10 10 H 90 V 90 H 10 L 10 10",
+    "M 10.1 10.1 H 60.1 V 60.1 H 10.1 z",
+    "m 12 15 h 50 v 40 h -50 z m 60 0 l 20 30 l -25 5 z",
+    "M 10 50 C 20 10 40 10 50 50 S 80 90 90 50 Z",
+    "M 10 50 S 30 10 50 50 S 80 90 90 50",
+    "M 10 80 Q 30 10 50 80 T 90 80 Z",
+    "M 10 80 T 50 20 T 90 80",
+    "M 10 40 c 10 -30 30 -30 40 0 s 30 40 40 0 q 10 30 -20 40 t -40 0 z",
+    "M 10 10 l 30 0 l 0 30 m 20 20 l 30 0 l 0 30 z",
+    "M 10 10 L 50 10 L 50 50 Z L 80 80 L 10 80",
+    "M 40 40 L 40 80 L 80 80 L 80 40 C 80 -20 40 100 40 40",
+    "M99 99 L999 99 L999 100 L99 100 Z",
+    "M0 0 L0 0 L60 0 L60 60 L0 60",
+    "M 5 5 L 95 20 L 30 95 Z M 50 10 L 90 90 L 10 60 Z",
+    "M -20 -20 L 130 30 L 40 140 Z",
+    "M 20 20 L 20 20 M 30 30 L 70 30 L 70 70 Z M 5 90",
+    "M 10 10 v 0 h 0 L 60 12 L 30 70 Z z",
+]
+
+XFORMS = [None, translate(3.25, -7.5), mat_mul(scale(1.7, 0.6), rotate(0.4)),
+          mat_mul(translate(50, 50), mat_mul(rotate(-1.1), translate(-50, -50)))]
+
+
+def _calls(rng, n, W, H):
+    """Random (stroke, path, xform, kwargs) calls drawn from PATHS."""
+    out = []
+    for i in range(n):
+        path = PATHS[int(rng.integers(len(PATHS)))]
+        xf = XFORMS[int(rng.integers(len(XFORMS)))]
+        off = translate(float(rng.uniform(-20, W - 60)), float(rng.uniform(-20, H - 60)))
+        xf = off if xf is None else mat_mul(off, xf)
+        rgb = rng.integers(0, 256, 3)
+        color = WL.premul(int(rgb[0]), int(rgb[1]), int(rgb[2]), int(rng.integers(40, 256)))
+        mode = int(rng.integers(0, 20)) if rng.random() < 0.5 else 0
+        if mode == 16 and rng.random() < 0.7:
+            mode = 0  # keep masks rare: they clear everything outside them
+        if rng.random() < 0.35:
+            out.append((True, path, xf, dict(color=color, strokeWidth=float(rng.uniform(0.5, 9)),
+                                             lineCap=int(rng.integers(0, 3)),
+                                             lineJoin=int(rng.integers(0, 3)),
+                                             miterLimit=float(rng.choice([1.5, 4.0, 10.0])),
+                                             blendMode=mode)))
+        else:
+            out.append((False, path, xf, dict(color=color, windingRule=int(rng.integers(0, 2)),
+                                              blendMode=mode)))
+    return out
+
+
+def _per_call(img, calls):
+    for stroke, path, xf, kw in calls:
+        kw = dict(kw)
+        color = kw.pop("color")
+        if stroke:
+            img.strokePath(path, color, transform=xf, **kw)
+        else:
+            img.fillPath(path, color, transform=xf, **kw)
+
+
+def _scene(calls):
+    sc = px.Scene()
+    for stroke, path, xf, kw in calls:
+        kw = dict(kw)
+        color = kw.pop("color")
+        if stroke:
+            sc.strokePath(path, color, transform=xf, **kw)
+        else:
+            sc.fillPath(path, color, transform=xf, **kw)
+    return sc
+
+
+def _oracle(orc, canvas, calls):
+    want = canvas.copy()
+    for stroke, path, xf, kw in calls:
+        c = px._cmds(path)
+        if stroke:
+            orc.stroke_path(want, c, kw["color"], width=kw["strokeWidth"], cap=kw["lineCap"],
+                            join=kw["lineJoin"], miter_limit=kw["miterLimit"], mode=kw["blendMode"],
+                            mat=xf)
+        else:
+            orc.fill_path(want, c, kw["color"], winding=kw["windingRule"], mode=kw["blendMode"],
+                          mat=xf)
+    return want
+
+
+def _compare(gpu, calls, canvas, what, orc=None):
+    ref = px.Image.from_array(canvas)
+    px.synchronize()
+    px.touched_pixels(reset=True)
+    _per_call(ref, calls)
+    want_touch = px.touched_pixels(reset=True)
+    want = ref.download()
+    sc = _scene(calls)
+    img = px.Image.from_array(canvas)
+    px.touched_pixels(reset=True)
+    sc.render(img)
+    got = img.download()
+    got_touch = px.touched_pixels(reset=True)
+    print("%s: %d calls, N_touch %d" % (what, len(sc), got_touch), flush=True)
+    _exact(got, want, what + " scene vs per-call")
+    assert got_touch == want_touch, (got_touch, want_touch)
+    # a second render reuses the validated pools without a host round trip
+    img2 = px.Image.from_array(canvas)
+    sc.render(img2)
+    _exact(img2.download(), want, what + " second render")
+    if orc is not None:
+        _cov_check(got, _oracle(orc, canvas, calls), what + " vs oracle")
+    return sc
+
+
+def test_scene_symbols_and_empty(gpu):
+    sc = px.Scene()
+    assert len(sc) == 0
+    img = px.Image(16, 16)
+    sc.render(img)  # nothing recorded: nothing to do
+    assert not img.download().any()
+
+
+@pytest.mark.parametrize("k", range(len(PATHS)))
+def test_scene_geometry_matches_host(gpu, k):
+    path = PATHS[k]
+    calls = []
+    for j, xf in enumerate(XFORMS):
+        calls.append((False, path, xf, dict(color=(255, 0, 0, 255), windingRule=j & 1,
+                                            blendMode=0)))
+        calls.append((True, path, xf, dict(color=(0, 0, 255, 255), strokeWidth=1.0 + 2.5 * j,
+                                           lineCap=j % 3, lineJoin=(j + 1) % 3, miterLimit=4.0,
+                                           blendMode=0)))
+    sc = _scene(calls)
+    img = px.Image(130, 110)
+    sc.render(img)
+    st = sc.status()
+    for i, (stroke, p, xf, kw) in enumerate(calls):
+        info, parts, ents, wind = px.debug_fill_geometry(
+            p, 130, 110, transform=xf, stroke=stroke, width=kw.get("strokeWidth", 1.0),
+            cap=kw.get("lineCap", 0), join=kw.get("lineJoin", 0), miter=kw.get("miterLimit", 4.0))
+        dinfo, dparts, dents, dwind = sc.debug_geometry(i)
+        what = "path %d call %d" % (k, i)
+        assert st[i] == 0, what
+        keep = [j for j in range(16) if j not in (6, 15)]  # partBase, pad
+        assert np.array_equal(dinfo[keep], info[keep]), (what, dinfo, info)
+        if info[2] <= 0:
+            continue
+        assert np.array_equal(dparts, parts), what
+        assert np.array_equal(dents.view(np.int32), ents.view(np.int32)), what
+        assert np.array_equal(dwind, wind), what
+
+
+@pytest.mark.parametrize("seed,W,H", [(1, 256, 192), (2, 301, 157), (3, 96, 400)])
+def test_scene_random_calls(gpu, orc, seed, W, H):
+    rng = np.random.default_rng(seed)
+    calls = _calls(rng, 80, W, H)
+    canvas = WL.random_premultiplied_image(rng, W, H)
+    _compare(gpu, calls, canvas, "random calls seed %d %dx%d" % (seed, W, H), orc)
+
+
+@pytest.mark.parametrize("mode", range(20))
+def test_scene_blend_modes(gpu, orc, mode):
+    rng = np.random.default_rng(40 + mode)
+    calls = []
+    for i in range(6):
+        path = PATHS[(3 * i + mode) % len(PATHS)]
+        xf = translate(float(rng.uniform(0, 60)), float(rng.uniform(0, 60)))
+        color = WL.premul(200, 40 + 30 * i, 90, 255 if i % 2 else 150)
+        if i == 5:
+            color = (0, 0, 0, 0)  # only drawn by OverwriteBlend
+        calls.append((i % 3 == 2, path, xf, dict(color=color, strokeWidth=3.0, lineCap=1,
+                                                  lineJoin=i % 3, miterLimit=4.0, blendMode=mode)
+                      if i % 3 == 2 else dict(color=color, windingRule=i & 1, blendMode=mode)))
+    canvas = WL.random_premultiplied_image(rng, 160, 150)
+    _compare(gpu, calls, canvas, "mode %d" % mode, orc)
+
+
+@pytest.mark.parametrize("seed", [2, 12])
+def test_scene_mixed_small(gpu, orc, seed):
+    cmds, canvas = WL.mixed_scene(seed=seed, n_paths=60, W=512, H=512)
+    calls = []
+    for c in cmds:
+        if c.stroke:
+            calls.append((True, c.cmds, None, dict(color=c.color, strokeWidth=c.width,
+                                                   lineCap=c.cap, lineJoin=c.join,
+                                                   miterLimit=c.miter, blendMode=c.mode)))
+        else:
+            calls.append((False, c.cmds, None, dict(color=c.color, windingRule=c.winding,
+                                                    blendMode=c.mode)))
+    _compare(gpu, calls, canvas, "mixed seed %d" % seed, orc)
+
+
+def test_scene_a8(gpu, orc):
+    rng = np.random.default_rng(4)
+    sc = px.Scene()
+    img = px.Image(300, 260, a8=True)
+    ref = px.Image(300, 260, a8=True)
+    want = orc.new_image(300, 260)
+    for i in range(10):
+        p = WL.glyph_path(rng, 200, 200)
+        mode = 0 if i != 7 else 17  # one subtract-mask call
+        sc.fillPath(p, (255, 255, 255, 255), blendMode=mode)
+        ref.fillPath(p, (255, 255, 255, 255), blendMode=mode)
+        orc.fill_path(want, p.array(), (255, 255, 255, 255), mode=mode)
+    sc.render(img)
+    _exact(img.download(), ref.download(), "a8 scene vs per-call")
+    _cov_check(img.download(), want[..., 3], "a8 scene vs oracle")
+
+
+def test_scene_wide_bands_and_pool_growth(gpu, orc):
+    # a band with more entries than LDS holds (global row scratch), more tasks
+    # than the first pool estimate, and a flattening that needs more points
+    # than the first estimate (huge transformed circle)
+    W, H = 3000, 2300
+    zig = Path()
+    zig.move_to(0, 100)
+    for i in range(700):
+        zig.line_to(i * 4 + 2, 110 if i % 2 == 0 else 100)
+    zig.line_to(2800, 140)
+    zig.line_to(0, 140)
+    zig.close_path()
+    circle = Path()
+    circle.ellipse(0, 0, 1, 1)
+    calls = [
+        (False, zig.array(), None, dict(color=(255, 0, 0, 255), windingRule=1, blendMode=0)),
+        (False, circle.array(), mat_mul(translate(1500, 1150), scale(2.5e5)),
+         dict(color=(0, 120, 0, 200), windingRule=0, blendMode=0)),
+        (False, "M 10 10 L 2990 20 L 2900 2290 L 30 2200 Z", None,
+         dict(color=(0, 0, 200, 128), windingRule=0, blendMode=3)),
+    ]
+    canvas = np.zeros((H, W, 4), np.uint8)
+    _compare(gpu, calls, canvas, "wide bands + pool growth", orc)
+
+
+def test_scene_status_and_errors(gpu):
+    sc = px.Scene()
+    with pytest.raises(px.PixieError):
+        sc.fillPath("M 10 10 A 5 5 0 0 1 20 20 Z", (255, 0, 0, 255))
+    assert len(sc) == 0
+    sc.fillPath("M 0 0 L 10 0 L 10 10 Z", (0, 0, 0, 0))            # transparent: skipped
+    sc.strokePath("M 0 0 L 10 0", (255, 0, 0, 255), strokeWidth=0)  # zero width: skipped
+    assert len(sc) == 0
+    sc.fillPath("M 1 1 L 30 1 L 30 30 Z", (255, 0, 0, 255))
+    sc.fillPath("M 1e30 0 L -1e30 0 L 0 10 Z", (0, 255, 0, 255))    # int overflow
+    sc.fillPath("M 5 5 L 20 5 L 20 20 Z", (0, 0, 255, 255))
+    img = px.Image(40, 40)
+    sc.render(img)
+    st = sc.status()
+    ref = px.Image(40, 40)
+    ref.fillPath("M 1 1 L 30 1 L 30 30 Z", (255, 0, 0, 255))
+    with pytest.raises(px.PixieError) as e:
+        ref.fillPath("M 1e30 0 L -1e30 0 L 0 10 Z", (0, 255, 0, 255))
+    ref.fillPath("M 5 5 L 20 5 L 20 20 Z", (0, 0, 255, 255))
+    assert st[0] == 0 and st[2] == 0
+    assert st[1] == e.value.code
+    _exact(img.download(), ref.download(), "status scene")
+
+
+def test_scene_band_over_capacity_is_an_error(gpu):
+    # more than 8192 segments crossing one row: the batched form reports a
+    # capacity error and leaves the image untouched
+    zig = Path()
+    zig.move_to(0, 10)
+    for i in range(9000):
+        zig.line_to(i * 0.25, 20 if i % 2 == 0 else 10)
+    zig.line_to(0, 30)
+    zig.close_path()
+    sc = px.Scene()
+    sc.fillPath(zig.array(), (255, 0, 0, 255))
+    img = px.Image(2300, 40)
+    with pytest.raises(px.PixieError) as e:
+        sc.render(img)
+    assert e.value.code == 9
+    assert not img.download().any()
