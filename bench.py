@@ -7,6 +7,10 @@ of 4-32 cubic Beziers (bbox side lognormal(ln 64, 1) clipped to [4, 2048]),
 30% of paths also stroked (width 1-8, random joins and caps) -- rendered onto
 a 4096x4096 premultiplied RGBA8 canvas that is resident in HBM.
 
+Forms: "batched" (default) records the calls once as a device-resident scene
+and renders them per step in one device pass (geometry, bands, row programs,
+in-register compose); "plain" issues one fillPath / strokePath per call.
+
 value = N_touch / step time, in Mpix/s, where N_touch counts the pixels the
 fills write (the unit pixie's fillShapes does its per-pixel work in).
 Algorithmic bytes = 8 B per touched pixel (RGBA8 read + write).
@@ -14,7 +18,7 @@ Algorithmic bytes = 8 B per touched pixel (RGBA8 read + write).
 The CPU baseline is the oracle (a C restatement of pixie's scalar path) on a
 bounded prefix of the same scene, one core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form plain]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--form batched|plain]
 """
 import argparse
 import json
@@ -41,6 +45,43 @@ def run_plain(px, img, cmds):
                            miterLimit=c.miter, blendMode=c.mode)
         else:
             img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+
+
+def record_scene(px, cmds):
+    sc = px.Scene()
+    for c in cmds:
+        if c.stroke:
+            sc.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                          miterLimit=c.miter, blendMode=c.mode)
+        else:
+            sc.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    return sc
+
+
+def timed(px, fn, steps):
+    """Runs fn() `steps` times between device synchronisations; returns
+    (wall seconds, N_touch, {stage: ms}) with per-launch event timing."""
+    px.synchronize()
+    px.touched_pixels(reset=True)
+    px.profile_begin()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        fn()
+        log("step %d queued" % k)
+    px.synchronize()
+    wall = time.perf_counter() - t0
+    px.profile_end()
+    touched = px.touched_pixels(reset=True)
+    stages = {}
+    for name in STAGES:
+        ms, n = px.profile_kernel(name)
+        if n:
+            stages[name] = (ms, n)
+    return wall, touched, stages
+
+
+STAGES = ("fill_rows_kernel", "scene_geometry", "scene_bands", "scene_rows_kernel",
+          "scene_compose_kernel")
 
 
 def cpu_baseline(cmds, canvas, seconds):
@@ -90,14 +131,18 @@ def star_extra(px, WL, reps=50):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--form", default="plain", choices=["plain"])
+    ap.add_argument("--steps", type=int, default=None, help="default 20 batched, 3 plain")
+    ap.add_argument("--warmup", type=int, default=None, help="default 2 batched, 1 plain")
+    ap.add_argument("--form", default="batched", choices=["batched", "plain"])
     ap.add_argument("--paths", type=int, default=2000)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 20 if args.form == "batched" else 3
+    if args.warmup is None:
+        args.warmup = 2 if args.form == "batched" else 1
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box in this setup has one MI355X; "
               "run with --gpus 1", file=sys.stderr)
@@ -113,26 +158,27 @@ def main():
     log("scene: %d calls generated in %.1fs" % (len(cmds), time.perf_counter() - t))
     img = px.Image.from_array(canvas)
 
+    if args.form == "batched":
+        t = time.perf_counter()
+        sc = record_scene(px, cmds)
+        log("recorded %d calls in %.2fs" % (len(sc), time.perf_counter() - t))
+        step = lambda: sc.render(img)
+    else:
+        step = lambda: run_plain(px, img, cmds)
     for _ in range(args.warmup):
-        run_plain(px, img, cmds)
+        step()
     px.synchronize()
-    px.touched_pixels(reset=True)
-    px.profile_begin()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        run_plain(px, img, cmds)
-        log("step %d done" % k)
-    px.synchronize()
-    wall = time.perf_counter() - t0
-    kernel_ms, launches = px.profile_end()
-    touched = px.touched_pixels(reset=True)
+    wall, touched, stages = timed(px, step, args.steps)
 
     per_step_touch = touched / args.steps
     ms_per_step = wall / args.steps * 1e3
     value = per_step_touch / (wall / args.steps) / 1e6
     bytes_per_step = 8.0 * per_step_touch
-    kernel_s_per_step = kernel_ms / args.steps / 1e3
-    achieved = bytes_per_step / kernel_s_per_step / 1e9 if kernel_s_per_step > 0 else 0.0
+    # dominant kernel: the stage with the most device time
+    kname, (kms, klaunch) = max(stages.items(), key=lambda kv: kv[1][0])
+    avg_launch_s = kms / klaunch / 1e3
+    bytes_per_launch = bytes_per_step * args.steps / klaunch
+    achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     result = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -151,13 +197,22 @@ def main():
                    "form": args.form, "n_touch_per_step": int(per_step_touch)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "fill_rows_kernel", "launches_per_step": launches // args.steps,
-                     "kernel_ms_per_step": round(kernel_ms / args.steps, 3)},
+                     "kernel": kname, "launches_per_step": klaunch // args.steps,
+                     "avg_launch_ms": round(kms / klaunch, 4),
+                     "stages_ms_per_step": {k: round(v[0] / args.steps, 4)
+                                            for k, v in stages.items()}},
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cmds, canvas, args.cpu_seconds)
     if not args.no_extra:
-        result["extra"] = {"star512": star_extra(px, WL)}
+        extra = {"star512": star_extra(px, WL)}
+        if args.form == "batched":
+            pimg = px.Image.from_array(canvas)
+            pw, pt, pst = timed(px, lambda: run_plain(px, pimg, cmds), 1)
+            extra["plain_form"] = {"mpix_s": round(pt / pw / 1e6, 2), "ms_per_step": round(pw * 1e3, 2),
+                                   "kernel_ms": round(sum(v[0] for v in pst.values()), 3),
+                                   "batched_speedup": round(value / (pt / pw / 1e6), 2)}
+        result["extra"] = extra
     print(json.dumps(result), flush=True)
 
 

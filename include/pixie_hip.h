@@ -75,12 +75,15 @@ void pxh_free(void* p);
 
 /* fillPath (paths.nim:2093-2113) with a solid paint: cmds is a pixie command
  * array, mat9 a vmath Mat3 (column-major, NULL = identity), rgbx the paint
- * colour with opacity applied, premultiplied.  Solid paints with alpha 0 are
- * skipped unless blend is OverwriteBlend (paths.nim:2107). */
+ * colour with opacity applied, premultiplied (the fillShapes argument).  The
+ * caller keeps fillPath's early returns, which test the float colour before
+ * it is converted (opacity == 0; color.a == 0 unless OverwriteBlend,
+ * paths.nim:2101-2107): this function always rasterises. */
 int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
                   uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
 
-/* strokePath (paths.nim:2144-2176) with a solid paint. */
+/* strokePath (paths.nim:2144-2176) with a solid paint; the caller keeps the
+ * early returns of paths.nim:2157-2162 as for pxh_fill_path. */
 int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
                     uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
                     float miter_limit, const float* dashes, int32_t ndashes,
@@ -97,8 +100,9 @@ int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, in
  * arc commands) and renders them in one device pass.  The result is the image
  * the same calls leave when issued one by one in order.  Pixie has no batched
  * entry point; this one replaces a loop over fillPath / strokePath on one
- * image.  Calls that draw nothing (transparent paint that is not
- * OverwriteBlend, stroke width <= 0) are not recorded.  The recording stays
+ * image.  As for pxh_fill_path the caller keeps fillPath's paint checks;
+ * strokes of width <= 0 (no stroke shapes, paths.nim:1931) are not
+ * recorded.  The recording stays
  * resident on the device across renders; scene pools are sized on the first
  * render at a given image size (which waits for the device once) and reused
  * afterwards without host synchronisation. */
@@ -151,6 +155,10 @@ int pxh_event_elapsed(int32_t a, int32_t b, float* ms);
 /* Per-launch timing of raster kernels between begin and end (summed ms). */
 int pxh_profile_begin(void);
 int pxh_profile_end(double* kernel_ms, int64_t* launches);
+/* The share of one kernel (or scene stage) in the last profile: names are
+ * fill_rows_kernel, scene_geometry, scene_bands, scene_rows_kernel,
+ * scene_compose_kernel. */
+int pxh_profile_kernel(const char* name, double* kernel_ms, int64_t* launches);
 
 /* Host geometry of one fillPath (stroke = 0) or strokePath (stroke = 1) call
  * without device work: info16 receives the 16-int fill header (startX,

@@ -85,6 +85,7 @@ def lib():
         L.pxh_touched_pixels.argtypes = [P(ctypes.c_int64), i32]
         L.pxh_event_record.argtypes = [i32]
         L.pxh_profile_end.argtypes = [P(ctypes.c_double), P(ctypes.c_int64)]
+        L.pxh_profile_kernel.argtypes = [ctypes.c_char_p, P(ctypes.c_double), P(ctypes.c_int64)]
         L.pxh_event_elapsed.argtypes = [i32, i32, P(ctypes.c_float)]
         L.pxh_debug_fill_geometry.argtypes = [f32p, i32, f32p, i32, i32, i32, f, i32, i32, f,
                                               i32p, P(i32p), i32p, P(f32p),
@@ -128,6 +129,13 @@ def _cmds(path):
     elif isinstance(path, Path):
         path = path.array()
     return np.ascontiguousarray(path, dtype=np.float32)
+
+
+def _paint_skipped(color, mode):
+    """fillPath / strokePath draw nothing for a transparent solid paint unless
+    it overwrites (paths.nim:2106-2107); colours here are bytes, so the float
+    test color.a > 0 is alpha byte > 0."""
+    return not (int(color[3]) > 0 or mode == BlendMode.Overwrite)
 
 
 def _fp(a):
@@ -203,6 +211,8 @@ class Image:
         _check(lib().pxh_image_fill(self._h, pack(color)))
 
     def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
+        if _paint_skipped(color, blendMode):
+            return
         c = _cmds(path)
         m = _mat(transform)
         _check(lib().pxh_fill_path(self._h, _fp(c), len(c), _fp(m), pack(color), windingRule,
@@ -210,6 +220,8 @@ class Image:
 
     def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
                    lineJoin=MiterJoin, miterLimit=4.0, dashes=(), blendMode=0):
+        if _paint_skipped(color, blendMode):
+            return
         c = _cmds(path)
         m = _mat(transform)
         d = np.ascontiguousarray(dashes, dtype=np.float32) if len(dashes) else None
@@ -271,6 +283,8 @@ class Scene:
         _check(lib().pxh_scene_clear(self._h))
 
     def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
+        if _paint_skipped(color, blendMode):
+            return
         c = _cmds(path)
         m = _mat(transform)
         _check(lib().pxh_scene_fill_path(self._h, _fp(c), len(c), _fp(m), pack(color),
@@ -278,6 +292,8 @@ class Scene:
 
     def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
                    lineJoin=MiterJoin, miterLimit=4.0, blendMode=0):
+        if _paint_skipped(color, blendMode):
+            return
         c = _cmds(path)
         m = _mat(transform)
         _check(lib().pxh_scene_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color),
@@ -326,6 +342,13 @@ def profile_end():
     """(summed kernel ms, launches) since profile_begin()."""
     ms, n = ctypes.c_double(), ctypes.c_int64()
     _check(lib().pxh_profile_end(ctypes.byref(ms), ctypes.byref(n)))
+    return ms.value, n.value
+
+
+def profile_kernel(name):
+    """(ms, launches) of one kernel or scene stage in the last profile."""
+    ms, n = ctypes.c_double(), ctypes.c_int64()
+    _check(lib().pxh_profile_kernel(name.encode(), ctypes.byref(ms), ctypes.byref(n)))
     return ms.value, n.value
 
 

@@ -96,7 +96,12 @@ struct Ctx {
   // per-launch kernel timing (pxh_profile_begin / _end)
   bool profiling = false;
   std::vector<hipEvent_t> evPool;
-  std::vector<std::pair<hipEvent_t, hipEvent_t>> evPairs;
+  struct Timed {
+    const char* name;
+    hipEvent_t a, b;
+  };
+  std::vector<Timed> evPairs;
+  std::map<std::string, std::pair<double, int64_t>> kernelTimes;  // last profile: ms, launches
   std::map<int, uint32_t*> luts;  // gaussian LUT per radius (device, uint32 taps)
 };
 Ctx g;
@@ -171,8 +176,9 @@ hipEvent_t pool_event() {
 
 // Brackets one kernel launch with events when profiling is on.
 struct LaunchTimer {
+  const char* name;
   hipEvent_t a = nullptr, b = nullptr;
-  LaunchTimer() {
+  explicit LaunchTimer(const char* kernel) : name(kernel) {
     if (g.profiling) {
       a = pool_event();
       b = pool_event();
@@ -182,7 +188,7 @@ struct LaunchTimer {
   ~LaunchTimer() {
     if (a) {
       (void)hipEventRecord(b, g.stream);
-      g.evPairs.push_back({a, b});
+      g.evPairs.push_back({name, a, b});
     }
   }
 };
@@ -245,7 +251,7 @@ int raster_fill(pxh_image* img, FillGeom& geo) {
     a.gscratch = g.gscratch.p;
   }
   {
-    LaunchTimer t;
+    LaunchTimer t("fill_rows_kernel");
     if (launch_fill_rows(a, grid, lds, img->fmt == PXH_FORMAT_A8, g.stream) != 0)
       return fail(kErrDevice, std::string("fill launch: ") + hipGetErrorString(hipGetLastError()));
   }
@@ -449,7 +455,6 @@ int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9,
   if (mode < 0 || mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
   if (rule != kNonZero && rule != kEvenOdd) return fail(kErrArg, "invalid winding rule");
   if (!mat9) mat9 = kIdentity;
-  if (!(ch_a(rgbx) > 0 || mode == kOverwrite)) return 0;  // paths.nim:2107 / 2162
   if (stroke && width <= 0.0f) return 0;  // strokeShapes yields nothing (paths.nim:1931-1932)
   SceneItem it = {};
   it.cmdBase = (int32_t)sc->cmds.size();
@@ -585,10 +590,25 @@ int scene_launch(pxh_scene* sc, pxh_image* img) {
   p.W = img->w;
   p.H = img->h;
   PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, g.stream));
-  if (scene_launch_geometry(p, g.stream) || scene_launch_bands(p, g.numCUs * 16, g.stream) ||
-      scene_launch_rows(p, g.numCUs * 8, g.stream) ||
-      scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched, g.numCUs * 16, g.stream))
-    return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
+  int r;
+  {
+    LaunchTimer t("scene_geometry");
+    r = scene_launch_geometry(p, g.stream);
+  }
+  if (!r) {
+    LaunchTimer t("scene_bands");
+    r = scene_launch_bands(p, g.numCUs * 16, g.stream);
+  }
+  if (!r) {
+    LaunchTimer t("scene_rows_kernel");
+    r = scene_launch_rows(p, g.numCUs * 8, g.stream);
+  }
+  if (!r) {
+    LaunchTimer t("scene_compose_kernel");
+    r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched, g.numCUs * 16,
+                             g.stream);
+  }
+  if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
 }
 
@@ -716,7 +736,6 @@ int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float*
   PXH_INIT();
   if (int r = check_image(img)) return r;
   if (!mat9) mat9 = kIdentity;
-  if (!(ch_a(rgbx) > 0 || blend_mode == kOverwrite)) return 0;  // paths.nim:2107
   Shapes sh;
   std::string err;
   int32_t r = commands_to_shapes(cmds, ncmds, true, pixel_scale(mat9), &sh, &err);
@@ -731,7 +750,6 @@ int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const floa
   PXH_INIT();
   if (int r = check_image(img)) return r;
   if (!mat9) mat9 = kIdentity;
-  if (!(ch_a(rgbx) > 0 || blend_mode == kOverwrite)) return 0;  // paths.nim:2162
   float ps = pixel_scale(mat9);
   Shapes sh, st;
   std::string err;
@@ -1032,16 +1050,28 @@ int pxh_profile_end(double* kernel_ms, int64_t* launches) {
   g.profiling = false;
   double tot = 0;
   PXH_HIP(hipStreamSynchronize(g.stream));
+  g.kernelTimes.clear();
   for (auto& pr : g.evPairs) {
     float ms = 0;
-    PXH_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+    PXH_HIP(hipEventElapsedTime(&ms, pr.a, pr.b));
     tot += ms;
-    g.evPool.push_back(pr.first);
-    g.evPool.push_back(pr.second);
+    auto& kt = g.kernelTimes[pr.name];
+    kt.first += ms;
+    kt.second++;
+    g.evPool.push_back(pr.a);
+    g.evPool.push_back(pr.b);
   }
   *kernel_ms = tot;
   *launches = (int64_t)g.evPairs.size();
   g.evPairs.clear();
+  return 0;
+}
+
+// Per-kernel share of the last profile (after pxh_profile_end).
+int pxh_profile_kernel(const char* name, double* kernel_ms, int64_t* launches) {
+  auto it = g.kernelTimes.find(name ? name : "");
+  *kernel_ms = it == g.kernelTimes.end() ? 0.0 : it->second.first;
+  *launches = it == g.kernelTimes.end() ? 0 : it->second.second;
   return 0;
 }
 

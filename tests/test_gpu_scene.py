@@ -249,7 +249,7 @@ def test_scene_a8(gpu, orc):
     want = orc.new_image(300, 260)
     for i in range(10):
         p = WL.glyph_path(rng, 200, 200)
-        mode = 0 if i != 7 else 17  # one subtract-mask call
+        mode = 0 if i != 7 else 18  # one SubtractMask call
         sc.fillPath(p, (255, 255, 255, 255), blendMode=mode)
         ref.fillPath(p, (255, 255, 255, 255), blendMode=mode)
         orc.fill_path(want, p.array(), (255, 255, 255, 255), mode=mode)
