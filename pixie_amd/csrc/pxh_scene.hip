@@ -622,10 +622,29 @@ __global__ void __launch_bounds__(kBlock) scene_entry_scan_kernel(ScenePools P) 
   }
 }
 
+// Entry of segment s in band [top, bot]: slope-intercept form of the whole
+// segment, y range clipped to the band when the segment spans it
+// (partitionSegments 1236-1248).  *clipped receives the clipped segment.
+__device__ __forceinline__ Entry band_entry(float4 s, float top, float bot, float4* clipped) {
+  V2 at = v2(s.x, s.y), to = v2(s.z, s.w);
+  Entry e = init_entry(at, to);
+  if (at.y <= top && to.y >= bot) {
+    V2 c = v2(0, 0);
+    clip_to_line(at, to, top, &c);
+    at = c;
+    clip_to_line(at, to, bot, &c);
+    to = c;
+    e.aty = at.y;
+    e.toy = to.y;
+  }
+  *clipped = make_float4(at.x, at.y, to.x, to.y);
+  return e;
+}
+
 // One wave per band: ordered entries, clipping, two-spanning flag
-// (partitionSegments 1194-1262).
+// (partitionSegments 1194-1262).  Lanes exchange data only through ballots
+// and shuffles, which every lane executes.
 __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
-  __shared__ float4 sTwo[2];
   const int lane = threadIdx.x;
   const int nb = P.counters[kCntBands];
   if (nb > P.bandCap || P.counters[kCntEntries] > P.entryCap) return;
@@ -634,58 +653,79 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
     const ItemGeom g = P.geo[it];
     const FillInfo f = P.fills[it];
     const int k = b - g.bandBase;
-    PartInfo pi = P.parts[b];
+    const PartInfo pi = P.parts[b];
     float top, bot;
     band_rows(f, k, &top, &bot);
-    int pos = 0;
-    for (int j0 = 0; j0 < g.segCount; j0 += 64) {
+    if (pi.count != 2) {
+      int pos = 0;
+      for (int j0 = 0; j0 < g.segCount; j0 += 64) {
+        int j = j0 + lane;
+        bool in = false;
+        float4 sg = make_float4(0, 0, 0, 0);
+        if (j < g.segCount) {
+          sg = P.segs[g.segBase + j];
+          in = seg_in_band(sg, f, k);
+        }
+        uint64_t m = __ballot(in);
+        int p = pos + __popcll(m & ((1ull << lane) - 1ull));
+        if (in) {
+          float4 c;
+          P.entries[pi.entryBase + p] = band_entry(sg, top, bot, &c);
+          P.entW[pi.entryBase + p] = P.segW[g.segBase + j];
+        }
+        pos += __popcll(m);
+      }
+      continue;
+    }
+    // two entries: gather both segments into every lane, decide the
+    // two-spanning flag and order (1250-1262), then write
+    float4 s0 = make_float4(0, 0, 0, 0), s1 = s0;
+    int w0 = 0, w1 = 0, found = 0;
+    for (int j0 = 0; j0 < g.segCount && found < 2; j0 += 64) {
       int j = j0 + lane;
       bool in = false;
-      float4 s = make_float4(0, 0, 0, 0);
+      float4 sg = make_float4(0, 0, 0, 0);
+      int wv = 0;
       if (j < g.segCount) {
-        s = P.segs[g.segBase + j];
-        in = seg_in_band(s, f, k);
+        sg = P.segs[g.segBase + j];
+        in = seg_in_band(sg, f, k);
+        wv = P.segW[g.segBase + j];
       }
       uint64_t m = __ballot(in);
-      int p = pos + __popcll(m & ((1ull << lane) - 1ull));
-      if (in) {
-        V2 at = v2(s.x, s.y), to = v2(s.z, s.w);
-        Entry e = init_entry(at, to);
-        if (at.y <= top && to.y >= bot) {
-          V2 c = v2(0, 0);
-          clip_to_line(at, to, top, &c);
-          at = c;
-          clip_to_line(at, to, bot, &c);
-          to = c;
-          e.aty = at.y;
-          e.toy = to.y;
+      while (m != 0 && found < 2) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        float4 t = make_float4(__shfl(sg.x, src, 64), __shfl(sg.y, src, 64), __shfl(sg.z, src, 64),
+                               __shfl(sg.w, src, 64));
+        int tw = __shfl(wv, src, 64);
+        if (found == 0) {
+          s0 = t;
+          w0 = tw;
+        } else {
+          s1 = t;
+          w1 = tw;
         }
-        P.entries[pi.entryBase + p] = e;
-        P.entW[pi.entryBase + p] = P.segW[g.segBase + j];
-        if (p < 2) sTwo[p] = make_float4(at.x, at.y, to.x, to.y);
+        found++;
       }
-      pos += __popcll(m);
     }
-    __syncthreads();
-    if (lane == 0 && pi.count == 2) {
-      float4 a = sTwo[0], c = sTwo[1];
-      V2 a0 = v2(a.x, a.y), a1 = v2(a.z, a.w), c0 = v2(c.x, c.y), c1 = v2(c.z, c.w);
-      if (!intersects_inside(a0, a1, c0, c1)) {
-        if (a0.y <= top && a1.y >= bot && c0.y <= top && c1.y >= bot) {
-          pi.flags |= 2;
-          if (midpoint_x(a0, a1) > midpoint_x(c0, c1)) {
-            Entry t = P.entries[pi.entryBase];
-            P.entries[pi.entryBase] = P.entries[pi.entryBase + 1];
-            P.entries[pi.entryBase + 1] = t;
-            int8_t w = P.entW[pi.entryBase];
-            P.entW[pi.entryBase] = P.entW[pi.entryBase + 1];
-            P.entW[pi.entryBase + 1] = w;
-          }
-        }
+    float4 a, c;
+    Entry e0 = band_entry(s0, top, bot, &a);
+    Entry e1 = band_entry(s1, top, bot, &c);
+    int32_t flags = pi.flags;
+    bool swap = false;
+    V2 a0 = v2(a.x, a.y), a1 = v2(a.z, a.w), c0 = v2(c.x, c.y), c1 = v2(c.z, c.w);
+    if (!intersects_inside(a0, a1, c0, c1)) {
+      if (a0.y <= top && a1.y >= bot && c0.y <= top && c1.y >= bot) {
+        flags |= 2;
+        swap = midpoint_x(a0, a1) > midpoint_x(c0, c1);
       }
-      P.parts[b].flags = pi.flags;
     }
-    __syncthreads();
+    // every lane stores the same values
+    P.entries[pi.entryBase] = swap ? e1 : e0;
+    P.entries[pi.entryBase + 1] = swap ? e0 : e1;
+    P.entW[pi.entryBase] = (int8_t)(swap ? w1 : w0);
+    P.entW[pi.entryBase + 1] = (int8_t)(swap ? w0 : w1);
+    P.parts[b].flags = flags;
   }
 }
 
@@ -702,9 +742,20 @@ __device__ __forceinline__ int find_item(const ItemGeom* geo, int n, int task) {
   return lo;
 }
 
+// Work-queue fetch for a one-wave workgroup: every lane takes part in the
+// atomic (lane 0 adds `v`, the others 0) and lane 0's old value is broadcast
+// with a shuffle.  Broadcasting through LDS from a lane-0-only branch is not
+// safe here: with a one-wave workgroup __syncthreads() compiles to nothing,
+// and the compiler may then let lanes 1..63 loop back to the LDS read
+// without lane 0, spinning on a stale value.
+__device__ __forceinline__ int wave_fetch_add(int32_t* counter, int v) {
+  const int lane = threadIdx.x & 63;
+  int old = atomicAdd(counter, lane == 0 ? v : 0);
+  return __shfl(old, 0, 64);
+}
+
 __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
-  __shared__ int sTask;
   const int lane = threadIdx.x;
   const size_t fixedBytes = row_lds_fixed_bytes(kWinA);
   RowLds Ll = row_lds_carve(smem, smem + fixedBytes, P.ldsCap, kWinA);
@@ -714,10 +765,7 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
   const int ntasks = P.counters[kCntTasks];
   if (P.counters[kCntError]) return;
   for (;;) {
-    if (lane == 0) sTask = atomicAdd(&P.counters[kCntUnitsA], 1);
-    __syncthreads();
-    const int task = sTask;
-    __syncthreads();
+    const int task = wave_fetch_add(&P.counters[kCntUnitsA], 1);
     if (task >= ntasks) break;
     const int it = find_item(P.geo, P.nItems, task);
     const ItemGeom g = P.geo[it];
@@ -748,7 +796,7 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
     rr.kind = kind;
     rr.pad = 0;
     if (kind < 0) {
-      if (lane == 0) atomicOr(&P.counters[kCntError], 8);
+      atomicOr(&P.counters[kCntError], 8);
     } else if (kind != kRowNone) {
       // compact D[0..wlen] (nonzero deltas) and the area factors
       const int wlen = C.wx1 - C.wx0;
@@ -792,12 +840,9 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
         }
         if (pass == 0) {
           total = written;
-          if (lane == 0) sTask = atomicAdd(&P.counters[kCntRuns], total);
-          __syncthreads();
-          rr.off = sTask;
-          __syncthreads();
+          rr.off = wave_fetch_add(&P.counters[kCntRuns], total);
           if (rr.off + total > P.runCap) {
-            if (lane == 0) atomicOr(&P.counters[kCntError], 16);
+            atomicOr(&P.counters[kCntError], 16);
             for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
             total = 0;
             break;
@@ -807,7 +852,7 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
       rr.n = total;
       __syncthreads();
     }
-    if (lane == 0) P.tasks[task] = rr;
+    P.tasks[task] = rr;  // every lane stores the same value
   }
 }
 
@@ -819,7 +864,6 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
                                                           unsigned long long* touchedOut) {
   __shared__ int32_t D[kWinB + 4];
   __shared__ uint32_t AX[kWinB];
-  __shared__ int sUnit;
   const int lane = threadIdx.x;
   for (int i = lane; i < kWinB + 4; i += 64) D[i] = 0;
   const int W = P.W, H = P.H;
@@ -828,10 +872,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
   uint32_t ntouch = 0;
   if (P.counters[kCntError]) return;
   for (;;) {
-    __syncthreads();
-    if (lane == 0) sUnit = atomicAdd(&P.counters[kCntUnitsB], 1);
-    __syncthreads();
-    const int unit = sUnit;
+    const int unit = wave_fetch_add(&P.counters[kCntUnitsB], 1);
     if (unit >= nUnits) break;
     const int y = unit / winsPerRow;
     const int b0 = (unit % winsPerRow) * kWinB;
@@ -956,8 +997,6 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
             }
           }
         }
-        // end markers of runs that stop at the window edge live at index b1-b0
-        if (lane == 0) D[kWinB] = 0;
         __syncthreads();
       }
     }

@@ -192,7 +192,7 @@ def test_scene_geometry_matches_host(gpu, k):
         dinfo, dparts, dents, dwind = sc.debug_geometry(i)
         what = "path %d call %d" % (k, i)
         assert st[i] == 0, what
-        keep = [j for j in range(16) if j not in (6, 15)]  # partBase, pad
+        keep = [0, 1, 2, 3, 4, 5, 10, 11, 12, 13, 14]  # bounds, bands, clip, status, size
         assert np.array_equal(dinfo[keep], info[keep]), (what, dinfo, info)
         if info[2] <= 0:
             continue
