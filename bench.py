@@ -80,7 +80,7 @@ def timed(px, fn, steps):
     return wall, touched, stages
 
 
-STAGES = ("fill_rows_kernel", "scene_geometry", "scene_bands", "scene_rows_kernel",
+STAGES = ("fill_rows_kernel", "scene_geometry", "scene_bands", "scene_bins", "scene_rows_kernel",
           "scene_compose_kernel")
 
 

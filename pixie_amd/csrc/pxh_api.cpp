@@ -37,7 +37,8 @@ struct pxh_scene {
   char* dMem = nullptr;
   size_t dMemBytes = 0;
   pxh::ScenePools pools{};
-  int32_t ptCap = 0, segCap = 0, bandCap = 0, entryCap = 0, taskCap = 0, runCap = 0;
+  int32_t ptCap = 0, segCap = 0, bandCap = 0, entryCap = 0, taskCap = 0, runCap = 0, binCap = 0;
+  int32_t binSlots = 0;  // row bins the pools hold (height / kBinRows)
 };
 
 namespace pxh {
@@ -332,7 +333,7 @@ int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius
 // ---- scene (batched form) ----
 
 const int kSceneGcap = 8192;  // band entries a wave can hold in global scratch
-const int kSceneLdsCap = 256; // band entries held in LDS
+const int kSceneLdsCap = 128; // band entries held in LDS (7 KB of LDS per row wave)
 
 // Converts pixie commands to the device's canonical absolute form
 // (commandsToShapes' command semantics, paths.nim:886-1057): relative,
@@ -515,8 +516,12 @@ int scene_alloc(pxh_scene* sc) {
   size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
   size_t oEnt = carve((size_t)sc->entryCap * sizeof(Entry)), oEW = carve((size_t)sc->entryCap);
   size_t oTask = carve((size_t)sc->taskCap * sizeof(RowRuns)), oRun = carve((size_t)sc->runCap * 8);
+  size_t oTaskItem = carve((size_t)sc->taskCap * 4);
+  size_t oRows = carve(nI * sizeof(int4)), oBinC = carve((size_t)sc->binSlots * 4);
+  size_t oBinO = carve(((size_t)sc->binSlots + 1) * 4), oBinL = carve((size_t)sc->binCap * 4);
+  size_t oQ = carve(2 * kQueues * kQueueStride * 4);
   size_t oCnt = carve(kNumCounters * 4);
-  const int gridRows = g.numCUs * 8;
+  const int gridRows = scene_queue_grid(g.numCUs * kSceneRowWavesPerCU);
   size_t oScr = carve(scene_gscratch_bytes(gridRows, kSceneGcap));
   if (off > sc->dMemBytes) {
     PXH_HIP(hipStreamSynchronize(g.stream));
@@ -550,9 +555,17 @@ int scene_alloc(pxh_scene* sc) {
   p.entW = (int8_t*)(b + oEW);
   p.entryCap = sc->entryCap;
   p.tasks = (RowRuns*)(b + oTask);
+  p.taskItem = (int32_t*)(b + oTaskItem);
   p.taskCap = sc->taskCap;
   p.runs = (int2*)(b + oRun);
   p.runCap = sc->runCap;
+  p.itemRows = (int4*)(b + oRows);
+  p.binCount = (int32_t*)(b + oBinC);
+  p.binOff = (int32_t*)(b + oBinO);
+  p.binList = (int32_t*)(b + oBinL);
+  p.binCap = sc->binCap;
+  p.binSlots = sc->binSlots;
+  p.queues = (int32_t*)(b + oQ);
   p.counters = (int32_t*)(b + oCnt);
   p.gscratch = b + oScr;
   p.gcap = kSceneGcap;
@@ -580,7 +593,11 @@ int scene_upload(pxh_scene* sc) {
   sc->bandCap = (int32_t)std::min<size_t>(64 * sc->items.size() + 4096, 1u << 26);
   sc->entryCap = sc->segCap * 2;
   sc->taskCap = (int32_t)std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28);
-  sc->runCap = sc->taskCap * 16;
+  sc->runCap = (int32_t)std::min<int64_t>(
+      (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kSceneRowWavesPerCU) * kRunChunk,
+      (int64_t)1 << 30);
+  sc->binCap = (int32_t)std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28);
+  sc->binSlots = 0;  // sized by the first render
   sc->uploaded = true;
   return scene_alloc(sc);
 }
@@ -590,6 +607,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img) {
   p.W = img->w;
   p.H = img->h;
   PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, g.stream));
+  PXH_HIP(hipMemsetAsync(p.queues, 0, 2 * kQueues * kQueueStride * 4, g.stream));
   int r;
   {
     LaunchTimer t("scene_geometry");
@@ -600,12 +618,17 @@ int scene_launch(pxh_scene* sc, pxh_image* img) {
     r = scene_launch_bands(p, g.numCUs * 16, g.stream);
   }
   if (!r) {
+    LaunchTimer t("scene_bins");
+    r = scene_launch_bins(p, g.numCUs * 4, g.stream);
+  }
+  if (!r) {
     LaunchTimer t("scene_rows_kernel");
-    r = scene_launch_rows(p, g.numCUs * 8, g.stream);
+    r = scene_launch_rows(p, scene_queue_grid(g.numCUs * kSceneRowWavesPerCU), g.stream);
   }
   if (!r) {
     LaunchTimer t("scene_compose_kernel");
-    r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched, g.numCUs * 16,
+    r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched,
+                             scene_queue_grid(g.numCUs * 16),
                              g.stream);
   }
   if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
@@ -943,6 +966,12 @@ int pxh_scene_render(pxh_scene* sc, pxh_image* img) {
   if (int r = check_image(img)) return r;
   if (sc->items.empty()) return 0;
   if (int r = scene_upload(sc)) return r;
+  const int32_t nBins = (img->h + kBinRows - 1) / kBinRows;
+  if (nBins > sc->binSlots) {  // row bins for a taller image
+    sc->binSlots = nBins;
+    sc->validW = sc->validH = -1;
+    if (int r = scene_alloc(sc)) return r;
+  }
   if (sc->validW == img->w && sc->validH == img->h) return scene_launch(sc, img);
   // First render at this size: check the pools and grow them until the render
   // fits.  A render that overflows a pool stops before touching the canvas.
@@ -951,7 +980,15 @@ int pxh_scene_render(pxh_scene* sc, pxh_image* img) {
     int32_t cnt[kNumCounters];
     PXH_HIP(hipMemcpyAsync(cnt, sc->pools.counters, sizeof(cnt), hipMemcpyDeviceToHost, g.stream));
     PXH_HIP(hipStreamSynchronize(g.stream));
-    if (!cnt[kCntError]) {
+    // the run pool must also hold every wave's partly used chunk, whatever
+    // order the waves take tasks in on later renders
+    const int64_t runNeed =
+        (int64_t)cnt[kCntRunsUsed] + (int64_t)scene_queue_grid(g.numCUs * kSceneRowWavesPerCU) * kRunChunk;
+    if (!cnt[kCntError]) {  // this render is complete; later ones get the slack
+      if (runNeed > sc->runCap) {
+        sc->runCap = (int32_t)std::min<int64_t>(runNeed + runNeed / 4, (int64_t)1 << 30);
+        if (int r = scene_alloc(sc)) return r;
+      }
       sc->validW = img->w;
       sc->validH = img->h;
       return 0;
@@ -966,7 +1003,8 @@ int pxh_scene_render(pxh_scene* sc, pxh_image* img) {
     sc->bandCap = grow(sc->bandCap, cnt[kCntBands]);
     sc->entryCap = grow(sc->entryCap, cnt[kCntEntries]);
     sc->taskCap = grow(sc->taskCap, cnt[kCntTasks]);
-    sc->runCap = grow(sc->runCap, cnt[kCntRuns]);
+    if ((cnt[kCntError] & 16) || runNeed > sc->runCap) sc->runCap = grow(sc->runCap, runNeed);
+    sc->binCap = grow(sc->binCap, cnt[kCntBinEntries]);
     if (int r = scene_alloc(sc)) return r;
   }
   return fail(kErrCapacity, "scene pools could not be sized");

@@ -285,10 +285,19 @@ struct FillSpan {
   }
 };
 
+// The row's selected entries.  When the selection fits in one wave (the
+// common case), lane i keeps selection i in registers for all sample lines;
+// otherwise entries are re-read through L.A.
+struct SelCache {
+  bool inReg;
+  Entry e;
+  int32_t w;
+};
+
 // Collects the hits of one sample line (paths.nim:1373-1388) from the
 // selection L.A[0..ne) into L.keys sorted stably by x; windings go to L.B by
 // hit position.  Returns the number of hits.
-__device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine) {
+__device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, const SelCache& S) {
   int lane = lane_id();
   const float Wf = (float)C.F->canvasW;
   int nh = 0;
@@ -297,12 +306,20 @@ __device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine) {
     bool hit = false;
     int32_t at = 0, w = 0;
     if (i < ne) {
-      int k = L.A[i];
-      Entry e = C.E[k];
+      Entry e;
+      int32_t ew;
+      if (S.inReg) {
+        e = S.e;
+        ew = S.w;
+      } else {
+        int k = L.A[i];
+        e = C.E[k];
+        ew = C.EW[k];
+      }
       if (e.aty <= yLine && e.toy >= yLine) {
         float x = e.m == 0.0f ? e.b : (yLine - e.b) / e.m;
         at = fixed32(nim_minf(x, Wf));
-        w = C.EW[k];
+        w = ew;
         hit = true;
       }
     }
@@ -387,10 +404,19 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     }
   }
   wave_sync();
+  SelCache S;
+  S.inReg = ne <= kWave;
+  S.e = Entry{0.0f, 0.0f, 0.0f, 0.0f};
+  S.w = 0;
+  if (S.inReg && lane < ne) {
+    int k = L.A[lane];
+    S.e = E[k];
+    S.w = EW[k];
+  }
 
   if (allSpan && (ne % 2) == 0) {  // trapezoid fast path, 1691-1872
     for (int i = lane; i < ne; i += kWave) {
-      Entry e = E[L.A[i]];
+      Entry e = S.inReg ? S.e : E[L.A[i]];
       float a = solve_x(e.m, e.b, scanTop), b = solve_x(e.m, e.b, scanBottom);
       L.tx0[i] = a;
       L.tx1[i] = b;
@@ -523,14 +549,14 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     CovSpan cs{&C, &L, sampleCoverage, (int32_t)covLo, (int32_t)covHi};
     for (int m = 0; m < quality; m++) {
       yLine += offset;
-      int nh = row_hits(Cp, L, ne, yLine);
+      int nh = row_hits(Cp, L, ne, yLine, S);
       wave_walk(L, nh, F.rule, cs);
       wave_sync();
     }
     return kRowCoverage;
   }
   yLine += offset;
-  int nh = row_hits(Cp, L, ne, yLine);
+  int nh = row_hits(Cp, L, ne, yLine, S);
   FillSpan fs{&C, &L, mask ? kFill - kClear : kFill, mask ? (int64_t)F.startX : (int64_t)INT64_MIN};
   wave_walk(L, nh, F.rule, fs);
   wave_sync();

@@ -75,10 +75,35 @@ struct RowRuns {
 constexpr int32_t kRunArea = 1 << 30;
 constexpr int32_t kRunXMask = (1 << 28) - 1;
 
+// Phase-A waves take run-pool space in chunks of kRunChunk entries (more
+// than one task of a 512-px window can write: 513 deltas + 512 area
+// factors).  A pool of (runs written) + (waves x kRunChunk) entries is
+// always enough, whatever order the waves take tasks in.
+constexpr int kRunChunk = 2048;
+
 enum SceneCounter {
-  kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks, kCntRuns, kCntUnitsA,
-  kCntUnitsB, kCntError, kCntMaxBand, kNumCounters
+  kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks,
+  kCntRuns,      // run-pool entries handed out in chunks
+  kCntRunsUsed,  // run entries written
+  kCntError, kCntMaxBand, kCntBinEntries, kNumCounters
 };
+
+// Phase B looks calls up by row bins of kBinRows canvas rows: each bin lists,
+// in call order, the calls whose rows meet it.
+constexpr int kBinRows = 16;
+
+// Phase A and B deal out work through kQueues independent queues (queue q
+// serves work items q, q + kQueues, ...), so that no single counter takes
+// every fetch.
+constexpr int kQueues = 16;
+constexpr int kQueueStride = 64;  // ints between queue heads (256 B)
+
+// Grid of a queue-fed kernel: at least one wave per queue, a multiple of
+// kQueues so that every queue is served by the same number of waves.
+inline int scene_queue_grid(int waves) {
+  if (waves < kQueues) waves = kQueues;
+  return (waves + kQueues - 1) / kQueues * kQueues;
+}
 
 struct ScenePools {
   // recorded inputs
@@ -109,9 +134,17 @@ struct ScenePools {
   int8_t* entW;
   int32_t entryCap;
   RowRuns* tasks;
+  int32_t* taskItem;         // call of each task
   int32_t taskCap;
   int2* runs;
   int32_t runCap;
+  int4* itemRows;            // per call: first row, end row, x range (empty if it draws nothing)
+  int32_t* binCount;         // nBins
+  int32_t* binOff;           // nBins + 1
+  int32_t* binList;          // call indices per bin
+  int32_t binCap;
+  int32_t binSlots;          // capacity of binCount / binOff (bins)
+  int32_t* queues;           // work-queue heads: 2 phases x kQueues, one per 256 B
   int32_t* counters;         // SceneCounter
   // row scratch for bands larger than the LDS capacity
   char* gscratch;
@@ -119,10 +152,15 @@ struct ScenePools {
   int32_t ldsCap;            // entries that fit in LDS
 };
 
+// Phase-A row waves per CU: the row program is latency-bound, so the grid
+// keeps as many waves resident as LDS (about 7 KB each) and registers allow.
+constexpr int kSceneRowWavesPerCU = 20;
+
 int scene_set_lds_limits(size_t bytes);
 size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows
 int scene_launch_geometry(const ScenePools& p, hipStream_t s);
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);
+int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
                          int grid, hipStream_t s);

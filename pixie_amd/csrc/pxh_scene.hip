@@ -12,7 +12,7 @@
 namespace pxh {
 
 constexpr int kBlock = 256;
-constexpr int kWinA = 1024;   // phase-A column window
+constexpr int kWinA = 512;    // phase-A column window
 constexpr int kWinB = 1024;   // phase-B column window (4 chunks of 256 px)
 constexpr int kChunksB = kWinB / 256;
 
@@ -510,6 +510,7 @@ __global__ void __launch_bounds__(kBlock) scene_plan_kernel(ScenePools P) {
     int nb = 0, nt = 0;
     ItemGeom g;
     FillInfo f;
+    int4 rows = make_int4(0, 0, 0, 0);
     if (it < P.nItems) {
       g = P.geo[it];
       f = P.fills[it];
@@ -526,7 +527,11 @@ __global__ void __launch_bounds__(kBlock) scene_plan_kernel(ScenePools P) {
         g.winBase = lo;
         g.nWin = hi > lo ? (hi - lo + kWinA - 1) / kWinA : 0;
         nt = g.rowCount * g.nWin;
+        // a mask call touches every row (it clears the rows outside it)
+        rows = f.mode == kMask ? make_int4(0, P.H, 0, P.W)
+                               : make_int4(g.rowStart, g.rowStart + g.rowCount, f.xlo, f.xhi);
       }
+      P.itemRows[it] = rows;
     }
     int tb, tt;
     int bOff = block_excl_scan(nb, &tb, sh);
@@ -730,28 +735,85 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
 }
 
 // ---------------------------------------------------------------------------
-// phase A: row programs of every (call, row, window) -> sparse runs
+// row bins for phase B: one wave per bin, calls compacted in call order
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int find_item(const ItemGeom* geo, int n, int task) {
-  int lo = 0, hi = n - 1;  // last item with taskBase <= task
-  while (lo < hi) {
-    int mid = (lo + hi + 1) >> 1;
-    if (geo[mid].taskBase <= task) lo = mid;
-    else hi = mid - 1;
+template <bool kWrite>
+__global__ void __launch_bounds__(64) scene_bin_kernel(ScenePools P) {
+  const int lane = threadIdx.x;
+  const int nBins = (P.H + kBinRows - 1) / kBinRows;
+  if (kWrite && P.counters[kCntBinEntries] > P.binCap) return;
+  for (int bin = blockIdx.x; bin < nBins; bin += gridDim.x) {
+    const int r0 = bin * kBinRows, r1 = r0 + kBinRows;
+    int pos = kWrite ? P.binOff[bin] : 0;
+    for (int i0 = 0; i0 < P.nItems; i0 += 64) {
+      const int it = i0 + lane;
+      bool in = false;
+      if (it < P.nItems) {
+        int4 r = P.itemRows[it];
+        in = r.x < r1 && r.y > r0;
+      }
+      uint64_t m = __ballot(in);
+      if (kWrite && in) P.binList[pos + __popcll(m & ((1ull << lane) - 1ull))] = it;
+      pos += __popcll(m);
+    }
+    if (!kWrite) P.binCount[bin] = pos;  // every lane stores the same value
   }
-  return lo;
 }
 
-// Work-queue fetch for a one-wave workgroup: every lane takes part in the
-// atomic (lane 0 adds `v`, the others 0) and lane 0's old value is broadcast
-// with a shuffle.  Broadcasting through LDS from a lane-0-only branch is not
-// safe here: with a one-wave workgroup __syncthreads() compiles to nothing,
-// and the compiler may then let lanes 1..63 loop back to the LDS read
-// without lane 0, spinning on a stale value.
-__device__ __forceinline__ int wave_fetch_add(int32_t* counter, int v) {
+__global__ void __launch_bounds__(kBlock) scene_bin_scan_kernel(ScenePools P) {
+  __shared__ int sh[8];
+  const int nBins = (P.H + kBinRows - 1) / kBinRows;
+  int tot = 0;
+  for (int b0 = 0; b0 < nBins; b0 += kBlock) {
+    int b = b0 + threadIdx.x;
+    int c = b < nBins ? P.binCount[b] : 0;
+    int t;
+    int off = block_excl_scan(c, &t, sh);
+    if (b < nBins) P.binOff[b] = tot + off;
+    tot += t;
+  }
+  if (threadIdx.x == 0) {
+    P.binOff[nBins] = tot;
+    P.counters[kCntBinEntries] = tot;
+    if (tot > P.binCap) atomicOr(&P.counters[kCntError], 32);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phase A: row programs of every (call, row, window) -> sparse runs
+// ---------------------------------------------------------------------------
+
+// Next work item of this wave's queue, or a value >= n when it is empty.
+// Every lane takes part in the atomic (lane 0 adds 1, the others 0) and lane
+// 0's old value is broadcast with a shuffle: broadcasting it through LDS
+// from a lane-0-only branch is not safe, because __syncthreads() compiles to
+// nothing in a one-wave workgroup and the compiler may then let lanes 1..63
+// loop back to the LDS read without lane 0, spinning on a stale value.
+__device__ __forceinline__ int queue_next(int32_t* heads, int n) {
   const int lane = threadIdx.x & 63;
-  int old = atomicAdd(counter, lane == 0 ? v : 0);
-  return __shfl(old, 0, 64);
+  const int q = (int)(blockIdx.x % kQueues);
+  int k = atomicAdd(heads + q * kQueueStride, lane == 0 ? 1 : 0);
+  k = __shfl(k, 0, 64);
+  return k < (n + kQueues - 1) / kQueues ? q + kQueues * k : n;
+}
+
+// Takes n entries of a pool counter for the whole wave (same convergent
+// form as queue_next).
+__device__ __forceinline__ int queue_take(int32_t* counter, int n) {
+  const int lane = threadIdx.x & 63;
+  int base = atomicAdd(counter, lane == 0 ? n : 0);
+  return __shfl(base, 0, 64);
+}
+
+// Call of every phase-A task (one wave per call writes its task range).
+__global__ void __launch_bounds__(64) scene_task_map_kernel(ScenePools P) {
+  const int lane = threadIdx.x;
+  if (P.counters[kCntError]) return;
+  for (int it = blockIdx.x; it < P.nItems; it += gridDim.x) {
+    const ItemGeom g = P.geo[it];
+    const int nt = g.rowCount * g.nWin;
+    for (int t = lane; t < nt; t += 64) P.taskItem[g.taskBase + t] = it;
+  }
 }
 
 __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
@@ -764,10 +826,13 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
   __syncthreads();
   const int ntasks = P.counters[kCntTasks];
   if (P.counters[kCntError]) return;
+  // Each wave appends its runs to chunks of the run pool it takes one at a
+  // time (one atomic per chunk, not per task).
+  int cursor = 0, chunkEnd = 0, used = 0;
   for (;;) {
-    const int task = wave_fetch_add(&P.counters[kCntUnitsA], 1);
+    const int task = queue_next(P.queues, ntasks);
     if (task >= ntasks) break;
-    const int it = find_item(P.geo, P.nItems, task);
+    const int it = P.taskItem[task];
     const ItemGeom g = P.geo[it];
     const FillInfo& F = P.fills[it];
     const int local = task - g.taskBase;
@@ -840,8 +905,12 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
         }
         if (pass == 0) {
           total = written;
-          rr.off = wave_fetch_add(&P.counters[kCntRuns], total);
-          if (rr.off + total > P.runCap) {
+          if (cursor + total > chunkEnd) {
+            cursor = queue_take(&P.counters[kCntRuns], kRunChunk);
+            chunkEnd = cursor + kRunChunk;
+          }
+          rr.off = cursor;
+          if (chunkEnd > P.runCap) {
             atomicOr(&P.counters[kCntError], 16);
             for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
             total = 0;
@@ -850,10 +919,13 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
         }
       }
       rr.n = total;
+      cursor += total;
+      used += total;
       __syncthreads();
     }
     P.tasks[task] = rr;  // every lane stores the same value
   }
+  if (threadIdx.x == 0 && used) atomicAdd(&P.counters[kCntRunsUsed], used);
 }
 
 // ---------------------------------------------------------------------------
@@ -872,7 +944,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
   uint32_t ntouch = 0;
   if (P.counters[kCntError]) return;
   for (;;) {
-    const int unit = wave_fetch_add(&P.counters[kCntUnitsB], 1);
+    const int unit = queue_next(P.queues + kQueues * kQueueStride, nUnits);
     if (unit >= nUnits) break;
     const int y = unit / winsPerRow;
     const int b0 = (unit % winsPerRow) * kWinB;
@@ -880,20 +952,20 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
     const bool vec = ((W & 3) == 0);
     uint32_t px[kChunksB][4];
     bool loaded = false, dirty = false;
-    for (int i0 = 0; i0 < P.nItems; i0 += 64) {
-      int it = i0 + lane;
+    const int bin = y / kBinRows;
+    const int l1 = P.binOff[bin + 1];
+    for (int i0 = P.binOff[bin]; i0 < l1; i0 += 64) {
+      const int li = i0 + lane;
+      int it = 0;
       bool cov = false;
-      if (it < P.nItems) {
-        const FillInfo& f = P.fills[it];
-        const ItemGeom& g = P.geo[it];
-        if (f.status == kOk && f.pathWidth > 0) {
-          if (f.mode == kMask) cov = true;
-          else cov = y >= g.rowStart && y < g.rowStart + g.rowCount && f.xhi > b0 && f.xlo < b1;
-        }
+      if (li < l1) {
+        it = P.binList[li];
+        int4 r = P.itemRows[it];
+        cov = y >= r.x && y < r.y && r.w > b0 && r.z < b1;
       }
       uint64_t m = __ballot(cov);
       while (m) {
-        const int it2 = i0 + __builtin_ctzll(m);
+        const int it2 = __shfl(it, __builtin_ctzll(m), 64);
         m &= m - 1;
         const FillInfo f = P.fills[it2];
         const ItemGeom g = P.geo[it2];
@@ -1057,6 +1129,14 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
   hipLaunchKernelGGL(scene_band_count_kernel, dim3(grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kBlock), 0, s, p);
   hipLaunchKernelGGL(scene_band_write_kernel, dim3(grid), dim3(64), 0, s, p);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(scene_bin_kernel<false>, dim3(grid), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(scene_bin_scan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_bin_kernel<true>, dim3(grid), dim3(64), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 
