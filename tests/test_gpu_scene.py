@@ -323,3 +323,42 @@ def test_scene_band_over_capacity_is_an_error(gpu):
         sc.render(img)
     assert e.value.code == 9
     assert not img.download().any()
+
+
+def test_scene_repeated_renders_are_identical(gpu):
+    # phase-A waves take tasks from work queues in a different order on every
+    # render; every render must still fit the pools and give the same image
+    cmds, canvas = WL.mixed_scene(seed=7, n_paths=150, W=640, H=480)
+    ref = px.Image.from_array(canvas)
+    sc = px.Scene()
+    for c in cmds:
+        if c.stroke:
+            ref.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                           miterLimit=c.miter, blendMode=c.mode)
+            sc.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                          miterLimit=c.miter, blendMode=c.mode)
+        else:
+            ref.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+            sc.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    px.synchronize()
+    want = ref.download()
+    touches = []
+    for k in range(8):
+        img = px.Image.from_array(canvas)
+        px.touched_pixels(reset=True)
+        sc.render(img)
+        _exact(img.download(), want, "render %d" % k)
+        touches.append(px.touched_pixels(reset=True))
+    assert len(set(touches)) == 1, touches
+    # renders accumulate on the same image exactly like repeated calls
+    img = px.Image.from_array(canvas)
+    sc.render(img)
+    sc.render(img)
+    ref2 = px.Image.from_array(want)
+    for c in cmds:
+        if c.stroke:
+            ref2.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                            miterLimit=c.miter, blendMode=c.mode)
+        else:
+            ref2.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    _exact(img.download(), ref2.download(), "two renders")
