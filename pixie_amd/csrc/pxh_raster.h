@@ -67,37 +67,52 @@ __device__ __forceinline__ int lane_id() { return threadIdx.x & (kWave - 1); }
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+// Wave-wide scans with DPP lane moves (GFX9 row_shr / row_bcast): a
+// Hillis-Steele scan inside each 16-lane row, then rows 1..3 pick up the
+// totals of the rows before them.  Lanes a move has no source for keep the
+// identity.  No LDS traffic, about two VALU operations per step.
+constexpr int kDppRowShr1 = 0x111, kDppRowShr2 = 0x112, kDppRowShr4 = 0x114,
+              kDppRowShr8 = 0x118, kDppRowBcast15 = 0x142, kDppRowBcast31 = 0x143,
+              kDppWaveShr1 = 0x138;
+
+__device__ __forceinline__ int32_t wave_incl_sum(int32_t v) {
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr8, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowBcast15, 0xa, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowBcast31, 0xc, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ int32_t imax(int32_t a, int32_t b) { return a > b ? a : b; }
+
+__device__ __forceinline__ int32_t wave_incl_max(int32_t v) {
+  const int32_t lo = INT32_MIN;
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr1, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr2, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr4, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr8, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowBcast15, 0xa, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowBcast31, 0xc, 0xf, false));
+  return v;
+}
+
+// Exclusive prefix sum over the wave; *total receives the wave's sum.
 __device__ __forceinline__ int32_t wave_excl_scan(int32_t v, int32_t* total) {
-  int lane = lane_id();
-  int32_t x = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    int32_t y = __shfl_up(x, o, kWave);
-    if (lane >= o) x += y;
-  }
-  *total = __shfl(x, kWave - 1, kWave);
+  int32_t x = wave_incl_sum(v);
+  *total = __builtin_amdgcn_readlane(x, kWave - 1);
   return x - v;
 }
 
+// Exclusive prefix max (INT32_MIN in lane 0).
 __device__ __forceinline__ int32_t wave_excl_max(int32_t v) {
-  int lane = lane_id();
-  int32_t x = v;
-#pragma unroll
-  for (int o = 1; o < kWave; o <<= 1) {
-    int32_t y = __shfl_up(x, o, kWave);
-    if (lane >= o) x = x > y ? x : y;
-  }
-  int32_t e = __shfl_up(x, 1, kWave);
-  return lane == 0 ? INT32_MIN : e;
+  int32_t x = wave_incl_max(v);
+  return __builtin_amdgcn_update_dpp(INT32_MIN, x, kDppWaveShr1, 0xf, 0xf, false);
 }
 
 __device__ __forceinline__ int32_t wave_max(int32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    int32_t y = __shfl_xor(v, o, kWave);
-    v = v > y ? v : y;
-  }
-  return v;
+  return __builtin_amdgcn_readlane(wave_incl_max(v), kWave - 1);
 }
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
@@ -109,6 +124,22 @@ __device__ void wave_sort_keys(uint64_t* keys, int n, int cap) {
   int np = 64;
   while (np < n) np <<= 1;
   if (np > cap) np = cap;  // n <= cap guaranteed by the caller
+  if (n <= 32) {
+    // rank sort: keys are distinct (their low word is the position), so each
+    // lane's rank is the number of smaller keys, found by broadcasting every
+    // key in turn
+    uint64_t k = lane < n ? keys[lane] : ~0ull;
+    uint32_t klo = (uint32_t)k, khi = (uint32_t)(k >> 32);
+    int rank = 0;
+    for (int j = 0; j < n; j++) {
+      uint32_t jlo = __builtin_amdgcn_readlane(klo, j), jhi = __builtin_amdgcn_readlane(khi, j);
+      rank += (jhi < khi || (jhi == khi && jlo < klo)) ? 1 : 0;
+    }
+    wave_sync();
+    if (lane < n) keys[rank] = k;
+    wave_sync();
+    return;
+  }
   for (int i = n + lane; i < np; i += kWave) keys[i] = ~0ull;
   wave_sync();
   if (np == 64) {
