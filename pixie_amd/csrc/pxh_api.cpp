@@ -513,6 +513,7 @@ int scene_alloc(pxh_scene* sc) {
   size_t oCC = carve(nC * 4), oCO = carve(nC * 4), oPP = carve(nP * 4), oPN = carve(nP * 4);
   size_t oPts = carve((size_t)sc->ptCap * 8), oPtP = carve((size_t)sc->ptCap * 4);
   size_t oSeg = carve((size_t)sc->segCap * 16), oSegW = carve((size_t)sc->segCap);
+  size_t oSegB = carve((size_t)sc->segCap * 8), oSegC = carve(((size_t)sc->segCap / 64 + 1) * 8);
   size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
   size_t oEnt = carve((size_t)sc->entryCap * sizeof(Entry)), oEW = carve((size_t)sc->entryCap);
   size_t oTask = carve((size_t)sc->taskCap * sizeof(RowRuns)), oRun = carve((size_t)sc->runCap * 8);
@@ -547,6 +548,8 @@ int scene_alloc(pxh_scene* sc) {
   p.ptCap = sc->ptCap;
   p.segs = (float4*)(b + oSeg);
   p.segW = (int8_t*)(b + oSegW);
+  p.segBand = (int2*)(b + oSegB);
+  p.segChunk = (int2*)(b + oSegC);
   p.segCap = sc->segCap;
   p.parts = (PartInfo*)(b + oParts);
   p.bandItem = (int32_t*)(b + oBI);

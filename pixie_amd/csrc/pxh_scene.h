@@ -81,6 +81,9 @@ constexpr int32_t kRunXMask = (1 << 28) - 1;
 // always enough, whatever order the waves take tasks in.
 constexpr int kRunChunk = 2048;
 
+// segBand flag: the segment needs anti-aliasing (requiresAntiAliasing).
+constexpr int32_t kSegAA = 1 << 30;
+
 enum SceneCounter {
   kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks,
   kCntRuns,      // run-pool entries handed out in chunks
@@ -126,6 +129,8 @@ struct ScenePools {
   int32_t ptCap;
   float4* segs;
   int8_t* segW;
+  int2* segBand;             // per segment: first band, last band | kSegAA
+  int2* segChunk;            // per 64-segment chunk (one call's): band range
   int32_t segCap;
   PartInfo* parts;
   int32_t* bandItem;

@@ -19,7 +19,8 @@ constexpr int kChunksB = kWinB / 256;
 // ---------------------------------------------------------------------------
 // block helpers (256 threads = 4 waves)
 // ---------------------------------------------------------------------------
-__device__ int block_excl_scan(int v, int* total, int* sh) {
+template <int NB>
+__device__ int block_excl_scan_n(int v, int* total, int* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int wt;
   int x = wave_excl_scan(v, &wt);
@@ -28,7 +29,7 @@ __device__ int block_excl_scan(int v, int* total, int* sh) {
   __syncthreads();
   int off = 0, tot = 0;
 #pragma unroll
-  for (int i = 0; i < kBlock / 64; i++) {
+  for (int i = 0; i < NB / 64; i++) {
     int t = sh[i];
     if (i < w) off += t;
     tot += t;
@@ -36,6 +37,14 @@ __device__ int block_excl_scan(int v, int* total, int* sh) {
   *total = tot;
   return off + x;
 }
+
+__device__ __forceinline__ int block_excl_scan(int v, int* total, int* sh) {
+  return block_excl_scan_n<kBlock>(v, total, sh);
+}
+
+// Single-workgroup scans over every band / call / bin use the largest
+// workgroup so that they take few serial steps.
+constexpr int kScanBlock = 1024;
 
 __device__ int block_sum(int v, int* sh) {
   int t;
@@ -279,6 +288,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   __shared__ int sh[8];
   __shared__ int sBase;
   __shared__ Bounds sBd[kBlock / 64];
+  __shared__ FillInfo sF;
   __shared__ int sErr;
   const int it = blockIdx.x;
   const int tid = threadIdx.x;
@@ -448,7 +458,8 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       nseg = segOff;
       __syncthreads();
       if (tid == 0) {
-        int base = atomicAdd(&P.counters[kCntSegs], nseg);
+        // segments start on a 64-segment chunk of their own
+        int base = atomicAdd(&P.counters[kCntSegs], (nseg + 63) & ~63);
         if (base + nseg > P.segCap) {
           atomicOr(&P.counters[kCntError], 1);
           sErr = kErrCapacity;
@@ -480,6 +491,32 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
         f.status = r;
         if (r == kOk && f.pathWidth > 0) band_layout(&f, nseg);
         P.fills[it] = f;
+        sF = f;
+      }
+      __syncthreads();
+      // 7. band range of every segment (partitionSegments 1201-1213) and its
+      // requiresAntiAliasing bit, and the band range of each 64-segment chunk
+      const FillInfo fl = sF;
+      if (fl.status == kOk && fl.pathWidth > 0) {
+        for (int j0 = 0; j0 < nseg; j0 += kBlock) {
+          const int j = j0 + tid;
+          int lo = 0x7fffffff, hi = -1;
+          if (j < nseg) {
+            const float4 sg = P.segs[sBase + j];
+            uint32_t l = 0, h = 0;
+            if (fl.nParts > 1)
+              partition_range(sg.y, sg.w, (uint32_t)fl.startY, (uint32_t)fl.bandH, (uint32_t)fl.nParts, &l, &h);
+            const bool aa = seg_requires_aa(v2(sg.x, sg.y), v2(sg.z, sg.w));
+            P.segBand[sBase + j] = make_int2((int)l, (int)h | (aa ? kSegAA : 0));
+            lo = (int)l;
+            hi = (int)h;
+          }
+          for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o, 64));
+            hi = max(hi, __shfl_xor(hi, o, 64));
+          }
+          if ((tid & 63) == 0 && j < nseg) P.segChunk[(sBase + j) >> 6] = make_int2(lo, hi);
+        }
       }
     }
   }
@@ -501,11 +538,11 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
 // ---------------------------------------------------------------------------
 // planning (one block): bands, phase-A tasks, band -> call map
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) scene_plan_kernel(ScenePools P) {
-  __shared__ int sh[8];
+__global__ void __launch_bounds__(kScanBlock) scene_plan_kernel(ScenePools P) {
+  __shared__ int sh[kScanBlock / 64];
   const int tid = threadIdx.x;
   int bandTot = 0, taskTot = 0;
-  for (int i0 = 0; i0 < P.nItems; i0 += kBlock) {
+  for (int i0 = 0; i0 < P.nItems; i0 += kScanBlock) {
     int it = i0 + tid;
     int nb = 0, nt = 0;
     ItemGeom g;
@@ -534,8 +571,8 @@ __global__ void __launch_bounds__(kBlock) scene_plan_kernel(ScenePools P) {
       P.itemRows[it] = rows;
     }
     int tb, tt;
-    int bOff = block_excl_scan(nb, &tb, sh);
-    int tOff = block_excl_scan(nt, &tt, sh);
+    int bOff = block_excl_scan_n<kScanBlock>(nb, &tb, sh);
+    int tOff = block_excl_scan_n<kScanBlock>(nt, &tt, sh);
     if (it < P.nItems) {
       g.bandBase = bandTot + bOff;
       g.taskBase = taskTot + tOff;
@@ -559,12 +596,6 @@ __device__ __forceinline__ void band_rows(const FillInfo& f, int k, float* top, 
   *bot = (k == f.nParts - 1) ? (float)f.pathHeight : (float)(f.startY + (k + 1) * f.bandH);
 }
 
-__device__ __forceinline__ bool seg_in_band(const float4& s, const FillInfo& f, int k) {
-  if (f.nParts == 1) return true;
-  uint32_t lo, hi;
-  partition_range(s.y, s.w, (uint32_t)f.startY, (uint32_t)f.bandH, (uint32_t)f.nParts, &lo, &hi);
-  return (uint32_t)k >= lo && (uint32_t)k <= hi;
-}
 
 // One wave per band: entry count and requiresAntiAliasing.
 __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
@@ -578,13 +609,16 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
     const int k = b - g.bandBase;
     int cnt = 0;
     bool aa = false;
-    for (int j0 = 0; j0 < g.segCount; j0 += 64) {
-      int j = j0 + lane;
+    const int s0 = g.segBase, s1 = g.segBase + g.segCount;  // s0 is 64-aligned
+    for (int c = s0 >> 6; (c << 6) < s1; c++) {
+      const int2 cr = P.segChunk[c];
+      if (k < cr.x || k > cr.y) continue;
+      const int j = (c << 6) + lane;
       bool in = false, a = false;
-      if (j < g.segCount) {
-        float4 s = P.segs[g.segBase + j];
-        in = seg_in_band(s, f, k);
-        if (in) a = seg_requires_aa(v2(s.x, s.y), v2(s.z, s.w));
+      if (j < s1) {
+        const int2 sb = P.segBand[j];
+        in = k >= sb.x && k <= (sb.y & ~kSegAA);
+        a = in && (sb.y & kSegAA);
       }
       cnt += __popcll(__ballot(in));
       if (__ballot(a)) aa = true;
@@ -601,20 +635,20 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
 }
 
 // One block: entry offsets of all bands, largest band.
-__global__ void __launch_bounds__(kBlock) scene_entry_scan_kernel(ScenePools P) {
-  __shared__ int sh[8];
+__global__ void __launch_bounds__(kScanBlock) scene_entry_scan_kernel(ScenePools P) {
+  __shared__ int sh[kScanBlock / 64];
   __shared__ int sMax;
   const int nb = P.counters[kCntBands];
   if (threadIdx.x == 0) sMax = 0;
   __syncthreads();
   if (nb > P.bandCap) return;
   int tot = 0, mx = 0;
-  for (int b0 = 0; b0 < nb; b0 += kBlock) {
+  for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
     int b = b0 + threadIdx.x;
     int c = b < nb ? P.parts[b].count : 0;
     mx = c > mx ? c : mx;
     int t;
-    int off = block_excl_scan(c, &t, sh);
+    int off = block_excl_scan_n<kScanBlock>(c, &t, sh);
     if (b < nb) P.parts[b].entryBase = tot + off;
     tot += t;
   }
@@ -661,22 +695,25 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
     const PartInfo pi = P.parts[b];
     float top, bot;
     band_rows(f, k, &top, &bot);
+    const int s0 = g.segBase, s1 = g.segBase + g.segCount;
     if (pi.count != 2) {
       int pos = 0;
-      for (int j0 = 0; j0 < g.segCount; j0 += 64) {
-        int j = j0 + lane;
+      for (int c = s0 >> 6; (c << 6) < s1 && pos < pi.count; c++) {
+        const int2 cr = P.segChunk[c];
+        if (k < cr.x || k > cr.y) continue;
+        const int j = (c << 6) + lane;
         bool in = false;
-        float4 sg = make_float4(0, 0, 0, 0);
-        if (j < g.segCount) {
-          sg = P.segs[g.segBase + j];
-          in = seg_in_band(sg, f, k);
+        if (j < s1) {
+          const int2 sb = P.segBand[j];
+          in = k >= sb.x && k <= (sb.y & ~kSegAA);
         }
         uint64_t m = __ballot(in);
         int p = pos + __popcll(m & ((1ull << lane) - 1ull));
         if (in) {
-          float4 c;
-          P.entries[pi.entryBase + p] = band_entry(sg, top, bot, &c);
-          P.entW[pi.entryBase + p] = P.segW[g.segBase + j];
+          const float4 sg = P.segs[j];
+          float4 cl;
+          P.entries[pi.entryBase + p] = band_entry(sg, top, bot, &cl);
+          P.entW[pi.entryBase + p] = P.segW[j];
         }
         pos += __popcll(m);
       }
@@ -684,17 +721,22 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
     }
     // two entries: gather both segments into every lane, decide the
     // two-spanning flag and order (1250-1262), then write
-    float4 s0 = make_float4(0, 0, 0, 0), s1 = s0;
+    float4 g0 = make_float4(0, 0, 0, 0), g1 = g0;
     int w0 = 0, w1 = 0, found = 0;
-    for (int j0 = 0; j0 < g.segCount && found < 2; j0 += 64) {
-      int j = j0 + lane;
+    for (int c = s0 >> 6; (c << 6) < s1 && found < 2; c++) {
+      const int2 cr = P.segChunk[c];
+      if (k < cr.x || k > cr.y) continue;
+      const int j = (c << 6) + lane;
       bool in = false;
       float4 sg = make_float4(0, 0, 0, 0);
       int wv = 0;
-      if (j < g.segCount) {
-        sg = P.segs[g.segBase + j];
-        in = seg_in_band(sg, f, k);
-        wv = P.segW[g.segBase + j];
+      if (j < s1) {
+        const int2 sb = P.segBand[j];
+        in = k >= sb.x && k <= (sb.y & ~kSegAA);
+        if (in) {
+          sg = P.segs[j];
+          wv = P.segW[j];
+        }
       }
       uint64_t m = __ballot(in);
       while (m != 0 && found < 2) {
@@ -704,18 +746,18 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
                                __shfl(sg.w, src, 64));
         int tw = __shfl(wv, src, 64);
         if (found == 0) {
-          s0 = t;
+          g0 = t;
           w0 = tw;
         } else {
-          s1 = t;
+          g1 = t;
           w1 = tw;
         }
         found++;
       }
     }
     float4 a, c;
-    Entry e0 = band_entry(s0, top, bot, &a);
-    Entry e1 = band_entry(s1, top, bot, &c);
+    Entry e0 = band_entry(g0, top, bot, &a);
+    Entry e1 = band_entry(g1, top, bot, &c);
     int32_t flags = pi.flags;
     bool swap = false;
     V2 a0 = v2(a.x, a.y), a1 = v2(a.z, a.w), c0 = v2(c.x, c.y), c1 = v2(c.z, c.w);
@@ -760,15 +802,15 @@ __global__ void __launch_bounds__(64) scene_bin_kernel(ScenePools P) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock) scene_bin_scan_kernel(ScenePools P) {
-  __shared__ int sh[8];
+__global__ void __launch_bounds__(kScanBlock) scene_bin_scan_kernel(ScenePools P) {
+  __shared__ int sh[kScanBlock / 64];
   const int nBins = (P.H + kBinRows - 1) / kBinRows;
   int tot = 0;
-  for (int b0 = 0; b0 < nBins; b0 += kBlock) {
+  for (int b0 = 0; b0 < nBins; b0 += kScanBlock) {
     int b = b0 + threadIdx.x;
     int c = b < nBins ? P.binCount[b] : 0;
     int t;
-    int off = block_excl_scan(c, &t, sh);
+    int off = block_excl_scan_n<kScanBlock>(c, &t, sh);
     if (b < nBins) P.binOff[b] = tot + off;
     tot += t;
   }
@@ -1121,13 +1163,13 @@ size_t scene_gscratch_bytes(int rowGrid, int gcap) { return (size_t)rowGrid * ro
 int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
   if (p.nItems <= 0) return 0;
   hipLaunchKernelGGL(scene_geom_kernel, dim3(p.nItems), dim3(kBlock), 0, s, p);
-  hipLaunchKernelGGL(scene_plan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_plan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
   hipLaunchKernelGGL(scene_band_count_kernel, dim3(grid), dim3(64), 0, s, p);
-  hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
   hipLaunchKernelGGL(scene_band_write_kernel, dim3(grid), dim3(64), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
@@ -1135,7 +1177,7 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
   hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<false>, dim3(grid), dim3(64), 0, s, p);
-  hipLaunchKernelGGL(scene_bin_scan_kernel, dim3(1), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_bin_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<true>, dim3(grid), dim3(64), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
