@@ -84,6 +84,48 @@ STAGES = ("fill_rows_kernel", "scene_geometry", "scene_bands", "scene_bins", "sc
           "scene_compose_kernel")
 
 
+def glyph_extra(px, WL, reps=5, cpu_seconds=5.0):
+    """10,000-glyph atlas: batched (one scene render) vs one fillPath per glyph."""
+    import oracle
+    from pixie_amd.paths import parse_command_count
+    t = time.perf_counter()
+    cells, aw, ah = WL.glyph_atlas()
+    gen = time.perf_counter() - t
+    white = (255, 255, 255, 255)
+    sc = px.Scene()
+    for cmds, x, y, w, h in cells:
+        sc.fillPath(cmds, white)
+    atlas = px.Image(aw, ah, a8=True)
+    sc.render(atlas)  # first render sizes the pools
+    wall, touched, stages = timed(px, lambda: sc.render(atlas), reps)
+    n = len(cells)
+    kname, (kms, klaunch) = max(stages.items(), key=lambda kv: kv[1][0])
+    # input points: the command array minus one kind per command, two floats each
+    points = sum((len(c[0]) - parse_command_count(c[0])) // 2 for c in cells)
+    plain = px.Image(aw, ah, a8=True)
+    pw, pt, _ = timed(px, lambda: [plain.fillPath(c[0], white) for c in cells], 1)
+    # CPU: the oracle on a bounded prefix of the glyphs
+    cimg = oracle.new_image(aw, ah)
+    t1 = time.perf_counter()
+    k = 0
+    for cmds, x, y, w, h in cells:
+        oracle.fill_path(cimg, cmds, white)
+        k += 1
+        if time.perf_counter() - t1 > cpu_seconds:
+            break
+    cdt = time.perf_counter() - t1
+    algo_bytes = aw * ah + 8 * points
+    return {"workload": "glyph atlas: %d glyphs of 10-120 quads, 1-4 contours, NonZero, into one "
+                        "%dx%d 8-bit atlas (seed 4; generated in %.1f s)" % (n, aw, ah, gen),
+            "glyphs_per_s": round(n * reps / wall, 1), "ms_per_atlas": round(wall / reps * 1e3, 3),
+            "n_touch_per_atlas": int(touched / reps),
+            "stages_ms": {k2: round(v[0] / reps, 4) for k2, v in stages.items()},
+            "algorithmic_gbs": round(algo_bytes / (wall / reps) / 1e9, 2),
+            "dominant_kernel": kname,
+            "plain_glyphs_per_s": round(n / pw, 1), "batched_speedup": round(n * reps / wall / (n / pw), 2),
+            "cpu_glyphs_per_s": round(k / cdt, 1), "cpu_sample": "first %d glyphs, %.1f s, 1 core" % (k, cdt)}
+
+
 def cpu_baseline(cmds, canvas, seconds):
     import oracle
     img = canvas.copy()
@@ -138,6 +180,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-glyphs", action="store_true")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 20 if args.form == "batched" else 3
@@ -206,6 +249,8 @@ def main():
         result["cpu_baseline"] = cpu_baseline(cmds, canvas, args.cpu_seconds)
     if not args.no_extra:
         extra = {"star512": star_extra(px, WL)}
+        if not args.no_glyphs:
+            extra["glyph_atlas"] = glyph_extra(px, WL)
         if args.form == "batched":
             pimg = px.Image.from_array(canvas)
             pw, pt, pst = timed(px, lambda: run_plain(px, pimg, cmds), 1)

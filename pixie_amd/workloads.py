@@ -153,10 +153,11 @@ def glyph_path(rng, w, h):
     p = Path()
     cx, cy = w / 2.0, h / 2.0
     for ci, k in enumerate(counts):
-        # outer contour spans the cell, holes shrink inward and reverse direction
-        scale = 0.48 if ci == 0 else 0.12 + 0.25 * rng.random()
-        ox = cx + (0 if ci == 0 else rng.uniform(-0.2, 0.2) * w)
-        oy = cy + (0 if ci == 0 else rng.uniform(-0.2, 0.2) * h)
+        # outer contour spans the cell, holes shrink inward and reverse
+        # direction; every point and control point stays inside the cell
+        scale = 0.48 if ci == 0 else 0.12 + 0.18 * rng.random()
+        ox = cx + (0 if ci == 0 else rng.uniform(-0.15, 0.15) * w)
+        oy = cy + (0 if ci == 0 else rng.uniform(-0.15, 0.15) * h)
         direction = 1 if ci == 0 else -1
         angles = np.sort(rng.uniform(0, 2 * math.pi, size=k)) * direction
         rad = rng.uniform(0.6, 1.0, size=k)
@@ -167,7 +168,7 @@ def glyph_path(rng, w, h):
             a0 = angles[j]
             a1 = angles[(j + 1) % k] + (2 * math.pi * direction if j == k - 1 else 0)
             am = (a0 + a1) / 2
-            rc = rng.uniform(0.5, 1.2) * scale
+            rc = rng.uniform(0.5, 1.0) * scale
             ctrl = (ox + math.cos(am) * rc * w, oy + math.sin(am) * rc * h)
             to = pts[(j + 1) % k]
             p.quadratic_curve_to(ctrl[0], ctrl[1], to[0], to[1])

@@ -362,3 +362,26 @@ def test_scene_repeated_renders_are_identical(gpu):
         else:
             ref2.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
     _exact(img.download(), ref2.download(), "two renders")
+
+
+def test_scene_glyph_atlas(gpu, orc):
+    # the glyph-atlas workload at small scale: every glyph filled NonZero into
+    # its own cell of one 8-bit atlas, in one batched render
+    cells, aw, ah = WL.glyph_atlas(seed=11, n=300, atlas_w=1024)
+    sc = px.Scene()
+    ref = px.Image(aw, ah, a8=True)
+    want = orc.new_image(aw, ah)
+    for cmds, x, y, w, h in cells:
+        sc.fillPath(cmds, (255, 255, 255, 255))
+        ref.fillPath(cmds, (255, 255, 255, 255))
+        orc.fill_path(want, cmds, (255, 255, 255, 255))
+    img = px.Image(aw, ah, a8=True)
+    sc.render(img)
+    got = img.download()
+    _exact(got, ref.download(), "atlas scene vs per-call")
+    _cov_check(got, want[..., 3], "atlas vs oracle")
+    # every cell holds only its own glyph: nothing is written in the gutters
+    mask = np.zeros((ah, aw), bool)
+    for cmds, x, y, w, h in cells:
+        mask[y:y + h, x:x + w] = True
+    assert not got[~mask].any()
