@@ -126,6 +126,67 @@ def glyph_extra(px, WL, reps=5, cpu_seconds=5.0):
             "cpu_glyphs_per_s": round(k / cdt, 1), "cpu_sample": "first %d glyphs, %.1f s, 1 core" % (k, cdt)}
 
 
+def image_ops_extra(px, WL):
+    """Blur / shadow / mask pipeline on a 4096x4096 random premultiplied image:
+    each op on a fresh copy, timed with HIP events on the library stream."""
+    import oracle
+    import numpy as np
+    src_arr = WL.blurmask_image()
+    H, W = src_arr.shape[:2]
+    src = px.Image.from_array(src_arr)
+    work = px.Image(W, H)
+    npx = W * H
+    out = {"workload": "blur/shadow/mask pipeline, %dx%d RGBA8, alpha U{0..255}, channels <= alpha "
+                       "(seed 3); each op on a fresh copy" % (W, H)}
+
+    def timed_op(fn, reps=5):
+        ms = []
+        for _ in range(reps):
+            work_reset()
+            px.event_record(0)
+            fn()
+            px.event_record(1)
+            ms.append(px.event_elapsed_ms(0, 1))
+        return min(ms), sorted(ms)[len(ms) // 2]
+
+    def work_reset():
+        lib = px.lib()
+        lib.pxh_image_copy(work.handle, src.handle)
+
+    for r in (2, 8, 32):
+        best, med = timed_op(lambda: work.blur(r))
+        out["blur_r%d" % r] = {"ms": round(med, 4), "algorithmic_gbs": round(16.0 * npx / (med / 1e3) / 1e9, 1),
+                               "bytes_per_px": 16}
+    shadow_color = WL.premul(20, 30, 40, 200)
+    res = {}
+
+    def do_shadow():
+        res["img"] = work.shadow((12, 12), 4, 16, shadow_color)
+    best, med = timed_op(do_shadow)
+    out["shadow_o12_s4_b16"] = {"ms": round(med, 4), "algorithmic_gbs": round(33.0 * npx / (med / 1e3) / 1e9, 1),
+                                "bytes_per_px": 33}
+    # the mask: 500 random circles filled NonZero into an 8-bit plane
+    circ = WL.circles_mask_cmds()
+    mask = px.Image(W, H, a8=True)
+    px.event_record(0)
+    mask.fillPath(circ.cmds, circ.color)
+    px.event_record(1)
+    out["mask_fill_500_circles_ms"] = round(px.event_elapsed_ms(0, 1), 4)
+    for name, mode in (("subtract", px.BlendMode.SubtractMask), ("intersect", px.BlendMode.Mask),
+                       ("exclude", px.BlendMode.ExcludeMask)):
+        best, med = timed_op(lambda: work.draw(mask, 0, 0, mode))
+        out["mask_%s" % name] = {"ms": round(med, 4), "algorithmic_gbs": round(9.0 * npx / (med / 1e3) / 1e9, 1),
+                                 "bytes_per_px": 9}
+    # CPU: the oracle's blur r=8 on a 1024x1024 crop, scaled per pixel
+    crop = np.ascontiguousarray(src_arr[:1024, :1024])
+    t = time.perf_counter()
+    oracle.blur(crop, 8)
+    cdt = time.perf_counter() - t
+    out["cpu_blur_r8_mpix_s"] = round(1024 * 1024 / cdt / 1e6, 2)
+    out["gpu_blur_r8_mpix_s"] = round(npx / (out["blur_r8"]["ms"] / 1e3) / 1e6, 1)
+    return out
+
+
 def cpu_baseline(cmds, canvas, seconds):
     import oracle
     img = canvas.copy()
@@ -245,12 +306,22 @@ def main():
                      "stages_ms_per_step": {k: round(v[0] / args.steps, 4)
                                             for k, v in stages.items()}},
     }
+    # HBM traffic of the dominant kernel, as measured with rocprofv3 --pmc
+    # FETCH_SIZE / WRITE_SIZE (copy-calibrated) and recorded beside the profile
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tr = json.load(f)
+        if tr.get("kernel") == kname and args.form == "batched":
+            result["roofline"]["traffic"] = tr["traffic_bytes_per_launch"]
+            result["roofline"]["traffic_source"] = "profiles/traffic.json: " + tr["source"]
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(cmds, canvas, args.cpu_seconds)
     if not args.no_extra:
         extra = {"star512": star_extra(px, WL)}
         if not args.no_glyphs:
             extra["glyph_atlas"] = glyph_extra(px, WL)
+        extra["image_ops"] = image_ops_extra(px, WL)
         if args.form == "batched":
             pimg = px.Image.from_array(canvas)
             pw, pt, pst = timed(px, lambda: run_plain(px, pimg, cmds), 1)
