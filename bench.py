@@ -187,6 +187,27 @@ def image_ops_extra(px, WL):
     return out
 
 
+def serve_extra(px, WL, rates=(50.0, 200.0, 800.0), n=60, plain_n=24):
+    """Serving loop: Poisson arrivals of scenes (1-500 paths, side 256-4096,
+    blur 0 or 8), continuous batching over the execution lanes vs the plain
+    per-call form serving the same arrivals one at a time."""
+    from pixie_amd import serve
+    out = {"workload": "serving: Poisson arrivals, scenes of 1-500 paths (log-uniform) as in the mixed "
+                       "scene, canvas side log-uniform 256-4096 px RGBA8, blur 0 or 8 (seed 5)",
+           "lanes": px.lanes() - 1, "hbm_budget_gib": 4}
+    server = serve.Server(hbm_budget=4 << 30)
+    warm = WL.serve_requests(seed=99, n=8, rate=0)
+    server.run(warm)
+    for rate in rates:
+        reqs = WL.serve_requests(seed=5, n=n, rate=rate)
+        res = server.run(reqs)
+        out["batched_rate_%d" % rate] = serve.latency_stats(res, reqs)
+    reqs = WL.serve_requests(seed=5, n=plain_n, rate=rates[0])
+    res = serve.serve_plain(reqs)
+    out["plain_rate_%d" % rates[0]] = serve.latency_stats(res, reqs)
+    return out
+
+
 def cpu_baseline(cmds, canvas, seconds):
     import oracle
     img = canvas.copy()
@@ -322,6 +343,7 @@ def main():
         if not args.no_glyphs:
             extra["glyph_atlas"] = glyph_extra(px, WL)
         extra["image_ops"] = image_ops_extra(px, WL)
+        extra["serving"] = serve_extra(px, WL)
         if args.form == "batched":
             pimg = px.Image.from_array(canvas)
             pw, pt, pst = timed(px, lambda: run_plain(px, pimg, cmds), 1)

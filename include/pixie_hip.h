@@ -127,6 +127,22 @@ int pxh_scene_debug_geometry(pxh_scene* scene, int32_t i, int32_t* info16, int32
                              int32_t* nparts, float** entries4, int8_t** windings,
                              int32_t* nentries);
 
+/* Execution lanes (no reference counterpart: pixie is single-threaded).
+ * The library owns pxh_lanes() independent device streams.  Lane 0 is the
+ * one every other function uses.  Work issued on different lanes, each
+ * driven by its own host thread, overlaps on the device.  Lanes have their
+ * own temporaries, so one host thread per lane may call the *_on functions
+ * concurrently, on distinct scenes and images.  Lane i's work is ordered
+ * after its earlier work only. */
+int32_t pxh_lanes(void);
+int pxh_lane_synchronize(int32_t lane);
+int pxh_scene_render_on(pxh_scene* scene, pxh_image* img, int32_t lane);
+int pxh_blur_on(pxh_image* img, float radius, uint32_t oob_rgbx, int32_t lane);
+int pxh_image_fill_on(pxh_image* img, uint32_t rgbx, int32_t lane);
+/* A non-owning image over caller-owned device memory (width*height*bpp
+ * bytes); pxh_image_free releases only the handle. */
+pxh_image* pxh_image_view(void* device_ptr, int32_t width, int32_t height, int32_t format);
+
 /* Image operations ------------------------------------------------------ */
 /* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.
  * blendRect (images.nim:468-529, reached from draw images.nim:636-678). */

@@ -90,6 +90,13 @@ def lib():
         L.pxh_debug_fill_geometry.argtypes = [f32p, i32, f32p, i32, i32, i32, f, i32, i32, f,
                                               i32p, P(i32p), i32p, P(f32p),
                                               P(P(ctypes.c_int8)), i32p]
+        L.pxh_lanes.restype = i32
+        L.pxh_lane_synchronize.argtypes = [i32]
+        L.pxh_scene_render_on.argtypes = [vp, vp, i32]
+        L.pxh_blur_on.argtypes = [vp, f, u32, i32]
+        L.pxh_image_fill_on.argtypes = [vp, u32, i32]
+        L.pxh_image_view.restype = vp
+        L.pxh_image_view.argtypes = [vp, i32, i32, i32]
         L.pxh_scene_new.restype = vp
         L.pxh_scene_free.argtypes = [vp]
         L.pxh_scene_clear.argtypes = [vp]
@@ -170,6 +177,18 @@ class Image:
         self.width, self.height, self.a8 = width, height, a8
 
     @classmethod
+    def view(cls, device_ptr, width, height, a8=False):
+        """A non-owning image over device memory the caller keeps alive."""
+        L = lib()
+        h = L.pxh_image_view(ctypes.c_void_p(device_ptr), width, height, 1 if a8 else 0)
+        if not h:
+            raise PixieError(6, L.pxh_last_error().decode())
+        img = cls.__new__(cls)
+        img._h = ctypes.c_void_p(h)
+        img.width, img.height, img.a8 = width, height, a8
+        return img
+
+    @classmethod
     def from_array(cls, arr):
         arr = np.ascontiguousarray(arr, dtype=np.uint8)
         a8 = arr.ndim == 2
@@ -207,8 +226,11 @@ class Image:
         _check(lib().pxh_image_copy(o._h, self._h))
         return o
 
-    def fill(self, color):
-        _check(lib().pxh_image_fill(self._h, pack(color)))
+    def fill(self, color, lane=0):
+        if lane:
+            _check(lib().pxh_image_fill_on(self._h, pack(color), lane))
+        else:
+            _check(lib().pxh_image_fill(self._h, pack(color)))
 
     def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
         if _paint_skipped(color, blendMode):
@@ -242,8 +264,8 @@ class Image:
     def draw(self, other, dx=0, dy=0, blendMode=0):
         _check(lib().pxh_draw(self._h, other._h, int(dx), int(dy), blendMode))
 
-    def blur(self, radius, outOfBounds=(0, 0, 0, 0)):
-        _check(lib().pxh_blur(self._h, float(radius), pack(outOfBounds)))
+    def blur(self, radius, outOfBounds=(0, 0, 0, 0), lane=0):
+        _check(lib().pxh_blur_on(self._h, float(radius), pack(outOfBounds), lane))
 
     def spread(self, s):
         _check(lib().pxh_spread(self._h, float(s)))
@@ -300,8 +322,9 @@ class Scene:
                                            strokeWidth, lineCap, lineJoin, miterLimit,
                                            blendMode))
 
-    def render(self, image):
-        _check(lib().pxh_scene_render(self._h, image.handle))
+    def render(self, image, lane=0):
+        """Renders onto `image`, queued on execution lane `lane`."""
+        _check(lib().pxh_scene_render_on(self._h, image.handle, lane))
 
     def status(self):
         n = len(self)
@@ -325,6 +348,15 @@ class Scene:
         for p in (parts, ents, wind):
             lib().pxh_free(p)
         return np.array(info[:], np.int32), pa.reshape(-1, 4), en.reshape(-1, 4), wd
+
+
+def lanes():
+    """Number of execution lanes (independent device streams)."""
+    return lib().pxh_lanes()
+
+
+def lane_synchronize(lane):
+    _check(lib().pxh_lane_synchronize(lane))
 
 
 def touched_pixels(reset=True):

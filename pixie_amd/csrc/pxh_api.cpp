@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -21,6 +22,7 @@ struct pxh_image {
   int32_t w, h, fmt;
   void* dev;
   size_t bytes;
+  bool owned = true;  // false for views over caller memory
 };
 
 struct pxh_scene {
@@ -29,10 +31,12 @@ struct pxh_scene {
   std::vector<pxh::ScenePoly> polys;
   bool uploaded = false;
   int32_t validW = -1, validH = -1;  // target size the pools were validated for
-  // device copies of the recording
+  int32_t lane = 0;  // lane of the last render
+  // device copies of the recording (grow-only)
   pxh::SceneItem* dItems = nullptr;
   pxh::SceneCmd* dCmds = nullptr;
   pxh::ScenePoly* dPolys = nullptr;
+  size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0;
   // per-render buffers
   char* dMem = nullptr;
   size_t dMemBytes = 0;
@@ -58,6 +62,8 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 // Growable device buffer.
+constexpr int kLanes = 5;  // lane 0 + 4 lanes for concurrent requests
+
 struct DevBuf {
   char* p = nullptr;
   size_t cap = 0;
@@ -87,6 +93,13 @@ struct Ctx {
   DevBuf devStage;
   hipEvent_t stageFree = nullptr;
   DevBuf gscratch;   // per-workgroup row scratch when it does not fit in LDS
+  // Execution lanes: independent streams (lane 0 is `stream`), each with its
+  // own blur temporary and scene row scratch, so that work issued on
+  // different lanes (by different host threads) overlaps on the device.
+  hipStream_t lane[kLanes] = {};
+  DevBuf laneTmp[kLanes];
+  DevBuf laneScratch[kLanes];
+  std::mutex lutMu;
   DevBuf tmp;        // blur / shadow temporaries
   DevBuf tmp2;
   DevBuf plane;      // alpha plane
@@ -118,6 +131,8 @@ int init() {
   g.maxLds = (size_t)lds;
   g.numCUs = cus;
   PXH_HIP(hipStreamCreate(&g.stream));
+  g.lane[0] = g.stream;
+  for (int l = 1; l < kLanes; l++) PXH_HIP(hipStreamCreate(&g.lane[l]));
   PXH_HIP(hipEventCreateWithFlags(&g.stageFree, hipEventDisableTiming));
   PXH_HIP(hipEventRecord(g.stageFree, g.stream));
   PXH_HIP(hipMalloc((void**)&g.errFlag, sizeof(int32_t)));
@@ -301,6 +316,7 @@ void gaussian_lut(int radius, std::vector<uint32_t>* out) {
 }
 
 int get_lut(int radius, uint32_t** dev) {
+  std::lock_guard<std::mutex> lock(g.lutMu);
   auto it = g.luts.find(radius);
   if (it != g.luts.end()) {
     *dev = it->second;
@@ -317,15 +333,16 @@ int get_lut(int radius, uint32_t** dev) {
 }
 
 int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius, uint32_t oob,
-              bool shadowColor, uint32_t color) {
+              bool shadowColor, uint32_t color, int lane = 0) {
   if (blur_x_lds(radius) > g.maxLds || blur_y_lds(radius) > g.maxLds)
     return fail(kErrCapacity, "blur radius too large for the LDS tiles");
   uint32_t* lut = nullptr;
   if (int r = get_lut(radius, &lut)) return r;
-  if (int e = g.tmp.ensure((size_t)w * h * 4)) return fail(e, "blur temporary allocation failed");
-  BlurArgs ax{src_dst, (uint32_t*)g.tmp.p, lut, w, h, radius, oob, 0, 0};
-  BlurArgs ay{(uint32_t*)g.tmp.p, out, lut, w, h, radius, oob, shadowColor ? 1 : 0, color};
-  if (launch_blur(ax, ay, g.stream) != 0) return fail(kErrDevice, "blur launch failed");
+  DevBuf& tmp = lane == 0 ? g.tmp : g.laneTmp[lane];
+  if (int e = tmp.ensure((size_t)w * h * 4)) return fail(e, "blur temporary allocation failed");
+  BlurArgs ax{src_dst, (uint32_t*)tmp.p, lut, w, h, radius, oob, 0, 0};
+  BlurArgs ay{(uint32_t*)tmp.p, out, lut, w, h, radius, oob, shadowColor ? 1 : 0, color};
+  if (launch_blur(ax, ay, g.lane[lane]) != 0) return fail(kErrDevice, "blur launch failed");
   return 0;
 }
 
@@ -497,6 +514,7 @@ void scene_free_device(pxh_scene* sc) {
   sc->dCmds = nullptr;
   sc->dPolys = nullptr;
   sc->dMem = nullptr;
+  sc->dItemsCap = sc->dCmdsCap = sc->dPolysCap = 0;
   sc->dMemBytes = 0;
 }
 
@@ -522,10 +540,8 @@ int scene_alloc(pxh_scene* sc) {
   size_t oBinO = carve(((size_t)sc->binSlots + 1) * 4), oBinL = carve((size_t)sc->binCap * 4);
   size_t oQ = carve(2 * kQueues * kQueueStride * 4);
   size_t oCnt = carve(kNumCounters * 4);
-  const int gridRows = scene_queue_grid(g.numCUs * kSceneRowWavesPerCU);
-  size_t oScr = carve(scene_gscratch_bytes(gridRows, kSceneGcap));
   if (off > sc->dMemBytes) {
-    PXH_HIP(hipStreamSynchronize(g.stream));
+    PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
     if (sc->dMem) PXH_HIP(hipFree(sc->dMem));
     sc->dMem = nullptr;
     if (hipMalloc((void**)&sc->dMem, off) != hipSuccess) return fail(kErrNoMem, "scene pool allocation failed");
@@ -570,25 +586,38 @@ int scene_alloc(pxh_scene* sc) {
   p.binSlots = sc->binSlots;
   p.queues = (int32_t*)(b + oQ);
   p.counters = (int32_t*)(b + oCnt);
-  p.gscratch = b + oScr;
   p.gcap = kSceneGcap;
   p.ldsCap = kSceneLdsCap;
   return 0;
 }
 
-int scene_upload(pxh_scene* sc) {
+// Grow-only device buffer of a scene.
+template <class T>
+int scene_buf(T** p, size_t* cap, size_t n) {
+  if (n <= *cap && *p) return 0;
+  if (*p) PXH_HIP(hipFree(*p));
+  *p = nullptr;
+  size_t c = n + n / 2 + 64;
+  if (hipMalloc((void**)p, c * sizeof(T)) != hipSuccess) return fail(kErrNoMem, "scene upload failed");
+  *cap = c;
+  return 0;
+}
+
+int scene_upload(pxh_scene* sc, int lane) {
   if (sc->uploaded) return 0;
-  PXH_HIP(hipStreamSynchronize(g.stream));  // a previous render may still read the buffers
-  scene_free_device(sc);
-  auto up = [&](void** d, const void* h, size_t bytes) -> int {
-    if (hipMalloc(d, bytes ? bytes : 16) != hipSuccess) return kErrNoMem;
-    if (bytes && hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice) != hipSuccess) return kErrDevice;
-    return 0;
-  };
-  if (up((void**)&sc->dItems, sc->items.data(), sc->items.size() * sizeof(SceneItem)) ||
-      up((void**)&sc->dCmds, sc->cmds.data(), sc->cmds.size() * sizeof(SceneCmd)) ||
-      up((void**)&sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly)))
-    return fail(kErrNoMem, "scene upload failed");
+  hipStream_t st = g.lane[lane];
+  PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));  // a previous render may still read the buffers
+  if (int r = scene_buf(&sc->dItems, &sc->dItemsCap, sc->items.size())) return r;
+  if (int r = scene_buf(&sc->dCmds, &sc->dCmdsCap, sc->cmds.size())) return r;
+  if (int r = scene_buf(&sc->dPolys, &sc->dPolysCap, sc->polys.size())) return r;
+  PXH_HIP(hipMemcpyAsync(sc->dItems, sc->items.data(), sc->items.size() * sizeof(SceneItem),
+                         hipMemcpyHostToDevice, st));
+  PXH_HIP(hipMemcpyAsync(sc->dCmds, sc->cmds.data(), sc->cmds.size() * sizeof(SceneCmd),
+                         hipMemcpyHostToDevice, st));
+  if (!sc->polys.empty())
+    PXH_HIP(hipMemcpyAsync(sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly),
+                           hipMemcpyHostToDevice, st));
+  sc->lane = lane;
   // first estimates of the pools; a render that overflows grows them
   size_t nC = sc->cmds.size();
   sc->ptCap = (int32_t)std::min<size_t>(64 * nC + 4096, 1u << 28);
@@ -600,39 +629,48 @@ int scene_upload(pxh_scene* sc) {
       (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kSceneRowWavesPerCU) * kRunChunk,
       (int64_t)1 << 30);
   sc->binCap = (int32_t)std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28);
-  sc->binSlots = 0;  // sized by the first render
   sc->uploaded = true;
   return scene_alloc(sc);
 }
 
-int scene_launch(pxh_scene* sc, pxh_image* img) {
+int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   ScenePools& p = sc->pools;
+  hipStream_t st = g.lane[lane];
+  sc->lane = lane;
   p.W = img->w;
   p.H = img->h;
-  PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, g.stream));
-  PXH_HIP(hipMemsetAsync(p.queues, 0, 2 * kQueues * kQueueStride * 4, g.stream));
+  p.items = sc->dItems;
+  p.cmds = sc->dCmds;
+  p.polys = sc->dPolys;
+  const int rowGrid = scene_queue_grid(g.numCUs * kSceneRowWavesPerCU);
+  DevBuf& scr = g.laneScratch[lane];
+  if (int e = scr.ensure(scene_gscratch_bytes(rowGrid, kSceneGcap)))
+    return fail(e, "scene row scratch allocation failed");
+  p.gscratch = scr.p;
+  PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, st));
+  PXH_HIP(hipMemsetAsync(p.queues, 0, 2 * kQueues * kQueueStride * 4, st));
   int r;
   {
     LaunchTimer t("scene_geometry");
-    r = scene_launch_geometry(p, g.stream);
+    r = scene_launch_geometry(p, st);
   }
   if (!r) {
     LaunchTimer t("scene_bands");
-    r = scene_launch_bands(p, g.numCUs * 16, g.stream);
+    r = scene_launch_bands(p, g.numCUs * 16, st);
   }
   if (!r) {
     LaunchTimer t("scene_bins");
-    r = scene_launch_bins(p, g.numCUs * 4, g.stream);
+    r = scene_launch_bins(p, g.numCUs * 4, st);
   }
   if (!r) {
     LaunchTimer t("scene_rows_kernel");
-    r = scene_launch_rows(p, scene_queue_grid(g.numCUs * kSceneRowWavesPerCU), g.stream);
+    r = scene_launch_rows(p, rowGrid, st);
   }
   if (!r) {
     LaunchTimer t("scene_compose_kernel");
     r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched,
                              scene_queue_grid(g.numCUs * 16),
-                             g.stream);
+                             st);
   }
   if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
   return 0;
@@ -695,8 +733,10 @@ pxh_image* pxh_image_new_a8(int32_t w, int32_t h) { return image_new(w, h, PXH_F
 
 void pxh_image_free(pxh_image* img) {
   if (!img) return;
-  (void)hipStreamSynchronize(g.stream);
-  if (img->dev) (void)hipFree(img->dev);
+  if (img->owned) {
+    (void)hipStreamSynchronize(g.stream);
+    if (img->dev) (void)hipFree(img->dev);
+  }
   delete img;
 }
 
@@ -850,6 +890,54 @@ int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx) {
   return blur_impl(p, p, img->w, img->h, (int)r, oob_rgbx, false, 0);
 }
 
+int pxh_blur_on(pxh_image* img, float radius, uint32_t oob_rgbx, int32_t lane) {
+  PXH_INIT();
+  if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "blur needs an RGBA8 image");
+  int64_t r = nim_f2i(roundf(radius));  // images.nim:308
+  if (r == 0) return 0;
+  if (r < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
+  if (r > 1 << 20) return fail(kErrCapacity, "blur radius too large");
+  uint32_t* p = (uint32_t*)img->dev;
+  return blur_impl(p, p, img->w, img->h, (int)r, oob_rgbx, false, 0, lane);
+}
+
+int32_t pxh_lanes(void) { return kLanes; }
+
+int pxh_lane_synchronize(int32_t lane) {
+  PXH_INIT();
+  if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
+  PXH_HIP(hipStreamSynchronize(g.lane[lane]));
+  return 0;
+}
+
+int pxh_image_fill_on(pxh_image* img, uint32_t rgbx, int32_t lane) {
+  PXH_INIT();
+  if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
+  if (int r = check_image(img)) return r;
+  if (img->fmt == PXH_FORMAT_A8)
+    PXH_HIP(hipMemsetAsync(img->dev, (int)(rgbx >> 24), img->bytes, g.lane[lane]));
+  else
+    PXH_HIP(hipMemsetD32Async((hipDeviceptr_t)img->dev, (int)rgbx, img->bytes / 4, g.lane[lane]));
+  return 0;
+}
+
+pxh_image* pxh_image_view(void* device_ptr, int32_t width, int32_t height, int32_t format) {
+  if (!device_ptr || width <= 0 || height <= 0 || (format != PXH_FORMAT_RGBA8 && format != PXH_FORMAT_A8)) {
+    fail(kErrArg, "invalid image view");
+    return nullptr;
+  }
+  pxh_image* img = new pxh_image();
+  img->w = width;
+  img->h = height;
+  img->fmt = format;
+  img->dev = device_ptr;
+  img->bytes = (size_t)width * height * (format == PXH_FORMAT_A8 ? 1 : 4);
+  img->owned = false;
+  return img;
+}
+
 int pxh_spread(pxh_image* img, float spread) {
   PXH_INIT();
   if (int r = check_image(img)) return r;
@@ -933,14 +1021,14 @@ pxh_scene* pxh_scene_new(void) { return new pxh_scene(); }
 
 void pxh_scene_free(pxh_scene* sc) {
   if (!sc) return;
-  if (g.inited) (void)hipStreamSynchronize(g.stream);
+  if (g.inited) (void)hipStreamSynchronize(g.lane[sc->lane]);
   scene_free_device(sc);
   delete sc;
 }
 
 int pxh_scene_clear(pxh_scene* sc) {
   if (!sc) return fail(kErrArg, "invalid scene handle");
-  if (g.inited) PXH_HIP(hipStreamSynchronize(g.stream));
+  if (g.inited) PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
   sc->items.clear();
   sc->cmds.clear();
   sc->polys.clear();
@@ -963,26 +1051,34 @@ int pxh_scene_stroke_path(pxh_scene* sc, const float* cmds, int32_t ncmds, const
                       line_cap, line_join, miter_limit);
 }
 
-int pxh_scene_render(pxh_scene* sc, pxh_image* img) {
+int pxh_scene_render(pxh_scene* sc, pxh_image* img) { return pxh_scene_render_on(sc, img, 0); }
+
+int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
   PXH_INIT();
   if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
   if (int r = check_image(img)) return r;
   if (sc->items.empty()) return 0;
-  if (int r = scene_upload(sc)) return r;
+  if (sc->lane != lane) {  // the scene's buffers move to another lane
+    PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
+    sc->lane = lane;
+  }
+  if (int r = scene_upload(sc, lane)) return r;
+  hipStream_t st = g.lane[lane];
   const int32_t nBins = (img->h + kBinRows - 1) / kBinRows;
   if (nBins > sc->binSlots) {  // row bins for a taller image
     sc->binSlots = nBins;
     sc->validW = sc->validH = -1;
     if (int r = scene_alloc(sc)) return r;
   }
-  if (sc->validW == img->w && sc->validH == img->h) return scene_launch(sc, img);
+  if (sc->validW == img->w && sc->validH == img->h) return scene_launch(sc, img, lane);
   // First render at this size: check the pools and grow them until the render
   // fits.  A render that overflows a pool stops before touching the canvas.
   for (int attempt = 0; attempt < 8; attempt++) {
-    if (int r = scene_launch(sc, img)) return r;
+    if (int r = scene_launch(sc, img, lane)) return r;
     int32_t cnt[kNumCounters];
-    PXH_HIP(hipMemcpyAsync(cnt, sc->pools.counters, sizeof(cnt), hipMemcpyDeviceToHost, g.stream));
-    PXH_HIP(hipStreamSynchronize(g.stream));
+    PXH_HIP(hipMemcpyAsync(cnt, sc->pools.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    PXH_HIP(hipStreamSynchronize(st));
     // the run pool must also hold every wave's partly used chunk, whatever
     // order the waves take tasks in on later renders
     const int64_t runNeed =
@@ -1020,8 +1116,9 @@ int pxh_scene_status(pxh_scene* sc, int32_t* status, int32_t n) {
   if (!sc || !sc->uploaded) return fail(kErrArg, "scene not rendered");
   int32_t m = std::min<int32_t>(n, (int32_t)sc->items.size());
   std::vector<ItemGeom> geo(m);
-  PXH_HIP(hipMemcpyAsync(geo.data(), sc->pools.geo, m * sizeof(ItemGeom), hipMemcpyDeviceToHost, g.stream));
-  PXH_HIP(hipStreamSynchronize(g.stream));
+  hipStream_t st = g.lane[sc->lane];
+  PXH_HIP(hipMemcpyAsync(geo.data(), sc->pools.geo, m * sizeof(ItemGeom), hipMemcpyDeviceToHost, st));
+  PXH_HIP(hipStreamSynchronize(st));
   for (int32_t i = 0; i < m; i++) status[i] = geo[i].status;
   return 0;
 }
@@ -1034,7 +1131,7 @@ int pxh_scene_debug_geometry(pxh_scene* sc, int32_t i, int32_t* info16, int32_t*
   PXH_INIT();
   if (!sc || !sc->uploaded || i < 0 || i >= (int32_t)sc->items.size())
     return fail(kErrArg, "invalid scene call");
-  PXH_HIP(hipStreamSynchronize(g.stream));
+  PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
   FillInfo f;
   ItemGeom geo;
   PXH_HIP(hipMemcpy(&f, sc->pools.fills + i, sizeof(f), hipMemcpyDeviceToHost));

@@ -1,0 +1,218 @@
+"""A serving loop for scene requests on one GPU, with continuous batching.
+
+A request is one scene: an ordered list of fillPath / strokePath calls on a
+fresh transparent canvas of its own size, optionally blurred afterwards.
+Pixie has no server (it is a library); this module is the serving form of
+the raster path.
+
+Continuous batching works like this:
+
+* Requests queue in arrival order.
+* A request is admitted as soon as an execution lane is free and its canvas
+  fits in the HBM arena. The arena is one device allocation sized by the
+  HBM budget, carved first-fit.
+* Each admitted request runs on its own lane, an independent device stream
+  driven by its own host thread. Requests that arrive while others are on
+  the GPU therefore join while the others are still running, and their
+  kernels overlap on the device.
+* Recording a request's scene on the host overlaps with the other lanes'
+  device work.
+
+Canvases and scene buffers stay resident:
+
+* The arena is allocated once.
+* Each lane keeps one scene object whose device buffers only grow.
+
+The plain baseline serves the same arrivals one request at a time, with one
+fillPath / strokePath launch per call.
+"""
+import threading
+import time
+
+from . import Image, Scene, lane_synchronize, lanes
+
+
+class Request:
+    __slots__ = ("calls", "side", "blur", "arrival")
+
+    def __init__(self, calls, side, blur, arrival):
+        self.calls = calls        # list of workloads.FillCmd
+        self.side = int(side)     # square canvas side in px
+        self.blur = float(blur)   # blur radius after the paths (0 = none)
+        self.arrival = float(arrival)
+
+
+class Result:
+    __slots__ = ("index", "admitted", "done", "latency", "pixels")
+
+    def __init__(self, index):
+        self.index = index
+        self.admitted = self.done = self.latency = 0.0
+        self.pixels = None
+
+
+class _Arena:
+    """First-fit allocator over one device allocation (byte offsets)."""
+
+    def __init__(self, nbytes):
+        rows = (nbytes + 65535) // 65536
+        self.image = Image(65536, rows, a8=True)
+        self.base = self.image.device_ptr()
+        self.size = 65536 * rows
+        self.free = [(0, self.size)]  # sorted (offset, length)
+
+    def alloc(self, n):
+        n = (n + 255) & ~255
+        for i, (off, ln) in enumerate(self.free):
+            if ln >= n:
+                if ln == n:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (off + n, ln - n)
+                return off, n
+        return None
+
+    def release(self, off, n):
+        self.free.append((off, n))
+        self.free.sort()
+        merged = []
+        for o, ln in self.free:
+            if merged and merged[-1][0] + merged[-1][1] == o:
+                merged[-1] = (merged[-1][0], merged[-1][1] + ln)
+            else:
+                merged.append((o, ln))
+        self.free = merged
+
+
+def _record(scene, calls):
+    for c in calls:
+        if c.stroke:
+            scene.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                             miterLimit=c.miter, blendMode=c.mode)
+        else:
+            scene.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+
+
+class Server:
+    """Serves requests on `n_lanes` execution lanes within `hbm_budget`
+    bytes of canvas memory."""
+
+    def __init__(self, n_lanes=None, hbm_budget=1 << 30, keep_pixels=False):
+        self.n_lanes = n_lanes or (lanes() - 1)
+        if not 1 <= self.n_lanes <= lanes() - 1:
+            raise ValueError("n_lanes must be in 1..%d" % (lanes() - 1))
+        self.arena = _Arena(hbm_budget)
+        self.keep_pixels = keep_pixels
+        self.scenes = [Scene() for _ in range(self.n_lanes)]
+
+    def run(self, requests):
+        """Serves `requests` (arrival times in seconds from the start) and
+        returns one Result per request, in request order."""
+        n = len(requests)
+        results = [Result(i) for i in range(n)]
+        cv = threading.Condition()
+        state = {"next": 0, "error": None}
+        biggest = max(r.side * r.side * 4 for r in requests) if requests else 0
+        if biggest > self.arena.size:
+            raise ValueError("a canvas is larger than the HBM budget")
+        t0 = time.perf_counter() + 0.01
+
+        def take():
+            # FIFO admission: the oldest arrived request, once its canvas fits
+            with cv:
+                while True:
+                    if state["error"] is not None or state["next"] >= n:
+                        return None
+                    i = state["next"]
+                    r = requests[i]
+                    wait = t0 + r.arrival - time.perf_counter()
+                    if wait > 0:
+                        cv.wait(timeout=wait)
+                        continue
+                    blk = self.arena.alloc(r.side * r.side * 4)
+                    if blk is None:
+                        cv.wait(timeout=0.001)
+                        continue
+                    state["next"] = i + 1
+                    cv.notify_all()
+                    return i, blk
+
+        def worker(k):
+            lane = k + 1
+            scene = self.scenes[k]
+            try:
+                while True:
+                    got = take()
+                    if got is None:
+                        return
+                    i, (off, nbytes) = got
+                    r = requests[i]
+                    res = results[i]
+                    res.admitted = time.perf_counter() - t0
+                    img = Image.view(self.arena.base + off, r.side, r.side)
+                    img.fill((0, 0, 0, 0), lane=lane)
+                    scene.clear()
+                    _record(scene, r.calls)
+                    scene.render(img, lane=lane)
+                    if r.blur:
+                        img.blur(r.blur, lane=lane)
+                    lane_synchronize(lane)
+                    res.done = time.perf_counter() - t0
+                    res.latency = res.done - r.arrival
+                    if self.keep_pixels:
+                        res.pixels = img.download()
+                    del img
+                    with cv:
+                        self.arena.release(off, nbytes)
+                        cv.notify_all()
+            except Exception as e:  # surfaced to the caller below
+                with cv:
+                    state["error"] = e
+                    cv.notify_all()
+
+        threads = [threading.Thread(target=worker, args=(k,)) for k in range(self.n_lanes)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if state["error"] is not None:
+            raise state["error"]
+        return results
+
+
+def serve_plain(requests, keep_pixels=False):
+    """The same arrivals served one request at a time with one launch per
+    call (the plain per-call form) on lane 0."""
+    results = [Result(i) for i in range(len(requests))]
+    t0 = time.perf_counter() + 0.01
+    for i, r in enumerate(requests):
+        wait = t0 + r.arrival - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        res = results[i]
+        res.admitted = time.perf_counter() - t0
+        img = Image(r.side, r.side)
+        for c in r.calls:
+            if c.stroke:
+                img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                               miterLimit=c.miter, blendMode=c.mode)
+            else:
+                img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+        if r.blur:
+            img.blur(r.blur)
+        lane_synchronize(0)
+        res.done = time.perf_counter() - t0
+        res.latency = res.done - r.arrival
+        if keep_pixels:
+            res.pixels = img.download()
+    return results
+
+
+def latency_stats(results, requests):
+    import numpy as np
+    lat = np.array([r.latency for r in results]) * 1e3
+    span = max(r.done for r in results) - min(q.arrival for q in requests)
+    return {"scenes": len(results), "scenes_per_s": round(len(results) / span, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(float(np.percentile(lat, 99)), 3),
+            "max_ms": round(float(lat.max()), 3)}

@@ -234,3 +234,19 @@ def serve_scene(rng, max_paths=500):
                                 join=int(rng.integers(0, 3))))
     blur = 8 if rng.random() < 0.5 else 0
     return cmds, side, blur
+
+
+def serve_requests(seed=5, n=100, rate=100.0, max_paths=500, max_side=4096):
+    """Poisson arrivals at `rate` scenes/s of serving requests (serve_scene),
+    as serve.Request objects."""
+    from .serve import Request
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for _ in range(n):
+        t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
+        cmds, side, blur = serve_scene(rng, max_paths=max_paths)
+        side = min(side, max_side)
+        # paths were placed for the drawn side; a smaller cap clips them
+        out.append(Request(cmds, side, blur, t))
+    return out
