@@ -96,7 +96,8 @@ def glyph_extra(px, WL, reps=5, cpu_seconds=5.0):
     for cmds, x, y, w, h in cells:
         sc.fillPath(cmds, white)
     atlas = px.Image(aw, ah, a8=True)
-    sc.render(atlas)  # first render sizes the pools
+    shape, _ = sc.autotune(atlas)
+    sc.render(atlas)
     wall, touched, stages = timed(px, lambda: sc.render(atlas), reps)
     n = len(cells)
     kname, (kms, klaunch) = max(stages.items(), key=lambda kv: kv[1][0])
@@ -121,7 +122,7 @@ def glyph_extra(px, WL, reps=5, cpu_seconds=5.0):
             "n_touch_per_atlas": int(touched / reps),
             "stages_ms": {k2: round(v[0] / reps, 4) for k2, v in stages.items()},
             "algorithmic_gbs": round(algo_bytes / (wall / reps) / 1e9, 2),
-            "dominant_kernel": kname,
+            "dominant_kernel": kname, "phase_a_shape": shape,
             "plain_glyphs_per_s": round(n / pw, 1), "batched_speedup": round(n * reps / wall / (n / pw), 2),
             "cpu_glyphs_per_s": round(k / cdt, 1), "cpu_sample": "first %d glyphs, %.1f s, 1 core" % (k, cdt)}
 
@@ -283,10 +284,13 @@ def main():
     log("scene: %d calls generated in %.1fs" % (len(cmds), time.perf_counter() - t))
     img = px.Image.from_array(canvas)
 
+    shape = None
     if args.form == "batched":
         t = time.perf_counter()
         sc = record_scene(px, cmds)
         log("recorded %d calls in %.2fs" % (len(sc), time.perf_counter() - t))
+        shape, tune_ms = sc.autotune(img)
+        log("autotuner: phase-A shape %d (%.3f ms per render)" % (shape, tune_ms))
         step = lambda: sc.render(img)
     else:
         step = lambda: run_plain(px, img, cmds)
@@ -319,7 +323,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "mixed fill+stroke scene 4096x4096 RGBA8: %d cubic paths, "
                                "%d fill/stroke calls, seed 2" % (args.paths, len(cmds)),
-                   "form": args.form, "n_touch_per_step": int(per_step_touch)},
+                   "form": args.form, "n_touch_per_step": int(per_step_touch),
+                   "phase_a_shape": shape if args.form == "batched" else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": kname, "launches_per_step": klaunch // args.steps,

@@ -118,6 +118,12 @@ int pxh_scene_stroke_path(pxh_scene* scene, const float* cmds, int32_t ncmds, co
                           int32_t line_join, float miter_limit, int32_t blend_mode);
 /* Renders the recorded calls onto img (queued on the library's stream). */
 int pxh_scene_render(pxh_scene* scene, pxh_image* img);
+/* Autotuner (no reference counterpart): renders the scene onto a scratch
+ * image of img's size with each phase-A launch shape (band entries kept in
+ * LDS x row waves per CU), `reps` times each, and keeps the fastest for
+ * later renders.  *shape receives its index, *ms its time per render. */
+int pxh_scene_autotune(pxh_scene* scene, pxh_image* img, int32_t reps, int32_t* shape,
+                       float* ms);
 /* Per-call status of the last render: 0, or the PXH_ERR_* code the call
  * would have raised (e.g. PXH_ERR_INT_OVERFLOW); such calls draw nothing. */
 int pxh_scene_status(pxh_scene* scene, int32_t* status, int32_t n);

@@ -93,6 +93,7 @@ def lib():
         L.pxh_lanes.restype = i32
         L.pxh_lane_synchronize.argtypes = [i32]
         L.pxh_scene_render_on.argtypes = [vp, vp, i32]
+        L.pxh_scene_autotune.argtypes = [vp, vp, i32, i32p, P(ctypes.c_float)]
         L.pxh_blur_on.argtypes = [vp, f, u32, i32]
         L.pxh_image_fill_on.argtypes = [vp, u32, i32]
         L.pxh_image_view.restype = vp
@@ -325,6 +326,14 @@ class Scene:
     def render(self, image, lane=0):
         """Renders onto `image`, queued on execution lane `lane`."""
         _check(lib().pxh_scene_render_on(self._h, image.handle, lane))
+
+    def autotune(self, image, reps=3):
+        """Picks the fastest phase-A launch shape for renders onto images of
+        this size (on a scratch image).  Returns (shape index, ms per render)."""
+        k, ms = ctypes.c_int32(), ctypes.c_float()
+        _check(lib().pxh_scene_autotune(self._h, image.handle, reps, ctypes.byref(k),
+                                        ctypes.byref(ms)))
+        return k.value, ms.value
 
     def status(self):
         n = len(self)

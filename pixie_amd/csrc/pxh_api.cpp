@@ -32,6 +32,7 @@ struct pxh_scene {
   bool uploaded = false;
   int32_t validW = -1, validH = -1;  // target size the pools were validated for
   int32_t lane = 0;  // lane of the last render
+  int32_t rowShape = pxh::kDefaultRowShape;  // phase-A launch shape (autotuned)
   // device copies of the recording (grow-only)
   pxh::SceneItem* dItems = nullptr;
   pxh::SceneCmd* dCmds = nullptr;
@@ -350,7 +351,6 @@ int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius
 // ---- scene (batched form) ----
 
 const int kSceneGcap = 8192;  // band entries a wave can hold in global scratch
-const int kSceneLdsCap = 128; // band entries held in LDS (7 KB of LDS per row wave)
 
 // Converts pixie commands to the device's canonical absolute form
 // (commandsToShapes' command semantics, paths.nim:886-1057): relative,
@@ -537,6 +537,7 @@ int scene_alloc(pxh_scene* sc) {
   size_t oTask = carve((size_t)sc->taskCap * sizeof(RowRuns)), oRun = carve((size_t)sc->runCap * 8);
   size_t oTaskItem = carve((size_t)sc->taskCap * 4);
   size_t oRows = carve(nI * sizeof(int4)), oBinC = carve((size_t)sc->binSlots * 4);
+  size_t oPaint = carve(nI * sizeof(int4)), oSpan = carve(nI * sizeof(int4));
   size_t oBinO = carve(((size_t)sc->binSlots + 1) * 4), oBinL = carve((size_t)sc->binCap * 4);
   size_t oQ = carve(2 * kQueues * kQueueStride * 4);
   size_t oCnt = carve(kNumCounters * 4);
@@ -579,6 +580,8 @@ int scene_alloc(pxh_scene* sc) {
   p.runs = (int2*)(b + oRun);
   p.runCap = sc->runCap;
   p.itemRows = (int4*)(b + oRows);
+  p.itemPaint = (int4*)(b + oPaint);
+  p.itemSpan = (int4*)(b + oSpan);
   p.binCount = (int32_t*)(b + oBinC);
   p.binOff = (int32_t*)(b + oBinO);
   p.binList = (int32_t*)(b + oBinL);
@@ -587,7 +590,6 @@ int scene_alloc(pxh_scene* sc) {
   p.queues = (int32_t*)(b + oQ);
   p.counters = (int32_t*)(b + oCnt);
   p.gcap = kSceneGcap;
-  p.ldsCap = kSceneLdsCap;
   return 0;
 }
 
@@ -626,7 +628,7 @@ int scene_upload(pxh_scene* sc, int lane) {
   sc->entryCap = sc->segCap * 2;
   sc->taskCap = (int32_t)std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28);
   sc->runCap = (int32_t)std::min<int64_t>(
-      (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kSceneRowWavesPerCU) * kRunChunk,
+      (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kMaxRowWavesPerCU) * kRunChunk,
       (int64_t)1 << 30);
   sc->binCap = (int32_t)std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28);
   sc->uploaded = true;
@@ -642,7 +644,9 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   p.items = sc->dItems;
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
-  const int rowGrid = scene_queue_grid(g.numCUs * kSceneRowWavesPerCU);
+  const RowShape shape = kRowShapes[sc->rowShape];
+  const int rowGrid = scene_queue_grid(g.numCUs * shape.wavesPerCU);
+  p.ldsCap = shape.ldsCap;
   DevBuf& scr = g.laneScratch[lane];
   if (int e = scr.ensure(scene_gscratch_bytes(rowGrid, kSceneGcap)))
     return fail(e, "scene row scratch allocation failed");
@@ -1082,7 +1086,7 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
     // the run pool must also hold every wave's partly used chunk, whatever
     // order the waves take tasks in on later renders
     const int64_t runNeed =
-        (int64_t)cnt[kCntRunsUsed] + (int64_t)scene_queue_grid(g.numCUs * kSceneRowWavesPerCU) * kRunChunk;
+        (int64_t)cnt[kCntRunsUsed] + (int64_t)scene_queue_grid(g.numCUs * kMaxRowWavesPerCU) * kRunChunk;
     if (!cnt[kCntError]) {  // this render is complete; later ones get the slack
       if (runNeed > sc->runCap) {
         sc->runCap = (int32_t)std::min<int64_t>(runNeed + runNeed / 4, (int64_t)1 << 30);
@@ -1107,6 +1111,49 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
     if (int r = scene_alloc(sc)) return r;
   }
   return fail(kErrCapacity, "scene pools could not be sized");
+}
+
+// Times every phase-A launch shape on a scratch image of img's size and
+// keeps the fastest for later renders of this scene.
+int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* shape_out,
+                       float* ms_out) {
+  PXH_INIT();
+  if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (int r = check_image(img)) return r;
+  if (reps < 1) reps = 1;
+  pxh_image* tmp = img->fmt == PXH_FORMAT_A8 ? pxh_image_new_a8(img->w, img->h) : pxh_image_new(img->w, img->h);
+  if (!tmp) return kErrNoMem;
+  int best = sc->rowShape;
+  float bestMs = 0;
+  for (int k = 0; k < kNumRowShapes; k++) {
+    sc->rowShape = k;
+    int rc = pxh_scene_render_on(sc, tmp, 0);  // also sizes the pools for this shape
+    float total = 0;
+    for (int r = 0; rc == 0 && r < reps; r++) {
+      if (hipEventRecord(g.ev[14], g.stream) != hipSuccess) rc = kErrDevice;
+      if (!rc) rc = scene_launch(sc, tmp, 0);
+      if (!rc && hipEventRecord(g.ev[15], g.stream) != hipSuccess) rc = kErrDevice;
+      float ms = 0;
+      if (!rc && (hipEventSynchronize(g.ev[15]) != hipSuccess ||
+                  hipEventElapsedTime(&ms, g.ev[14], g.ev[15]) != hipSuccess))
+        rc = kErrDevice;
+      total += ms;
+    }
+    if (rc) {
+      pxh_image_free(tmp);
+      sc->rowShape = best;
+      return rc;
+    }
+    if (k == 0 || total / reps < bestMs) {
+      bestMs = total / reps;
+      best = k;
+    }
+  }
+  pxh_image_free(tmp);
+  sc->rowShape = best;
+  if (shape_out) *shape_out = best;
+  if (ms_out) *ms_out = bestMs;
+  return 0;
 }
 
 // Per-call status of the last render (0 ok, else the PixieError code of that

@@ -144,6 +144,8 @@ struct ScenePools {
   int2* runs;
   int32_t runCap;
   int4* itemRows;            // per call: first row, end row, x range (empty if it draws nothing)
+  int4* itemPaint;           // per call: rgbx, mode, first task, windows per row
+  int4* itemSpan;            // per call: rowStart, rowCount, xlo, xhi
   int32_t* binCount;         // nBins
   int32_t* binOff;           // nBins + 1
   int32_t* binList;          // call indices per bin
@@ -157,9 +159,17 @@ struct ScenePools {
   int32_t ldsCap;            // entries that fit in LDS
 };
 
-// Phase-A row waves per CU: the row program is latency-bound, so the grid
-// keeps as many waves resident as LDS (about 7 KB each) and registers allow.
-constexpr int kSceneRowWavesPerCU = 20;
+// Phase-A launch shapes: band entries a row wave keeps in LDS, and row waves
+// per CU.  The row program is latency-bound, so the grid keeps about as many
+// waves resident as LDS and registers allow for that capacity.  A scene's
+// autotuner picks one shape; run-pool sizing assumes the largest grid.
+struct RowShape {
+  int ldsCap, wavesPerCU;
+};
+constexpr RowShape kRowShapes[] = {{64, 24}, {128, 20}, {256, 14}, {512, 9}};
+constexpr int kNumRowShapes = 4;
+constexpr int kDefaultRowShape = 1;
+constexpr int kMaxRowWavesPerCU = 24;
 
 int scene_set_lds_limits(size_t bytes);
 size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows

@@ -577,6 +577,9 @@ __global__ void __launch_bounds__(kScanBlock) scene_plan_kernel(ScenePools P) {
       g.bandBase = bandTot + bOff;
       g.taskBase = taskTot + tOff;
       P.geo[it] = g;
+      // compact per-call record for phase B
+      P.itemPaint[it] = make_int4((int)f.rgbx, f.mode, g.taskBase, g.nWin);
+      P.itemSpan[it] = make_int4(g.rowStart, g.rowCount, f.xlo, f.xhi);
       if (g.bandBase + nb <= P.bandCap)
         for (int b = 0; b < nb; b++) P.bandItem[g.bandBase + b] = it;
     }
@@ -973,16 +976,21 @@ __global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
 // ---------------------------------------------------------------------------
 // phase B: canvas row windows in registers, calls applied in order
 // ---------------------------------------------------------------------------
+// The canvas window lives in LDS (1024 px) while the calls are applied, so
+// that one copy of the per-pixel blend code serves every pixel: the kernel
+// stays small enough for the instruction cache.
 template <bool kA8>
 __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* canvas,
                                                           unsigned long long* touchedOut) {
-  __shared__ int32_t D[kWinB + 4];
+  __shared__ __align__(16) int32_t D[kWinB + 4];
   __shared__ uint32_t AX[kWinB];
+  __shared__ uint4 WIN[kWinB / 4];
   const int lane = threadIdx.x;
   for (int i = lane; i < kWinB + 4; i += 64) D[i] = 0;
   const int W = P.W, H = P.H;
   const int winsPerRow = (W + kWinB - 1) / kWinB;
   const int nUnits = H * winsPerRow;
+  const bool vec = ((W & 3) == 0);
   uint32_t ntouch = 0;
   if (P.counters[kCntError]) return;
   for (;;) {
@@ -991,74 +999,89 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
     const int y = unit / winsPerRow;
     const int b0 = (unit % winsPerRow) * kWinB;
     const int b1 = b0 + kWinB < W ? b0 + kWinB : W;
-    const bool vec = ((W & 3) == 0);
-    uint32_t px[kChunksB][4];
     bool loaded = false, dirty = false;
     const int bin = y / kBinRows;
     const int l1 = P.binOff[bin + 1];
     for (int i0 = P.binOff[bin]; i0 < l1; i0 += 64) {
       const int li = i0 + lane;
-      int it = 0;
       bool cov = false;
+      int4 paint = make_int4(0, 0, 0, 0), span = paint;
       if (li < l1) {
-        it = P.binList[li];
-        int4 r = P.itemRows[it];
+        const int it = P.binList[li];
+        const int4 r = P.itemRows[it];
         cov = y >= r.x && y < r.y && r.w > b0 && r.z < b1;
+        if (cov) {  // each lane fetches its call's record: the loads overlap
+          paint = P.itemPaint[it];
+          span = P.itemSpan[it];
+        }
       }
       uint64_t m = __ballot(cov);
       while (m) {
-        const int it2 = __shfl(it, __builtin_ctzll(m), 64);
+        const int src = __builtin_ctzll(m);
         m &= m - 1;
-        const FillInfo f = P.fills[it2];
-        const ItemGeom g = P.geo[it2];
-        if (!loaded) {
+        const uint32_t rgbx = (uint32_t)__builtin_amdgcn_readlane(paint.x, src);
+        const int mode = __builtin_amdgcn_readlane(paint.y, src);
+        const int taskBase = __builtin_amdgcn_readlane(paint.z, src);
+        const int nWin = __builtin_amdgcn_readlane(paint.w, src);
+        const int rowStart = __builtin_amdgcn_readlane(span.x, src);
+        const int rowCount = __builtin_amdgcn_readlane(span.y, src);
+        const int xlo = __builtin_amdgcn_readlane(span.z, src);
+        const int xhi = __builtin_amdgcn_readlane(span.w, src);
+        if (!loaded) {  // canvas window -> LDS
 #pragma unroll
           for (int c = 0; c < kChunksB; c++) {
-            int x0 = b0 + c * 256 + 4 * lane;
+            const int x0 = b0 + c * 256 + 4 * lane;
+            uint4 v = make_uint4(0, 0, 0, 0);
             if (kA8) {
               const uint8_t* row = (const uint8_t*)canvas + (size_t)y * W;
+              uint32_t u[4] = {0, 0, 0, 0};
               if (vec && x0 + 3 < b1) {
-                uint32_t v = *(const uint32_t*)(row + x0);
+                uint32_t w4 = *(const uint32_t*)(row + x0);
 #pragma unroll
-                for (int q = 0; q < 4; q++) px[c][q] = ((v >> (8 * q)) & 255u) * 0x01010101u;
+                for (int q = 0; q < 4; q++) u[q] = ((w4 >> (8 * q)) & 255u) * 0x01010101u;
               } else {
 #pragma unroll
-                for (int q = 0; q < 4; q++) px[c][q] = (x0 + q < b1) ? row[x0 + q] * 0x01010101u : 0u;
+                for (int q = 0; q < 4; q++) u[q] = (x0 + q < b1) ? row[x0 + q] * 0x01010101u : 0u;
               }
+              v = make_uint4(u[0], u[1], u[2], u[3]);
             } else {
               const uint32_t* row = (const uint32_t*)canvas + (size_t)y * W;
               if (vec && x0 + 3 < b1) {
-                uint4 v = *(const uint4*)(row + x0);
-                px[c][0] = v.x; px[c][1] = v.y; px[c][2] = v.z; px[c][3] = v.w;
+                v = *(const uint4*)(row + x0);
               } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) px[c][q] = (x0 + q < b1) ? row[x0 + q] : 0u;
+                v.x = (x0 < b1) ? row[x0] : 0u;
+                v.y = (x0 + 1 < b1) ? row[x0 + 1] : 0u;
+                v.z = (x0 + 2 < b1) ? row[x0 + 2] : 0u;
+                v.w = (x0 + 3 < b1) ? row[x0 + 3] : 0u;
               }
             }
+            WIN[c * 64 + lane] = v;
           }
           loaded = true;
         }
-        const bool mask = f.mode == kMask;
-        if (mask && !(y >= g.rowStart && y < g.rowStart + g.rowCount)) {
+        const bool mask = mode == kMask;
+        if (mask && !(y >= rowStart && y < rowStart + rowCount)) {
           // rows outside a mask fill's path are cleared (paths.nim:1910-1912)
-#pragma unroll
-          for (int c = 0; c < kChunksB; c++)
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              int x = b0 + c * 256 + 4 * lane + q;
-              if (x < b1) {
-                px[c][q] = 0;
-                ntouch++;
-              }
-            }
+          for (int c = 0; c < kChunksB; c++) {
+            WIN[c * 64 + lane] = make_uint4(0, 0, 0, 0);
+            const int x0 = b0 + c * 256 + 4 * lane;
+            const int n = b1 - x0;
+            ntouch += n <= 0 ? 0 : (n >= 4 ? 4 : n);
+          }
           dirty = true;
           continue;
         }
-        // gather this call's runs for row y into D / AX
-        const int tb = g.taskBase + (y - g.rowStart) * g.nWin;
+        // gather this call's runs for row y into D / AX.  Every phase-A
+        // window's deltas sum to zero, so windows that end before b0 or
+        // start at b1 or later change nothing here.
+        const int tb = taskBase + (y - rowStart) * nWin;
+        const int winBase = mask ? 0 : (xlo & ~3);
         int carry = 0;
         int kind = kRowNone;
-        for (int w = 0; w < g.nWin; w++) {
+        for (int w = 0; w < nWin; w++) {
+          const int wx0 = winBase + w * kWinA;
+          if (wx0 >= b1) break;
+          if (wx0 + kWinA < b0) continue;
           RowRuns rr = P.tasks[tb + w];
           if (rr.kind > kind) kind = rr.kind;
           for (int e0 = 0; e0 < rr.n; e0 += 64) {
@@ -1076,73 +1099,88 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
             }
           }
         }
-        for (int o = 32; o > 0; o >>= 1) carry += __shfl_xor(carry, o, 64);
+        carry = wave_incl_sum(carry);
+        carry = __builtin_amdgcn_readlane(carry, kWave - 1);
         __syncthreads();
         if (kind == kRowNone) continue;
-        const int lo = mask ? b0 : (f.xlo > b0 ? f.xlo : b0);
-        const int hi = mask ? b1 : (f.xhi < b1 ? f.xhi : b1);
+        const int lo = mask ? b0 : (xlo > b0 ? xlo : b0);
+        const int hi = mask ? b1 : (xhi < b1 ? xhi : b1);
         const int cLo = (lo - b0) / 256;
         int cHi = (hi - b0) / 256;
         if (cHi > kChunksB - 1) cHi = kChunksB - 1;
-#pragma unroll
-        for (int c = 0; c < kChunksB; c++) {
-          if (c < cLo || c > cHi) continue;
-          int i0 = c * 256 + 4 * lane;
-          int32_t d0 = D[i0], d1 = D[i0 + 1], d2 = D[i0 + 2], d3 = D[i0 + 3];
-          D[i0] = 0;
-          D[i0 + 1] = 0;
-          D[i0 + 2] = 0;
-          D[i0 + 3] = 0;
-          int32_t c0 = d0, c1 = c0 + d1, c2 = c1 + d2, c3 = c2 + d3;
+        for (int c = cLo; c <= cHi; c++) {
+          const int i0 = c * 256 + 4 * lane;
+          int4 dv = *(int4*)&D[i0];
+          *(int4*)&D[i0] = make_int4(0, 0, 0, 0);
+          int32_t c0 = dv.x, c1 = c0 + dv.y, c2 = c1 + dv.z, c3 = c2 + dv.w;
           int32_t tot;
           int32_t off = carry + wave_excl_scan(c3, &tot);
           carry += tot;
-          int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
-#pragma unroll
+          uint4 pv = WIN[c * 64 + lane];
+          // four pixels through one copy of the blend code: rotate the
+          // pixel / code registers instead of indexing them
+          uint32_t p0 = pv.x, p1 = pv.y, p2 = pv.z, p3 = pv.w;
+          int32_t k0 = off + c0, k1 = off + c1, k2 = off + c2, k3 = off + c3;
+#pragma unroll 1
           for (int q = 0; q < 4; q++) {
-            int x = b0 + i0 + q;
-            if (x < lo || x >= hi) continue;
-            uint32_t ax = (kind == kRowOps && code[q] >= kArea && code[q] < kClear) ? AX[i0 + q] : 0u;
-            bool t = false;
-            px[c][q] = apply_code(kind, code[q], ax, px[c][q], f.rgbx, f.mode, &t);
-            if (t) {
-              ntouch++;
-              dirty = true;
+            const int x = b0 + i0 + q;
+            if (x >= lo && x < hi) {
+              uint32_t ax = (kind == kRowOps && k0 >= kArea && k0 < kClear) ? AX[i0 + q] : 0u;
+              bool t = false;
+              p0 = apply_code(kind, k0, ax, p0, rgbx, mode, &t);
+              if (t) {
+                ntouch++;
+                dirty = true;
+              }
             }
+            uint32_t tp = p0;
+            p0 = p1;
+            p1 = p2;
+            p2 = p3;
+            p3 = tp;
+            int32_t tk = k0;
+            k0 = k1;
+            k1 = k2;
+            k2 = k3;
+            k3 = tk;
           }
+          WIN[c * 64 + lane] = make_uint4(p0, p1, p2, p3);
         }
         __syncthreads();
       }
     }
-    if (__ballot(dirty) != 0) {
+    if (__ballot(dirty) != 0) {  // LDS window -> canvas
 #pragma unroll
       for (int c = 0; c < kChunksB; c++) {
-        int x0 = b0 + c * 256 + 4 * lane;
+        const int x0 = b0 + c * 256 + 4 * lane;
+        const uint4 v = WIN[c * 64 + lane];
         if (kA8) {
           uint8_t* row = (uint8_t*)canvas + (size_t)y * W;
           if (vec && x0 + 3 < b1) {
-            *(uint32_t*)(row + x0) = ch_a(px[c][0]) | (ch_a(px[c][1]) << 8) | (ch_a(px[c][2]) << 16) |
-                                     (ch_a(px[c][3]) << 24);
+            *(uint32_t*)(row + x0) = ch_a(v.x) | (ch_a(v.y) << 8) | (ch_a(v.z) << 16) | (ch_a(v.w) << 24);
           } else {
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (x0 + q < b1) row[x0 + q] = (uint8_t)ch_a(px[c][q]);
+            if (x0 < b1) row[x0] = (uint8_t)ch_a(v.x);
+            if (x0 + 1 < b1) row[x0 + 1] = (uint8_t)ch_a(v.y);
+            if (x0 + 2 < b1) row[x0 + 2] = (uint8_t)ch_a(v.z);
+            if (x0 + 3 < b1) row[x0 + 3] = (uint8_t)ch_a(v.w);
           }
         } else {
           uint32_t* row = (uint32_t*)canvas + (size_t)y * W;
           if (vec && x0 + 3 < b1) {
-            *(uint4*)(row + x0) = make_uint4(px[c][0], px[c][1], px[c][2], px[c][3]);
+            *(uint4*)(row + x0) = v;
           } else {
-#pragma unroll
-            for (int q = 0; q < 4; q++)
-              if (x0 + q < b1) row[x0 + q] = px[c][q];
+            if (x0 < b1) row[x0] = v.x;
+            if (x0 + 1 < b1) row[x0 + 1] = v.y;
+            if (x0 + 2 < b1) row[x0 + 2] = v.z;
+            if (x0 + 3 < b1) row[x0 + 3] = v.w;
           }
         }
       }
     }
   }
   if (touchedOut) {
-    for (int o = 32; o > 0; o >>= 1) ntouch += __shfl_xor(ntouch, o, 64);
+    ntouch = (uint32_t)wave_incl_sum((int32_t)ntouch);
+    ntouch = __builtin_amdgcn_readlane(ntouch, kWave - 1);
     if (lane == 0 && ntouch) atomicAdd(touchedOut, (unsigned long long)ntouch);
   }
 }

@@ -385,3 +385,24 @@ def test_scene_glyph_atlas(gpu, orc):
     for cmds, x, y, w, h in cells:
         mask[y:y + h, x:x + w] = True
     assert not got[~mask].any()
+
+
+def test_scene_autotune_keeps_results(gpu):
+    cmds, canvas = WL.mixed_scene(seed=9, n_paths=80, W=700, H=500)
+    sc = px.Scene()
+    ref = px.Image.from_array(canvas)
+    for c in cmds:
+        if c.stroke:
+            sc.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                          miterLimit=c.miter, blendMode=c.mode)
+            ref.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                           miterLimit=c.miter, blendMode=c.mode)
+        else:
+            sc.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+            ref.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    img = px.Image.from_array(canvas)
+    shape, ms = sc.autotune(img, reps=2)
+    assert 0 <= shape < 4 and ms > 0
+    _exact(img.download(), canvas, "autotune leaves the image alone")
+    sc.render(img)
+    _exact(img.download(), ref.download(), "render after autotune")
