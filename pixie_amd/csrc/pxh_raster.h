@@ -115,6 +115,24 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
   return __builtin_amdgcn_readlane(wave_incl_max(v), kWave - 1);
 }
 
+// Scans inside each 16-lane row of the wave (four independent segments).
+__device__ __forceinline__ int32_t row16_incl_sum(int32_t v) {
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr1, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr2, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr4, 0xf, 0xf, false);
+  v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr8, 0xf, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ int32_t row16_excl_max(int32_t v) {
+  const int32_t lo = INT32_MIN;
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr1, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr2, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr4, 0xf, 0xf, false));
+  v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr8, 0xf, 0xf, false));
+  return __builtin_amdgcn_update_dpp(lo, v, kDppRowShr1, 0xf, 0xf, false);
+}
+
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
 
 // Bitonic sort of n keys in L.keys (n <= cap, cap a power of two).  Keys
@@ -367,6 +385,97 @@ __device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, const S
   return nh;
 }
 
+// The 5-sample coverage of a row whose selection holds at most 16 entries
+// (computeCoverage 1350-1431 with walk 1298-1334): four sample lines at a
+// time, one per 16-lane row of the wave, so a row takes two passes instead
+// of five.  Lane 16g + i works on entry i for sample line g of the pass.
+// Each segment is one whole sample line, so the walk needs no carries
+// between chunks.  S holds selection i in lane i.
+__device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const SelCache& S,
+                                   const float* yLines, int32_t rule, CovSpan& span) {
+  const int lane = lane_id();
+  const int grp = lane >> 4, i = lane & 15;
+  const float Wf = (float)C.F->canvasW;
+  // entry i of the selection in every segment
+  Entry e;
+  e.aty = __shfl(S.e.aty, i, kWave);
+  e.toy = __shfl(S.e.toy, i, kWave);
+  e.m = __shfl(S.e.m, i, kWave);
+  e.b = __shfl(S.e.b, i, kWave);
+  const int32_t ew = __shfl(S.w, i, kWave);
+  uint64_t* sorted = (uint64_t*)L.tx0;  // 64 sorted keys (tx0/tx1 are free here)
+  for (int pass = 0; pass < 2; pass++) {
+    const int sample = pass * 4 + grp;
+    float yl = yLines[0];
+    if (sample == 1) yl = yLines[1];
+    if (sample == 2) yl = yLines[2];
+    if (sample == 3) yl = yLines[3];
+    if (sample == 4) yl = yLines[4];
+    // hits (1373-1388), compacted per segment in entry order
+    bool hit = false;
+    int32_t at = 0;
+    if (sample < 5 && i < ne && e.aty <= yl && e.toy >= yl) {
+      float x = e.m == 0.0f ? e.b : (yl - e.b) / e.m;
+      at = fixed32(nim_minf(x, Wf));
+      hit = true;
+    }
+    const uint64_t m = ballot(hit);
+    const uint32_t segMask = (uint32_t)(m >> (16 * grp)) & 0xffffu;
+    const int nh = __popc(segMask);
+    const int pos = __popc(segMask & ((1u << i) - 1u));
+    const uint64_t key = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | (uint32_t)pos;
+    if (hit) {
+      L.keys[16 * grp + pos] = key;
+      L.B[16 * grp + pos] = ew;
+    }
+    wave_sync();
+    // stable sort by (x, position): rank among the segment's hits
+    if (hit) {
+      int rank = 0;
+      for (int j = 0; j < nh; j++) rank += L.keys[16 * grp + j] < key ? 1 : 0;
+      sorted[16 * grp + rank] = key;
+    }
+    wave_sync();
+    // walk: lane 16g + p holds the p-th hit of segment g
+    const bool valid = i < nh;
+    const uint64_t k = valid ? sorted[16 * grp + i] : 0ull;
+    const int32_t hat = valid ? key_at(k) : 0;
+    const int32_t w = valid ? L.B[16 * grp + (uint32_t)k] : 0;
+    const int32_t countBefore = row16_incl_sum(w) - w;
+    const bool filling = should_fill(rule, countBefore);
+    const bool hasNext = valid && i < nh - 1;
+    const uint64_t kn = hasNext ? sorted[16 * grp + i + 1] : 0ull;
+    const int32_t nextAt = hasNext ? key_at(kn) : 0;
+    const int32_t nextW = hasNext ? L.B[16 * grp + (uint32_t)kn] : 0;
+    const bool canSkip = valid && hat > 0 && filling && hasNext && nextAt == hat && w + nextW == 0;
+    uint64_t cm = ballot(canSkip);
+    uint64_t skipMask = 0, consMask = 0;
+    while (cm) {
+      const int b = __builtin_ctzll(cm);
+      cm &= cm - 1;
+      if (!((consMask >> b) & 1ull)) {
+        skipMask |= 1ull << b;
+        if (((b + 1) & 15) != 0) consMask |= 1ull << (b + 1);  // stays in its segment
+      }
+    }
+    const bool consumed = (consMask >> lane) & 1ull;
+    const bool skipStart = (skipMask >> lane) & 1ull;
+    int ev = 0;  // 0 none, 1 start, 2 end (yield)
+    if (valid && !consumed && !skipStart && hat > 0) {
+      if (filling) {
+        const bool cont = (rule == kNonZero) && hasNext && (countBefore + w != 0);
+        ev = cont ? 0 : 2;
+      } else {
+        ev = 1;
+      }
+    }
+    int32_t prevAt = row16_excl_max(ev ? hat : INT32_MIN);
+    if (prevAt < 0) prevAt = 0;  // prevAt starts at 0 (walk 1300)
+    if (ev == 2) span(prevAt, hat);
+    wave_sync();
+  }
+}
+
 // Runs the row program for (fill, row y, window) and leaves the row's codes
 // in L.D (and L.AX).  Returns the row kind; kRowNone means nothing to do and
 // -1 means the band holds more entries than L.cap (nothing was written).
@@ -578,6 +687,15 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   if (aa) {
     if (mask && lane == 0) add_run(C, L, covLo, covHi, -kClear);  // coverage range
     CovSpan cs{&C, &L, sampleCoverage, (int32_t)covLo, (int32_t)covHi};
+    if (S.inReg && ne <= 16 && L.cap >= kWave) {
+      float yLines[5];
+      for (int m = 0; m < 5; m++) {
+        yLine += offset;
+        yLines[m] = yLine;
+      }
+      row_coverage_small(Cp, L, ne, S, yLines, F.rule, cs);
+      return kRowCoverage;
+    }
     for (int m = 0; m < quality; m++) {
       yLine += offset;
       int nh = row_hits(Cp, L, ne, yLine, S);
