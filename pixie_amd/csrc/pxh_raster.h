@@ -115,22 +115,27 @@ __device__ __forceinline__ int32_t wave_max(int32_t v) {
   return __builtin_amdgcn_readlane(wave_incl_max(v), kWave - 1);
 }
 
-// Scans inside each 16-lane row of the wave (four independent segments).
-__device__ __forceinline__ int32_t row16_incl_sum(int32_t v) {
+// Scans inside independent SEG-lane segments of the wave (SEG = 16 or 32).
+template <int SEG>
+__device__ __forceinline__ int32_t seg_incl_sum(int32_t v) {
   v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr1, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr2, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr4, 0xf, 0xf, false);
   v += __builtin_amdgcn_update_dpp(0, v, kDppRowShr8, 0xf, 0xf, false);
+  if (SEG == 32) v += __builtin_amdgcn_update_dpp(0, v, kDppRowBcast15, 0xa, 0xf, false);
   return v;
 }
 
-__device__ __forceinline__ int32_t row16_excl_max(int32_t v) {
+template <int SEG>
+__device__ __forceinline__ int32_t seg_excl_max(int32_t v) {
   const int32_t lo = INT32_MIN;
   v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr1, 0xf, 0xf, false));
   v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr2, 0xf, 0xf, false));
   v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr4, 0xf, 0xf, false));
   v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowShr8, 0xf, 0xf, false));
-  return __builtin_amdgcn_update_dpp(lo, v, kDppRowShr1, 0xf, 0xf, false);
+  if (SEG == 32) v = imax(v, __builtin_amdgcn_update_dpp(lo, v, kDppRowBcast15, 0xa, 0xf, false));
+  v = __builtin_amdgcn_update_dpp(lo, v, kDppWaveShr1, 0xf, 0xf, false);
+  return (lane_id() & (SEG - 1)) == 0 ? lo : v;
 }
 
 __device__ __forceinline__ void wave_sync() { __syncthreads(); }
@@ -385,16 +390,19 @@ __device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, const S
   return nh;
 }
 
-// The 5-sample coverage of a row whose selection holds at most 16 entries
-// (computeCoverage 1350-1431 with walk 1298-1334): four sample lines at a
-// time, one per 16-lane row of the wave, so a row takes two passes instead
-// of five.  Lane 16g + i works on entry i for sample line g of the pass.
-// Each segment is one whole sample line, so the walk needs no carries
-// between chunks.  S holds selection i in lane i.
+// The 5-sample coverage of a row whose selection holds at most SEG entries
+// (computeCoverage 1350-1431 with walk 1298-1334), 64 / SEG sample lines at a
+// time, one per SEG-lane segment of the wave: two passes for SEG = 16, three
+// for SEG = 32, instead of five.  Lane SEG*g + i works on entry i for sample
+// line g of the pass.  A segment holds one whole sample line, so the walk
+// needs no carries between chunks.  S holds selection i in lane i.
+template <int SEG>
 __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const SelCache& S,
                                    const float* yLines, int32_t rule, CovSpan& span) {
+  constexpr int kPer = kWave / SEG;                 // sample lines per pass
+  constexpr int kPasses = (5 + kPer - 1) / kPer;
   const int lane = lane_id();
-  const int grp = lane >> 4, i = lane & 15;
+  const int grp = lane / SEG, i = lane & (SEG - 1);
   const float Wf = (float)C.F->canvasW;
   // entry i of the selection in every segment
   Entry e;
@@ -404,8 +412,9 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
   e.b = __shfl(S.e.b, i, kWave);
   const int32_t ew = __shfl(S.w, i, kWave);
   uint64_t* sorted = (uint64_t*)L.tx0;  // 64 sorted keys (tx0/tx1 are free here)
-  for (int pass = 0; pass < 2; pass++) {
-    const int sample = pass * 4 + grp;
+  const int base = SEG * grp;
+  for (int pass = 0; pass < kPasses; pass++) {
+    const int sample = pass * kPer + grp;
     float yl = yLines[0];
     if (sample == 1) yl = yLines[1];
     if (sample == 2) yl = yLines[2];
@@ -420,33 +429,33 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
       hit = true;
     }
     const uint64_t m = ballot(hit);
-    const uint32_t segMask = (uint32_t)(m >> (16 * grp)) & 0xffffu;
-    const int nh = __popc(segMask);
-    const int pos = __popc(segMask & ((1u << i) - 1u));
+    const uint64_t segMask = (m >> base) & (SEG == 32 ? 0xffffffffull : 0xffffull);
+    const int nh = __popcll(segMask);
+    const int pos = __popcll(segMask & ((1ull << i) - 1ull));
     const uint64_t key = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | (uint32_t)pos;
     if (hit) {
-      L.keys[16 * grp + pos] = key;
-      L.B[16 * grp + pos] = ew;
+      L.keys[base + pos] = key;
+      L.B[base + pos] = ew;
     }
     wave_sync();
     // stable sort by (x, position): rank among the segment's hits
     if (hit) {
       int rank = 0;
-      for (int j = 0; j < nh; j++) rank += L.keys[16 * grp + j] < key ? 1 : 0;
-      sorted[16 * grp + rank] = key;
+      for (int j = 0; j < nh; j++) rank += L.keys[base + j] < key ? 1 : 0;
+      sorted[base + rank] = key;
     }
     wave_sync();
-    // walk: lane 16g + p holds the p-th hit of segment g
+    // walk: lane SEG*g + p holds the p-th hit of segment g
     const bool valid = i < nh;
-    const uint64_t k = valid ? sorted[16 * grp + i] : 0ull;
+    const uint64_t k = valid ? sorted[base + i] : 0ull;
     const int32_t hat = valid ? key_at(k) : 0;
-    const int32_t w = valid ? L.B[16 * grp + (uint32_t)k] : 0;
-    const int32_t countBefore = row16_incl_sum(w) - w;
+    const int32_t w = valid ? L.B[base + (uint32_t)k] : 0;
+    const int32_t countBefore = seg_incl_sum<SEG>(w) - w;
     const bool filling = should_fill(rule, countBefore);
     const bool hasNext = valid && i < nh - 1;
-    const uint64_t kn = hasNext ? sorted[16 * grp + i + 1] : 0ull;
+    const uint64_t kn = hasNext ? sorted[base + i + 1] : 0ull;
     const int32_t nextAt = hasNext ? key_at(kn) : 0;
-    const int32_t nextW = hasNext ? L.B[16 * grp + (uint32_t)kn] : 0;
+    const int32_t nextW = hasNext ? L.B[base + (uint32_t)kn] : 0;
     const bool canSkip = valid && hat > 0 && filling && hasNext && nextAt == hat && w + nextW == 0;
     uint64_t cm = ballot(canSkip);
     uint64_t skipMask = 0, consMask = 0;
@@ -455,7 +464,7 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
       cm &= cm - 1;
       if (!((consMask >> b) & 1ull)) {
         skipMask |= 1ull << b;
-        if (((b + 1) & 15) != 0) consMask |= 1ull << (b + 1);  // stays in its segment
+        if (((b + 1) & (SEG - 1)) != 0) consMask |= 1ull << (b + 1);  // stays in its segment
       }
     }
     const bool consumed = (consMask >> lane) & 1ull;
@@ -469,7 +478,7 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
         ev = 1;
       }
     }
-    int32_t prevAt = row16_excl_max(ev ? hat : INT32_MIN);
+    int32_t prevAt = seg_excl_max<SEG>(ev ? hat : INT32_MIN);
     if (prevAt < 0) prevAt = 0;  // prevAt starts at 0 (walk 1300)
     if (ev == 2) span(prevAt, hat);
     wave_sync();
@@ -687,13 +696,15 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   if (aa) {
     if (mask && lane == 0) add_run(C, L, covLo, covHi, -kClear);  // coverage range
     CovSpan cs{&C, &L, sampleCoverage, (int32_t)covLo, (int32_t)covHi};
+    // (two sample lines per pass for selections of 17-32 entries measured
+    // slower than the five single passes, so only 16-entry rows take it)
     if (S.inReg && ne <= 16 && L.cap >= kWave) {
       float yLines[5];
       for (int m = 0; m < 5; m++) {
         yLine += offset;
         yLines[m] = yLine;
       }
-      row_coverage_small(Cp, L, ne, S, yLines, F.rule, cs);
+      row_coverage_small<16>(Cp, L, ne, S, yLines, F.rule, cs);
       return kRowCoverage;
     }
     for (int m = 0; m < quality; m++) {
