@@ -984,7 +984,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
                                                           unsigned long long* touchedOut) {
   __shared__ __align__(16) int32_t D[kWinB + 4];
   __shared__ uint32_t AX[kWinB];
-  __shared__ uint4 WIN[kWinB / 4];
+  __shared__ uint32_t WIN[kWinB];
   const int lane = threadIdx.x;
   for (int i = lane; i < kWinB + 4; i += 64) D[i] = 0;
   const int W = P.W, H = P.H;
@@ -1055,7 +1055,11 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
                 v.w = (x0 + 3 < b1) ? row[x0 + 3] : 0u;
               }
             }
-            WIN[c * 64 + lane] = v;
+            const int i4 = c * 256 + 4 * lane;
+            WIN[i4] = v.x;
+            WIN[i4 + 1] = v.y;
+            WIN[i4 + 2] = v.z;
+            WIN[i4 + 3] = v.w;
           }
           loaded = true;
         }
@@ -1063,7 +1067,8 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         if (mask && !(y >= rowStart && y < rowStart + rowCount)) {
           // rows outside a mask fill's path are cleared (paths.nim:1910-1912)
           for (int c = 0; c < kChunksB; c++) {
-            WIN[c * 64 + lane] = make_uint4(0, 0, 0, 0);
+            const int i4 = c * 256 + 4 * lane;
+            WIN[i4] = WIN[i4 + 1] = WIN[i4 + 2] = WIN[i4 + 3] = 0u;
             const int x0 = b0 + c * 256 + 4 * lane;
             const int n = b1 - x0;
             ntouch += n <= 0 ? 0 : (n >= 4 ? 4 : n);
@@ -1108,43 +1113,31 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         const int cLo = (lo - b0) / 256;
         int cHi = (hi - b0) / 256;
         if (cHi > kChunksB - 1) cHi = kChunksB - 1;
+        // pixel codes in 64-px slices (lane l takes pixel l of the slice), so
+        // a slice no pixel of this call needs is skipped by the whole wave
+        const bool maskCov = kind == kRowCoverage && mask;  // every pixel is written
         for (int c = cLo; c <= cHi; c++) {
-          const int i0 = c * 256 + 4 * lane;
-          int4 dv = *(int4*)&D[i0];
-          *(int4*)&D[i0] = make_int4(0, 0, 0, 0);
-          int32_t c0 = dv.x, c1 = c0 + dv.y, c2 = c1 + dv.z, c3 = c2 + dv.w;
-          int32_t tot;
-          int32_t off = carry + wave_excl_scan(c3, &tot);
-          carry += tot;
-          uint4 pv = WIN[c * 64 + lane];
-          // four pixels through one copy of the blend code: rotate the
-          // pixel / code registers instead of indexing them
-          uint32_t p0 = pv.x, p1 = pv.y, p2 = pv.z, p3 = pv.w;
-          int32_t k0 = off + c0, k1 = off + c1, k2 = off + c2, k3 = off + c3;
 #pragma unroll 1
           for (int q = 0; q < 4; q++) {
-            const int x = b0 + i0 + q;
-            if (x >= lo && x < hi) {
-              uint32_t ax = (kind == kRowOps && k0 >= kArea && k0 < kClear) ? AX[i0 + q] : 0u;
+            const int idx = c * 256 + q * 64 + lane;
+            const int x = b0 + idx;
+            const int32_t d = D[idx];
+            D[idx] = 0;
+            const int32_t incl = wave_incl_sum(d);
+            const int32_t code = carry + incl;
+            carry += __builtin_amdgcn_readlane(incl, kWave - 1);
+            const bool work = x >= lo && x < hi && (code != 0 || maskCov);
+            if (__ballot(work) == 0) continue;
+            if (work) {
+              uint32_t ax = (kind == kRowOps && code >= kArea && code < kClear) ? AX[idx] : 0u;
               bool t = false;
-              p0 = apply_code(kind, k0, ax, p0, rgbx, mode, &t);
+              WIN[idx] = apply_code(kind, code, ax, WIN[idx], rgbx, mode, &t);
               if (t) {
                 ntouch++;
                 dirty = true;
               }
             }
-            uint32_t tp = p0;
-            p0 = p1;
-            p1 = p2;
-            p2 = p3;
-            p3 = tp;
-            int32_t tk = k0;
-            k0 = k1;
-            k1 = k2;
-            k2 = k3;
-            k3 = tk;
           }
-          WIN[c * 64 + lane] = make_uint4(p0, p1, p2, p3);
         }
         __syncthreads();
       }
@@ -1153,7 +1146,8 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
 #pragma unroll
       for (int c = 0; c < kChunksB; c++) {
         const int x0 = b0 + c * 256 + 4 * lane;
-        const uint4 v = WIN[c * 64 + lane];
+        const int i4 = c * 256 + 4 * lane;
+        const uint4 v = make_uint4(WIN[i4], WIN[i4 + 1], WIN[i4 + 2], WIN[i4 + 3]);
         if (kA8) {
           uint8_t* row = (uint8_t*)canvas + (size_t)y * W;
           if (vec && x0 + 3 < b1) {
