@@ -58,25 +58,29 @@ def record_scene(px, cmds):
     return sc
 
 
-def timed(px, fn, steps):
+def timed(px, fn, steps, profile=True):
     """Runs fn() `steps` times between device synchronisations; returns
-    (wall seconds, N_touch, {stage: ms}) with per-launch event timing."""
+    (wall seconds, N_touch, {stage: ms}).  With profile, every launch is
+    bracketed by HIP events on its stream (renders then run eagerly rather
+    than as graph replays)."""
     px.synchronize()
     px.touched_pixels(reset=True)
-    px.profile_begin()
+    if profile:
+        px.profile_begin()
     t0 = time.perf_counter()
     for k in range(steps):
         fn()
         log("step %d queued" % k)
     px.synchronize()
     wall = time.perf_counter() - t0
-    px.profile_end()
-    touched = px.touched_pixels(reset=True)
     stages = {}
-    for name in STAGES:
-        ms, n = px.profile_kernel(name)
-        if n:
-            stages[name] = (ms, n)
+    if profile:
+        px.profile_end()
+        for name in STAGES:
+            ms, n = px.profile_kernel(name)
+            if n:
+                stages[name] = (ms, n)
+    touched = px.touched_pixels(reset=True)
     return wall, touched, stages
 
 
@@ -98,7 +102,8 @@ def glyph_extra(px, WL, reps=5, cpu_seconds=5.0):
     atlas = px.Image(aw, ah, a8=True)
     shape, _ = sc.autotune(atlas)
     sc.render(atlas)
-    wall, touched, stages = timed(px, lambda: sc.render(atlas), reps)
+    wall, touched, _ = timed(px, lambda: sc.render(atlas), reps, profile=False)
+    _, _, stages = timed(px, lambda: sc.render(atlas), reps, profile=True)
     n = len(cells)
     kname, (kms, klaunch) = max(stages.items(), key=lambda kv: kv[1][0])
     # input points: the command array minus one kind per command, two floats each
@@ -297,7 +302,10 @@ def main():
     for _ in range(args.warmup):
         step()
     px.synchronize()
-    wall, touched, stages = timed(px, step, args.steps)
+    # the value: K renders as they run in production (graph replays for the
+    # batched form); the stage times: the same K renders, each launch timed
+    wall, touched, _ = timed(px, step, args.steps, profile=False)
+    _, _, stages = timed(px, step, args.steps, profile=True)
 
     per_step_touch = touched / args.steps
     ms_per_step = wall / args.steps * 1e3
@@ -324,7 +332,8 @@ def main():
         "config": {"workload": "mixed fill+stroke scene 4096x4096 RGBA8: %d cubic paths, "
                                "%d fill/stroke calls, seed 2" % (args.paths, len(cmds)),
                    "form": args.form, "n_touch_per_step": int(per_step_touch),
-                   "phase_a_shape": shape if args.form == "batched" else None},
+                   "phase_a_shape": shape if args.form == "batched" else None,
+                   "graph_replay": sc.graph_active() if args.form == "batched" else None},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                      "kernel": kname, "launches_per_step": klaunch // args.steps,

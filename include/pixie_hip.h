@@ -124,6 +124,9 @@ int pxh_scene_render(pxh_scene* scene, pxh_image* img);
  * later renders.  *shape receives its index, *ms its time per render. */
 int pxh_scene_autotune(pxh_scene* scene, pxh_image* img, int32_t reps, int32_t* shape,
                        float* ms);
+/* 1 when renders of this scene replay a captured HIP graph (after the first
+ * render at a given image, lane and launch shape), 0 when they run eagerly. */
+int32_t pxh_scene_graph_active(const pxh_scene* scene);
 /* Per-call status of the last render: 0, or the PXH_ERR_* code the call
  * would have raised (e.g. PXH_ERR_INT_OVERFLOW); such calls draw nothing. */
 int pxh_scene_status(pxh_scene* scene, int32_t* status, int32_t n);

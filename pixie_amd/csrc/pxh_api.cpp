@@ -33,6 +33,22 @@ struct pxh_scene {
   int32_t validW = -1, validH = -1;  // target size the pools were validated for
   int32_t lane = 0;  // lane of the last render
   int32_t rowShape = pxh::kDefaultRowShape;  // phase-A launch shape (autotuned)
+  // a validated render replays as a captured graph while its target, lane,
+  // shape and buffers stay the same
+  hipGraphExec_t exec = nullptr;
+  bool graphsOff = false;
+  uint64_t gen = 0;  // bumped whenever the recording or the pools change
+  struct GraphKey {
+    void* dev;
+    int32_t w, h, fmt, lane, shape;
+    const void* mem;
+    const void* scratch;
+    uint64_t gen;
+    bool operator==(const GraphKey& o) const {
+      return dev == o.dev && w == o.w && h == o.h && fmt == o.fmt && lane == o.lane &&
+             shape == o.shape && mem == o.mem && scratch == o.scratch && gen == o.gen;
+    }
+  } key{};
   // device copies of the recording (grow-only)
   pxh::SceneItem* dItems = nullptr;
   pxh::SceneCmd* dCmds = nullptr;
@@ -194,17 +210,18 @@ hipEvent_t pool_event() {
 // Brackets one kernel launch with events when profiling is on.
 struct LaunchTimer {
   const char* name;
+  hipStream_t st;
   hipEvent_t a = nullptr, b = nullptr;
-  explicit LaunchTimer(const char* kernel) : name(kernel) {
+  explicit LaunchTimer(const char* kernel, hipStream_t s = nullptr) : name(kernel), st(s ? s : g.stream) {
     if (g.profiling) {
       a = pool_event();
       b = pool_event();
-      (void)hipEventRecord(a, g.stream);
+      (void)hipEventRecord(a, st);
     }
   }
   ~LaunchTimer() {
     if (a) {
-      (void)hipEventRecord(b, g.stream);
+      (void)hipEventRecord(b, st);
       g.evPairs.push_back({name, a, b});
     }
   }
@@ -501,11 +518,18 @@ int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9,
   it.strokeNone = (stroke && width <= 0.0f) ? 1 : 0;
   sc->items.push_back(it);
   sc->uploaded = false;
+  sc->gen++;
   sc->validW = sc->validH = -1;
   return 0;
 }
 
+void scene_drop_graph(pxh_scene* sc) {
+  if (sc->exec) (void)hipGraphExecDestroy(sc->exec);
+  sc->exec = nullptr;
+}
+
 void scene_free_device(pxh_scene* sc) {
+  scene_drop_graph(sc);
   if (sc->dItems) (void)hipFree(sc->dItems);
   if (sc->dCmds) (void)hipFree(sc->dCmds);
   if (sc->dPolys) (void)hipFree(sc->dPolys);
@@ -520,6 +544,7 @@ void scene_free_device(pxh_scene* sc) {
 
 // (Re)allocates the per-render pools of a scene in one device block.
 int scene_alloc(pxh_scene* sc) {
+  sc->gen++;
   const size_t nI = sc->items.size(), nC = sc->cmds.size(), nP = sc->polys.size();
   size_t off = 0;
   auto carve = [&](size_t bytes) {
@@ -651,32 +676,65 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   if (int e = scr.ensure(scene_gscratch_bytes(rowGrid, kSceneGcap)))
     return fail(e, "scene row scratch allocation failed");
   p.gscratch = scr.p;
-  PXH_HIP(hipMemsetAsync(p.counters, 0, kNumCounters * 4, st));
-  PXH_HIP(hipMemsetAsync(p.queues, 0, 2 * kQueues * kQueueStride * 4, st));
-  int r;
-  {
-    LaunchTimer t("scene_geometry");
+  int r = scene_launch_clear(p, st);
+  if (!r) {
+    LaunchTimer t("scene_geometry", st);
     r = scene_launch_geometry(p, st);
   }
   if (!r) {
-    LaunchTimer t("scene_bands");
+    LaunchTimer t("scene_bands", st);
     r = scene_launch_bands(p, g.numCUs * 16, st);
   }
   if (!r) {
-    LaunchTimer t("scene_bins");
+    LaunchTimer t("scene_bins", st);
     r = scene_launch_bins(p, g.numCUs * 4, st);
   }
   if (!r) {
-    LaunchTimer t("scene_rows_kernel");
+    LaunchTimer t("scene_rows_kernel", st);
     r = scene_launch_rows(p, rowGrid, st);
   }
   if (!r) {
-    LaunchTimer t("scene_compose_kernel");
+    LaunchTimer t("scene_compose_kernel", st);
     r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched,
                              scene_queue_grid(g.numCUs * 16),
                              st);
   }
   if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
+  return 0;
+}
+
+// A render of a validated scene: replays the captured graph of this render
+// (capturing it first when anything it depends on changed).  Profiled
+// renders run eagerly so that every launch can be timed.
+int scene_replay(pxh_scene* sc, pxh_image* img, int lane) {
+  if (g.profiling || sc->graphsOff) return scene_launch(sc, img, lane);
+  hipStream_t st = g.lane[lane];
+  const RowShape shape = kRowShapes[sc->rowShape];
+  DevBuf& scr = g.laneScratch[lane];
+  if (int e = scr.ensure(scene_gscratch_bytes(scene_queue_grid(g.numCUs * shape.wavesPerCU), kSceneGcap)))
+    return fail(e, "scene row scratch allocation failed");
+  pxh_scene::GraphKey key{img->dev, img->w, img->h, img->fmt, lane, sc->rowShape, sc->dMem, scr.p, sc->gen};
+  if (!sc->exec || !(key == sc->key)) {
+    scene_drop_graph(sc);
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      sc->graphsOff = true;
+      return scene_launch(sc, img, lane);
+    }
+    int r = scene_launch(sc, img, lane);
+    hipError_t e = hipStreamEndCapture(st, &graph);
+    if (r != 0 || e != hipSuccess || !graph ||
+        hipGraphInstantiate(&sc->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      sc->exec = nullptr;
+      sc->graphsOff = true;
+      (void)hipGetLastError();
+      return scene_launch(sc, img, lane);
+    }
+    (void)hipGraphDestroy(graph);
+    sc->key = key;
+  }
+  PXH_HIP(hipGraphLaunch(sc->exec, st));
   return 0;
 }
 
@@ -1033,6 +1091,7 @@ void pxh_scene_free(pxh_scene* sc) {
 int pxh_scene_clear(pxh_scene* sc) {
   if (!sc) return fail(kErrArg, "invalid scene handle");
   if (g.inited) PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
+  scene_drop_graph(sc);
   sc->items.clear();
   sc->cmds.clear();
   sc->polys.clear();
@@ -1075,7 +1134,7 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
     sc->validW = sc->validH = -1;
     if (int r = scene_alloc(sc)) return r;
   }
-  if (sc->validW == img->w && sc->validH == img->h) return scene_launch(sc, img, lane);
+  if (sc->validW == img->w && sc->validH == img->h) return scene_replay(sc, img, lane);
   // First render at this size: check the pools and grow them until the render
   // fits.  A render that overflows a pool stops before touching the canvas.
   for (int attempt = 0; attempt < 8; attempt++) {
@@ -1112,6 +1171,9 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
   }
   return fail(kErrCapacity, "scene pools could not be sized");
 }
+
+// 1 when the scene's last render replayed a captured graph.
+int32_t pxh_scene_graph_active(const pxh_scene* sc) { return sc && sc->exec && !sc->graphsOff ? 1 : 0; }
 
 // Times every phase-A launch shape on a scratch image of img's size and
 // keeps the fastest for later renders of this scene.

@@ -173,6 +173,7 @@ constexpr int kMaxRowWavesPerCU = 24;
 
 int scene_set_lds_limits(size_t bytes);
 size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows
+int scene_launch_clear(const ScenePools& p, hipStream_t s);
 int scene_launch_geometry(const ScenePools& p, hipStream_t s);
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s);

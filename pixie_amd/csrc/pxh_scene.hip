@@ -1192,6 +1192,20 @@ int scene_set_lds_limits(size_t bytes) {
 
 size_t scene_gscratch_bytes(int rowGrid, int gcap) { return (size_t)rowGrid * row_cap_bytes(gcap); }
 
+// Clears the counters and queue heads of a render.  A kernel rather than a
+// memset, so that a render captured in a graph holds kernel nodes only.
+__global__ void __launch_bounds__(256) scene_clear_kernel(int32_t* counters, int nc, int32_t* queues,
+                                                          int nq) {
+  for (int i = threadIdx.x; i < nc; i += blockDim.x) counters[i] = 0;
+  for (int i = threadIdx.x; i < nq; i += blockDim.x) queues[i] = 0;
+}
+
+int scene_launch_clear(const ScenePools& p, hipStream_t s) {
+  hipLaunchKernelGGL(scene_clear_kernel, dim3(1), dim3(256), 0, s, p.counters, (int)kNumCounters,
+                     p.queues, 2 * kQueues * kQueueStride);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
 int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
   if (p.nItems <= 0) return 0;
   hipLaunchKernelGGL(scene_geom_kernel, dim3(p.nItems), dim3(kBlock), 0, s, p);

@@ -350,6 +350,8 @@ def test_scene_repeated_renders_are_identical(gpu):
         _exact(img.download(), want, "render %d" % k)
         touches.append(px.touched_pixels(reset=True))
     assert len(set(touches)) == 1, touches
+    # renders after the first replay a captured graph (kernel nodes only)
+    assert sc.graph_active()
     # renders accumulate on the same image exactly like repeated calls
     img = px.Image.from_array(canvas)
     sc.render(img)
@@ -406,3 +408,33 @@ def test_scene_autotune_keeps_results(gpu):
     _exact(img.download(), canvas, "autotune leaves the image alone")
     sc.render(img)
     _exact(img.download(), ref.download(), "render after autotune")
+
+
+def test_scene_full_mixed_4k(gpu, orc):
+    # the benchmark's own workload at full size (4096x4096, 2,000 paths,
+    # 2,606 calls): batched form against the per-call form (bit for bit,
+    # same N_touch) and against the oracle (coverage bar)
+    cmds, canvas = WL.mixed_scene(n_paths=2000)
+    calls = []
+    for c in cmds:
+        if c.stroke:
+            calls.append((True, c.cmds, None, dict(color=c.color, strokeWidth=c.width,
+                                                   lineCap=c.cap, lineJoin=c.join,
+                                                   miterLimit=c.miter, blendMode=c.mode)))
+        else:
+            calls.append((False, c.cmds, None, dict(color=c.color, windingRule=c.winding,
+                                                    blendMode=c.mode)))
+    _compare(gpu, calls, canvas, "mixed 4096x4096 scene", orc)
+
+
+def test_scene_full_glyph_atlas(gpu):
+    # the full 10,000-glyph atlas: batched render against one launch per glyph
+    cells, aw, ah = WL.glyph_atlas()
+    sc = px.Scene()
+    ref = px.Image(aw, ah, a8=True)
+    for cmds, x, y, w, h in cells:
+        sc.fillPath(cmds, (255, 255, 255, 255))
+        ref.fillPath(cmds, (255, 255, 255, 255))
+    img = px.Image(aw, ah, a8=True)
+    sc.render(img)
+    _exact(img.download(), ref.download(), "10k-glyph atlas")
