@@ -613,9 +613,18 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
     int cnt = 0;
     bool aa = false;
     const int s0 = g.segBase, s1 = g.segBase + g.segCount;  // s0 is 64-aligned
-    for (int c = s0 >> 6; (c << 6) < s1; c++) {
-      const int2 cr = P.segChunk[c];
-      if (k < cr.x || k > cr.y) continue;
+    const int cEnd = (s1 + 63) >> 6;
+    for (int cb = s0 >> 6; cb < cEnd; cb += 64) {
+      // chunks that may reach band k, 64 chunk ranges at a time
+      bool may = false;
+      if (cb + lane < cEnd) {
+        const int2 cr = P.segChunk[cb + lane];
+        may = k >= cr.x && k <= cr.y;
+      }
+      uint64_t cm = __ballot(may);
+      while (cm) {
+      const int c = cb + __builtin_ctzll(cm);
+      cm &= cm - 1;
       const int j = (c << 6) + lane;
       bool in = false, a = false;
       if (j < s1) {
@@ -625,6 +634,7 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
       }
       cnt += __popcll(__ballot(in));
       if (__ballot(a)) aa = true;
+      }
     }
     if (lane == 0) {
       PartInfo pi;
@@ -701,9 +711,17 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
     const int s0 = g.segBase, s1 = g.segBase + g.segCount;
     if (pi.count != 2) {
       int pos = 0;
-      for (int c = s0 >> 6; (c << 6) < s1 && pos < pi.count; c++) {
-        const int2 cr = P.segChunk[c];
-        if (k < cr.x || k > cr.y) continue;
+      const int cEnd = (s1 + 63) >> 6;
+      for (int cb = s0 >> 6; cb < cEnd && pos < pi.count; cb += 64) {
+        bool may = false;
+        if (cb + lane < cEnd) {
+          const int2 cr = P.segChunk[cb + lane];
+          may = k >= cr.x && k <= cr.y;
+        }
+        uint64_t cm = __ballot(may);
+        while (cm && pos < pi.count) {
+        const int c = cb + __builtin_ctzll(cm);
+        cm &= cm - 1;
         const int j = (c << 6) + lane;
         bool in = false;
         if (j < s1) {
@@ -719,6 +737,7 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
           P.entW[pi.entryBase + p] = P.segW[j];
         }
         pos += __popcll(m);
+        }
       }
       continue;
     }
@@ -726,9 +745,17 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
     // two-spanning flag and order (1250-1262), then write
     float4 g0 = make_float4(0, 0, 0, 0), g1 = g0;
     int w0 = 0, w1 = 0, found = 0;
-    for (int c = s0 >> 6; (c << 6) < s1 && found < 2; c++) {
-      const int2 cr = P.segChunk[c];
-      if (k < cr.x || k > cr.y) continue;
+    const int cEnd2 = (s1 + 63) >> 6;
+    for (int cb = s0 >> 6; cb < cEnd2 && found < 2; cb += 64) {
+      bool may = false;
+      if (cb + lane < cEnd2) {
+        const int2 cr = P.segChunk[cb + lane];
+        may = k >= cr.x && k <= cr.y;
+      }
+      uint64_t cm = __ballot(may);
+      while (cm && found < 2) {
+      const int c = cb + __builtin_ctzll(cm);
+      cm &= cm - 1;
       const int j = (c << 6) + lane;
       bool in = false;
       float4 sg = make_float4(0, 0, 0, 0);
@@ -756,6 +783,7 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
           w1 = tw;
         }
         found++;
+      }
       }
     }
     float4 a, c;
