@@ -269,6 +269,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-glyphs", action="store_true")
+    ap.add_argument("--shape", type=int, default=None,
+                    help="fixed phase-A launch shape instead of the autotuner's choice (profiling)")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = 20 if args.form == "batched" else 3
@@ -294,8 +296,11 @@ def main():
         t = time.perf_counter()
         sc = record_scene(px, cmds)
         log("recorded %d calls in %.2fs" % (len(sc), time.perf_counter() - t))
-        shape, tune_ms = sc.autotune(img)
-        log("autotuner: phase-A shape %d (%.3f ms per render)" % (shape, tune_ms))
+        if args.shape is None:
+            shape, tune_ms = sc.autotune(img)
+            log("autotuner: phase-A shape %d (%.3f ms per render)" % (shape, tune_ms))
+        else:
+            shape = sc.set_shape(args.shape)
         step = lambda: sc.render(img)
     else:
         step = lambda: run_plain(px, img, cmds)

@@ -95,6 +95,7 @@ def lib():
         L.pxh_scene_render_on.argtypes = [vp, vp, i32]
         L.pxh_scene_autotune.argtypes = [vp, vp, i32, i32p, P(ctypes.c_float)]
         L.pxh_scene_graph_active.argtypes = [vp]
+        L.pxh_scene_set_shape.argtypes = [vp, i32]
         L.pxh_blur_on.argtypes = [vp, f, u32, i32]
         L.pxh_image_fill_on.argtypes = [vp, u32, i32]
         L.pxh_image_view.restype = vp
@@ -335,6 +336,11 @@ class Scene:
         _check(lib().pxh_scene_autotune(self._h, image.handle, reps, ctypes.byref(k),
                                         ctypes.byref(ms)))
         return k.value, ms.value
+
+    def set_shape(self, shape):
+        """Fixes the phase-A launch shape (what autotune() chooses)."""
+        _check(lib().pxh_scene_set_shape(self._h, shape))
+        return shape
 
     def graph_active(self):
         """Whether renders replay a captured HIP graph."""

@@ -1172,6 +1172,13 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
   return fail(kErrCapacity, "scene pools could not be sized");
 }
 
+// Sets the phase-A launch shape (as the autotuner would).
+int pxh_scene_set_shape(pxh_scene* sc, int32_t shape) {
+  if (!sc || shape < 0 || shape >= kNumRowShapes) return fail(kErrArg, "invalid scene or shape");
+  sc->rowShape = shape;
+  return 0;
+}
+
 // 1 when the scene's last render replayed a captured graph.
 int32_t pxh_scene_graph_active(const pxh_scene* sc) { return sc && sc->exec && !sc->graphsOff ? 1 : 0; }
 
