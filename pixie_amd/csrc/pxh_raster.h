@@ -351,7 +351,7 @@ struct SelCache {
 // Collects the hits of one sample line (paths.nim:1373-1388) from the
 // selection L.A[0..ne) into L.keys sorted stably by x; windings go to L.B by
 // hit position.  Returns the number of hits.
-__device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, const SelCache& S) {
+__device__ __forceinline__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, SelCache S) {
   int lane = lane_id();
   const float Wf = (float)C.F->canvasW;
   int nh = 0;
@@ -397,8 +397,9 @@ __device__ int row_hits(const RowCtx& C, RowLds& L, int ne, float yLine, const S
 // line g of the pass.  A segment holds one whole sample line, so the walk
 // needs no carries between chunks.  S holds selection i in lane i.
 template <int SEG>
-__device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const SelCache& S,
-                                   const float* yLines, int32_t rule, CovSpan& span) {
+__device__ __forceinline__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, SelCache S, float y0,
+                                   float y1, float y2, float y3, float y4, int32_t rule,
+                                   CovSpan& span) {
   constexpr int kPer = kWave / SEG;                 // sample lines per pass
   constexpr int kPasses = (5 + kPer - 1) / kPer;
   const int lane = lane_id();
@@ -415,11 +416,8 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
   const int base = SEG * grp;
   for (int pass = 0; pass < kPasses; pass++) {
     const int sample = pass * kPer + grp;
-    float yl = yLines[0];
-    if (sample == 1) yl = yLines[1];
-    if (sample == 2) yl = yLines[2];
-    if (sample == 3) yl = yLines[3];
-    if (sample == 4) yl = yLines[4];
+    // the sample line of this segment, selected from registers
+    const float yl = sample == 0 ? y0 : sample == 1 ? y1 : sample == 2 ? y2 : sample == 3 ? y3 : y4;
     // hits (1373-1388), compacted per segment in entry order
     bool hit = false;
     int32_t at = 0;
@@ -481,6 +479,115 @@ __device__ void row_coverage_small(const RowCtx& C, RowLds& L, int ne, const Sel
     int32_t prevAt = seg_excl_max<SEG>(ev ? hat : INT32_MIN);
     if (prevAt < 0) prevAt = 0;  // prevAt starts at 0 (walk 1300)
     if (ev == 2) span(prevAt, hat);
+    wave_sync();
+  }
+}
+
+// Finishes sorting n distinct keys that are nearly in order: odd-even
+// transposition passes until one swaps nothing, and a full sort once
+// kFixPasses passes have not been enough.
+constexpr int kFixPasses = 4;
+__device__ void wave_fix_keys(uint64_t* keys, int n, int cap) {
+  const int lane = lane_id();
+  for (int pass = 0;; pass++) {
+    bool swapped = false;
+    for (int par = 0; par < 2; par++) {
+      for (int i = 2 * lane + par; i + 1 < n; i += 2 * kWave) {
+        const uint64_t a = keys[i], b = keys[i + 1];
+        if (a > b) {
+          keys[i] = b;
+          keys[i + 1] = a;
+          swapped = true;
+        }
+      }
+      wave_sync();
+    }
+    if (ballot(swapped) == 0) return;
+    if (pass + 1 >= kFixPasses) {
+      wave_sort_keys(keys, n, cap);
+      return;
+    }
+  }
+}
+
+// The 5-sample coverage of a row with any number of selected entries
+// (computeCoverage 1350-1431).  pixie sorts each sample line's hits from
+// scratch (sort 1373-1388).  Here the selection is sorted once, by x where
+// each entry is nearest the row's middle; every sample line then collects
+// its hits in that order, which leaves them sorted except where entries cross
+// inside the row, and wave_fix_keys finishes the sort.  The hit keys carry
+// the selection index as tie-break, which orders equal x like pixie's
+// insertion sort over hits collected in selection order.
+__device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& L, int ne, SelCache S,
+                                       float yLine, float offset, int32_t rule, CovSpan& span) {
+  const int lane = lane_id();
+  const float Wf = (float)C.F->canvasW;
+  const float yMid = (float)C.y + 0.5f;
+  int32_t* order = (int32_t*)L.tx0;  // selection index by presorted position
+  int32_t* entry = (int32_t*)L.tx1;  // entry index by presorted position
+  for (int i = lane; i < ne; i += kWave) {
+    Entry e;
+    int32_t w;
+    if (S.inReg) {
+      e = S.e;
+      w = S.w;
+    } else {
+      const int k = L.A[i];
+      e = C.E[k];
+      w = C.EW[k];
+    }
+    const float yc = yMid < e.aty ? e.aty : (yMid > e.toy ? e.toy : yMid);
+    L.keys[i] = ((uint64_t)float_key(solve_x(e.m, e.b, yc)) << 32) | (uint32_t)i;
+    L.B[i] = w;  // windings by selection index, read by the walk
+  }
+  wave_sync();
+  if (ne > 1) wave_sort_keys(L.keys, ne, L.cap);
+  for (int j = lane; j < ne; j += kWave) {
+    const int s = (int)(uint32_t)L.keys[j];
+    order[j] = s;
+    if (!S.inReg) entry[j] = L.A[s];
+  }
+  wave_sync();
+  // a selection held in registers moves to its presorted lanes
+  Entry er = S.e;
+  int32_t sr = lane;
+  if (S.inReg) {
+    sr = lane < ne ? order[lane] : 0;
+    er.aty = __shfl(S.e.aty, sr, kWave);
+    er.toy = __shfl(S.e.toy, sr, kWave);
+    er.m = __shfl(S.e.m, sr, kWave);
+    er.b = __shfl(S.e.b, sr, kWave);
+  }
+  for (int m = 0; m < 5; m++) {
+    yLine += offset;
+    int nh = 0;
+    for (int base = 0; base < ne; base += kWave) {
+      const int j = base + lane;
+      bool hit = false;
+      int32_t at = 0, s = 0;
+      if (j < ne) {
+        Entry e;
+        if (S.inReg) {
+          e = er;
+          s = sr;
+        } else {
+          e = C.E[entry[j]];
+          s = order[j];
+        }
+        if (e.aty <= yLine && e.toy >= yLine) {
+          const float x = e.m == 0.0f ? e.b : (yLine - e.b) / e.m;
+          at = fixed32(nim_minf(x, Wf));
+          hit = true;
+        }
+      }
+      const uint64_t mk = ballot(hit);
+      const int pos = nh + __popcll(mk & ((1ull << lane) - 1ull));
+      if (hit) L.keys[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | (uint32_t)s;
+      nh += __popcll(mk);
+    }
+    wave_sync();
+    if (nh > 1) wave_fix_keys(L.keys, nh, L.cap);
+    wave_walk(L, nh, rule, span);
     wave_sync();
   }
 }
@@ -553,19 +660,23 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     }
   }
   wave_sync();
-  SelCache S;
-  S.inReg = ne <= kWave;
-  S.e = Entry{0.0f, 0.0f, 0.0f, 0.0f};
-  S.w = 0;
-  if (S.inReg && lane < ne) {
-    int k = L.A[lane];
-    S.e = E[k];
-    S.w = EW[k];
+  const bool inReg = ne <= kWave;
+  float sAty = 0.0f, sToy = 0.0f, sM = 0.0f, sB = 0.0f;
+  int32_t sW = 0;
+  if (inReg && lane < ne) {
+    const int k = L.A[lane];
+    sAty = E[k].aty;
+    sToy = E[k].toy;
+    sM = E[k].m;
+    sB = E[k].b;
+    sW = EW[k];
   }
+  const SelCache S{inReg, Entry{sAty, sToy, sM, sB}, sW};
 
   if (allSpan && (ne % 2) == 0) {  // trapezoid fast path, 1691-1872
     for (int i = lane; i < ne; i += kWave) {
-      Entry e = S.inReg ? S.e : E[L.A[i]];
+      Entry e = S.e;  // (a conditional copy, not a ?: on the two lvalues: that would keep S in memory)
+      if (!S.inReg) e = E[L.A[i]];
       float a = solve_x(e.m, e.b, scanTop), b = solve_x(e.m, e.b, scanBottom);
       L.tx0[i] = a;
       L.tx1[i] = b;
@@ -699,20 +810,12 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     // (two sample lines per pass for selections of 17-32 entries measured
     // slower than the five single passes, so only 16-entry rows take it)
     if (S.inReg && ne <= 16 && L.cap >= kWave) {
-      float yLines[5];
-      for (int m = 0; m < 5; m++) {
-        yLine += offset;
-        yLines[m] = yLine;
-      }
-      row_coverage_small<16>(Cp, L, ne, S, yLines, F.rule, cs);
+      const float y0 = yLine + offset, y1 = y0 + offset, y2 = y1 + offset, y3 = y2 + offset,
+                  y4 = y3 + offset;
+      row_coverage_small<16>(Cp, L, ne, S, y0, y1, y2, y3, y4, F.rule, cs);
       return kRowCoverage;
     }
-    for (int m = 0; m < quality; m++) {
-      yLine += offset;
-      int nh = row_hits(Cp, L, ne, yLine, S);
-      wave_walk(L, nh, F.rule, cs);
-      wave_sync();
-    }
+    row_coverage_presorted(Cp, L, ne, S, yLine, offset, F.rule, cs);
     return kRowCoverage;
   }
   yLine += offset;

@@ -889,7 +889,7 @@ __global__ void __launch_bounds__(64) scene_task_map_kernel(ScenePools P) {
   }
 }
 
-__global__ void __launch_bounds__(64) scene_rows_kernel(ScenePools P) {
+__global__ void __launch_bounds__(64, 4) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x;
   const size_t fixedBytes = row_lds_fixed_bytes(kWinA);
@@ -1013,6 +1013,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
   __shared__ __align__(16) int32_t D[kWinB + 4];
   __shared__ uint32_t AX[kWinB];
   __shared__ uint32_t WIN[kWinB];
+  __shared__ int32_t Q[2 * kWave];  // pixels queued for the blend
   const int lane = threadIdx.x;
   for (int i = lane; i < kWinB + 4; i += 64) D[i] = 0;
   const int W = P.W, H = P.H;
@@ -1141,13 +1142,19 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         const int cLo = (lo - b0) / 256;
         int cHi = (hi - b0) / 256;
         if (cHi > kChunksB - 1) cHi = kChunksB - 1;
-        // pixel codes in 64-px slices (lane l takes pixel l of the slice), so
-        // a slice no pixel of this call needs is skipped by the whole wave
+        // Pixel codes in 64-px slices (lane l takes pixel l of the slice).
+        // The pixels this call changes are queued, packed as code << 11 |
+        // index, and the blend runs on full batches of 64 queued pixels, so
+        // partly covered slices do not leave lanes idle in the blend.  A
+        // call changes each pixel at most once, so a batch never holds the
+        // same pixel twice.
         const bool maskCov = kind == kRowCoverage && mask;  // every pixel is written
-        for (int c = cLo; c <= cHi; c++) {
-#pragma unroll 1
-          for (int q = 0; q < 4; q++) {
-            const int idx = c * 256 + q * 64 + lane;
+        int s = cLo * 4;
+        const int sEnd = cHi * 4 + 4;
+        int nq = 0;
+        for (;;) {
+          while (s < sEnd && nq < kWave) {
+            const int idx = s * 64 + lane;
             const int x = b0 + idx;
             const int32_t d = D[idx];
             D[idx] = 0;
@@ -1155,17 +1162,29 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
             const int32_t code = carry + incl;
             carry += __builtin_amdgcn_readlane(incl, kWave - 1);
             const bool work = x >= lo && x < hi && (code != 0 || maskCov);
-            if (__ballot(work) == 0) continue;
-            if (work) {
-              uint32_t ax = (kind == kRowOps && code >= kArea && code < kClear) ? AX[idx] : 0u;
-              bool t = false;
-              WIN[idx] = apply_code(kind, code, ax, WIN[idx], rgbx, mode, &t);
-              if (t) {
-                ntouch++;
-                dirty = true;
-              }
+            const uint64_t wm = __ballot(work);
+            const int pos = nq + __builtin_amdgcn_mbcnt_hi((uint32_t)(wm >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)wm, 0u));
+            if (work) Q[pos] = code * 2048 + idx;
+            nq += __popcll(wm);
+            s++;
+          }
+          if (nq == 0) break;
+          if (lane < nq) {
+            const int32_t v = Q[lane];
+            const int idx = v & 2047;
+            const int32_t code = v >> 11;
+            uint32_t ax = (kind == kRowOps && code >= kArea && code < kClear) ? AX[idx] : 0u;
+            bool t = false;
+            WIN[idx] = apply_code(kind, code, ax, WIN[idx], rgbx, mode, &t);
+            if (t) {
+              ntouch++;
+              dirty = true;
             }
           }
+          // keep the overflow (queue positions 64 .. nq-1) for the next batch
+          nq = nq > kWave ? nq - kWave : 0;
+          if (lane < nq) Q[lane] = Q[kWave + lane];
         }
         __syncthreads();
       }
