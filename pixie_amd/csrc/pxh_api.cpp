@@ -560,12 +560,13 @@ int scene_alloc(pxh_scene* sc) {
   size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
   size_t oEnt = carve((size_t)sc->entryCap * sizeof(Entry)), oEW = carve((size_t)sc->entryCap);
   size_t oTask = carve((size_t)sc->taskCap * sizeof(RowRuns)), oRun = carve((size_t)sc->runCap * 8);
-  size_t oTaskItem = carve((size_t)sc->taskCap * 4);
+  size_t oTaskItem = carve((size_t)sc->taskCap * 4), oTaskOrd = carve((size_t)sc->taskCap * 4);
+  size_t oTaskCls = carve((size_t)sc->taskCap);
   size_t oRows = carve(nI * sizeof(int4)), oBinC = carve((size_t)sc->binSlots * 4);
   size_t oPaint = carve(nI * sizeof(int4)), oSpan = carve(nI * sizeof(int4));
   size_t oBinO = carve(((size_t)sc->binSlots + 1) * 4), oBinL = carve((size_t)sc->binCap * 4);
   size_t oQ = carve(2 * kQueues * kQueueStride * 4);
-  size_t oCnt = carve(kNumCounters * 4);
+  size_t oCnt = carve(kCounterInts * 4);
   if (off > sc->dMemBytes) {
     PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
     if (sc->dMem) PXH_HIP(hipFree(sc->dMem));
@@ -601,6 +602,8 @@ int scene_alloc(pxh_scene* sc) {
   p.entryCap = sc->entryCap;
   p.tasks = (RowRuns*)(b + oTask);
   p.taskItem = (int32_t*)(b + oTaskItem);
+  p.taskOrder = (int32_t*)(b + oTaskOrd);
+  p.taskClass = (int8_t*)(b + oTaskCls);
   p.taskCap = sc->taskCap;
   p.runs = (int2*)(b + oRun);
   p.runCap = sc->runCap;

@@ -91,6 +91,14 @@ enum SceneCounter {
   kCntError, kCntMaxBand, kCntBinEntries, kNumCounters
 };
 
+// Phase-A tasks run longest first: a task's cost class is the bit length of
+// its band's entry count, and the tasks are ordered by class, descending.
+// The class histogram and the scatter cursors follow the counters.
+constexpr int kCostClasses = 32;
+constexpr int kCntClassHist = kNumCounters;                  // kCostClasses ints
+constexpr int kCntClassCursor = kNumCounters + kCostClasses;  // kCostClasses ints
+constexpr int kCounterInts = kNumCounters + 2 * kCostClasses;
+
 // Phase B looks calls up by row bins of kBinRows canvas rows: each bin lists,
 // in call order, the calls whose rows meet it.
 constexpr int kBinRows = 16;
@@ -140,6 +148,8 @@ struct ScenePools {
   int32_t entryCap;
   RowRuns* tasks;
   int32_t* taskItem;         // call of each task
+  int32_t* taskOrder;        // tasks in the order phase A takes them (longest first)
+  int8_t* taskClass;         // cost class of each task
   int32_t taskCap;
   int2* runs;
   int32_t runCap;

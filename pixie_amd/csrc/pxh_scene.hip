@@ -879,14 +879,104 @@ __device__ __forceinline__ int queue_take(int32_t* counter, int n) {
 }
 
 // Call of every phase-A task (one wave per call writes its task range).
-__global__ void __launch_bounds__(64) scene_task_map_kernel(ScenePools P) {
-  const int lane = threadIdx.x;
+// Cost class of task t (0 .. nt-1) of a call: the bit length of the entry
+// count of its row's band (0 for rows outside the path).
+__device__ __forceinline__ int task_class(const ScenePools& P, const ItemGeom& g, const FillInfo& F,
+                                          int t) {
+  const int y = g.rowStart + t / g.nWin;
+  if (y < F.startY || y >= F.pathHeight || F.nParts <= 0 || F.bandH <= 0) return 0;
+  int pi = (y - F.startY) / F.bandH;
+  if (pi > F.nParts - 1) pi = F.nParts - 1;
+  const int cnt = P.parts[g.bandBase + pi].count;
+  return cnt <= 0 ? 0 : 32 - __clz(cnt);
+}
+
+// Maps tasks to calls, and records and counts (per block in LDS, then one
+// global atomic per class and block) the cost class of every task.  The
+// classes of a call's bands are staged in LDS first.
+constexpr int kLdsBandClasses = 4096;
+constexpr int kTaskMapBlock = 256;
+__global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePools P) {
+  __shared__ int32_t hist[kCostClasses];
+  __shared__ int8_t cls[kLdsBandClasses];
+  const int tid = threadIdx.x, lane = tid & 63;
   if (P.counters[kCntError]) return;
+  if (tid < kCostClasses) hist[tid] = 0;
+  __syncthreads();
   for (int it = blockIdx.x; it < P.nItems; it += gridDim.x) {
     const ItemGeom g = P.geo[it];
+    const FillInfo F = P.fills[it];
     const int nt = g.rowCount * g.nWin;
-    for (int t = lane; t < nt; t += 64) P.taskItem[g.taskBase + t] = it;
+    if (nt <= 0) continue;
+    const bool banded = F.nParts > 0 && F.bandH > 0;
+    const int nb = banded ? F.nParts : 0;
+    for (int b = tid; b < nb && b < kLdsBandClasses; b += kTaskMapBlock) {
+      const int cnt = P.parts[g.bandBase + b].count;
+      cls[b] = (int8_t)(cnt <= 0 ? 0 : 32 - __clz(cnt));
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < nt; t0 += kTaskMapBlock) {
+      const int t = t0 + tid;
+      const bool on = t < nt;
+      int c = 0;
+      if (on) {
+        const int y = g.rowStart + t / g.nWin;
+        if (banded && y >= F.startY && y < F.pathHeight) {
+          int pi = (y - F.startY) / F.bandH;
+          if (pi > nb - 1) pi = nb - 1;
+          c = pi < kLdsBandClasses ? cls[pi] : task_class(P, g, F, t);
+        }
+        P.taskItem[g.taskBase + t] = it;
+        P.taskClass[g.taskBase + t] = (int8_t)c;
+      }
+      // one LDS atomic per class present in the wave
+      uint64_t pending = __ballot(on);
+      while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int cl = __builtin_amdgcn_readlane(c, leader);
+        const uint64_t m = __ballot(on && c == cl);
+        if (lane == leader) atomicAdd(&hist[cl], (int)__popcll(m));
+        pending &= ~m;
+      }
+    }
+    __syncthreads();
   }
+  __syncthreads();
+  if (tid < kCostClasses && hist[tid]) atomicAdd(&P.counters[kCntClassHist + tid], hist[tid]);
+}
+
+// Block b's contiguous share of the tasks.
+__device__ __forceinline__ int2 task_share(int ntasks) {
+  const int per = ((ntasks + (int)gridDim.x - 1) / (int)gridDim.x + 63) & ~63;
+  const int lo = (int)blockIdx.x * per;
+  return make_int2(lo, lo + per < ntasks ? lo + per : ntasks);
+}
+
+// Start of each cost class in the task order, largest class first.
+__global__ void scene_task_scan_kernel(int32_t* counters) {
+  if (threadIdx.x != 0) return;
+  int off = 0;
+  for (int c = kCostClasses - 1; c >= 0; c--) {
+    counters[kCntClassCursor + c] = off;
+    off += counters[kCntClassHist + c];
+  }
+}
+
+// Places every task in the task order by its cost class: each block
+// reserves a range of every class for its share of the tasks, then deals it
+// out with LDS atomics.
+__global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
+  __shared__ int32_t cnt[kCostClasses], cur[kCostClasses];
+  const int lane = threadIdx.x;
+  if (P.counters[kCntError]) return;
+  if (lane < kCostClasses) cnt[lane] = 0;
+  __syncthreads();
+  const int2 r = task_share(P.counters[kCntTasks]);
+  for (int t = r.x + lane; t < r.y; t += 64) atomicAdd(&cnt[P.taskClass[t]], 1);
+  __syncthreads();
+  if (lane < kCostClasses) cur[lane] = cnt[lane] ? atomicAdd(&P.counters[kCntClassCursor + lane], cnt[lane]) : 0;
+  __syncthreads();
+  for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
 
 __global__ void __launch_bounds__(64, 4) scene_rows_kernel(ScenePools P) {
@@ -903,8 +993,9 @@ __global__ void __launch_bounds__(64, 4) scene_rows_kernel(ScenePools P) {
   // time (one atomic per chunk, not per task).
   int cursor = 0, chunkEnd = 0, used = 0;
   for (;;) {
-    const int task = queue_next(P.queues, ntasks);
-    if (task >= ntasks) break;
+    const int q = queue_next(P.queues, ntasks);
+    if (q >= ntasks) break;
+    const int task = P.taskOrder[q];
     const int it = P.taskItem[task];
     const ItemGeom g = P.geo[it];
     const FillInfo& F = P.fills[it];
@@ -1248,7 +1339,7 @@ __global__ void __launch_bounds__(256) scene_clear_kernel(int32_t* counters, int
 }
 
 int scene_launch_clear(const ScenePools& p, hipStream_t s) {
-  hipLaunchKernelGGL(scene_clear_kernel, dim3(1), dim3(256), 0, s, p.counters, (int)kNumCounters,
+  hipLaunchKernelGGL(scene_clear_kernel, dim3(1), dim3(256), 0, s, p.counters, (int)kCounterInts,
                      p.queues, 2 * kQueues * kQueueStride);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
@@ -1268,7 +1359,9 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
 }
 
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid), dim3(64), 0, s, p);
+  hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid), dim3(kTaskMapBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_task_scan_kernel, dim3(1), dim3(64), 0, s, p.counters);
+  hipLaunchKernelGGL(scene_task_order_kernel, dim3(grid >= 8 ? grid / 4 : grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<false>, dim3(grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<true>, dim3(grid), dim3(64), 0, s, p);
