@@ -97,7 +97,11 @@ enum SceneCounter {
 constexpr int kCostClasses = 32;
 constexpr int kCntClassHist = kNumCounters;                  // kCostClasses ints
 constexpr int kCntClassCursor = kNumCounters + kCostClasses;  // kCostClasses ints
-constexpr int kCounterInts = kNumCounters + 2 * kCostClasses;
+// The band-entry scan runs over kEntryTiles contiguous tiles of bands; the
+// tile sums, then the tile offsets, follow the class cursors.
+constexpr int kEntryTiles = 256;
+constexpr int kCntTileSum = kNumCounters + 2 * kCostClasses;  // kEntryTiles ints
+constexpr int kCounterInts = kNumCounters + 2 * kCostClasses + kEntryTiles;
 
 // Phase B looks calls up by row bins of kBinRows canvas rows: each bin lists,
 // in call order, the calls whose rows meet it.

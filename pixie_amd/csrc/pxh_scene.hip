@@ -580,8 +580,6 @@ __global__ void __launch_bounds__(kScanBlock) scene_plan_kernel(ScenePools P) {
       // compact per-call record for phase B
       P.itemPaint[it] = make_int4((int)f.rgbx, f.mode, g.taskBase, g.nWin);
       P.itemSpan[it] = make_int4(g.rowStart, g.rowCount, f.xlo, f.xhi);
-      if (g.bandBase + nb <= P.bandCap)
-        for (int b = 0; b < nb; b++) P.bandItem[g.bandBase + b] = it;
     }
     bandTot += tb;
     taskTot += tt;
@@ -648,29 +646,72 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
 }
 
 // One block: entry offsets of all bands, largest band.
-__global__ void __launch_bounds__(kScanBlock) scene_entry_scan_kernel(ScenePools P) {
-  __shared__ int sh[kScanBlock / 64];
-  __shared__ int sMax;
-  const int nb = P.counters[kCntBands];
-  if (threadIdx.x == 0) sMax = 0;
-  __syncthreads();
-  if (nb > P.bandCap) return;
-  int tot = 0, mx = 0;
-  for (int b0 = 0; b0 < nb; b0 += kScanBlock) {
-    int b = b0 + threadIdx.x;
-    int c = b < nb ? P.parts[b].count : 0;
-    mx = c > mx ? c : mx;
-    int t;
-    int off = block_excl_scan_n<kScanBlock>(c, &t, sh);
-    if (b < nb) P.parts[b].entryBase = tot + off;
-    tot += t;
+// Band -> call map (a workgroup per call, its bands across the threads).
+__global__ void __launch_bounds__(kBlock) scene_band_map_kernel(ScenePools P) {
+  const int tid = threadIdx.x;
+  for (int it = blockIdx.x; it < P.nItems; it += gridDim.x) {
+    const ItemGeom g = P.geo[it];
+    const FillInfo f = P.fills[it];
+    const int nb = (g.status == kOk && f.pathWidth > 0) ? f.nParts : 0;
+    if (g.bandBase + nb > P.bandCap) continue;
+    for (int b = tid; b < nb; b += kBlock) P.bandItem[g.bandBase + b] = it;
   }
-  atomicMax(&sMax, mx);
-  __syncthreads();
-  if (threadIdx.x == 0) {
+}
+
+// Band-entry offsets: a scan of the band entry counts over kEntryTiles
+// contiguous tiles of bands.  1. tile sums (and the largest band).
+__device__ __forceinline__ int2 entry_tile(int nb) {
+  const int per = (nb + kEntryTiles - 1) / kEntryTiles;
+  const int lo = (int)blockIdx.x * per;
+  return make_int2(lo, lo + per < nb ? lo + per : nb);
+}
+
+__global__ void __launch_bounds__(kBlock) scene_entry_tile_kernel(ScenePools P) {
+  __shared__ int sh[kBlock / 64];
+  const int nb = P.counters[kCntBands];
+  if (nb > P.bandCap) return;
+  const int2 r = entry_tile(nb);
+  int sum = 0, mx = 0;
+  for (int b = r.x + (int)threadIdx.x; b < r.y; b += kBlock) {
+    const int c = P.parts[b].count;
+    sum += c;
+    mx = c > mx ? c : mx;
+  }
+  sum = block_sum(sum, sh);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0 && mx > 0) atomicMax(&P.counters[kCntMaxBand], mx);
+  if (threadIdx.x == 0) P.counters[kCntTileSum + blockIdx.x] = sum;
+}
+
+// 2. tile offsets (in place of the tile sums) and the entry total.
+__global__ void __launch_bounds__(kEntryTiles) scene_entry_scan_kernel(ScenePools P) {
+  __shared__ int sh[kEntryTiles / 64];
+  const int nb = P.counters[kCntBands];
+  if (nb > P.bandCap) return;
+  const int t = threadIdx.x;
+  int tot;
+  const int off = block_excl_scan_n<kEntryTiles>(P.counters[kCntTileSum + t], &tot, sh);
+  P.counters[kCntTileSum + t] = off;
+  if (t == 0) {
     P.counters[kCntEntries] = tot;
-    P.counters[kCntMaxBand] = sMax;
     if (tot > P.entryCap) atomicOr(&P.counters[kCntError], 4);
+  }
+}
+
+// 3. entry offset of every band.
+__global__ void __launch_bounds__(kBlock) scene_entry_apply_kernel(ScenePools P) {
+  __shared__ int sh[kBlock / 64];
+  const int nb = P.counters[kCntBands];
+  if (nb > P.bandCap) return;
+  const int2 r = entry_tile(nb);
+  int base = P.counters[kCntTileSum + blockIdx.x];
+  for (int b0 = r.x; b0 < r.y; b0 += kBlock) {
+    const int b = b0 + (int)threadIdx.x;
+    const int c = b < r.y ? P.parts[b].count : 0;
+    int t;
+    const int off = block_excl_scan(c, &t, sh);
+    if (b < r.y) P.parts[b].entryBase = base + off;
+    base += t;
   }
 }
 
@@ -1352,8 +1393,11 @@ int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
 }
 
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(scene_band_map_kernel, dim3(grid / 4 > 0 ? grid / 4 : 1), dim3(kBlock), 0, s, p);
   hipLaunchKernelGGL(scene_band_count_kernel, dim3(grid), dim3(64), 0, s, p);
-  hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_entry_tile_kernel, dim3(kEntryTiles), dim3(kBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_entry_scan_kernel, dim3(1), dim3(kEntryTiles), 0, s, p);
+  hipLaunchKernelGGL(scene_entry_apply_kernel, dim3(kEntryTiles), dim3(kBlock), 0, s, p);
   hipLaunchKernelGGL(scene_band_write_kernel, dim3(grid), dim3(64), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
