@@ -1068,43 +1068,54 @@ __global__ void __launch_bounds__(64, 4) scene_rows_kernel(ScenePools P) {
     if (kind < 0) {
       atomicOr(&P.counters[kCntError], 8);
     } else if (kind != kRowNone) {
-      // compact D[0..wlen] (nonzero deltas) and the area factors
+      // compact D[0..wlen] (nonzero deltas) and, for ops rows, the area
+      // factors of the pixels whose code is an area code.  Coverage rows
+      // need no codes, only the nonzero deltas.  Pass 0 counts (ballots),
+      // pass 1 writes.  D has ww + 4 entries and is zero beyond wlen, so
+      // a lane with i0 <= wlen moves its 4 entries as one 16-byte access.
       const int wlen = C.wx1 - C.wx0;
+      const bool ops = kind == kRowOps;
       int total = 0;
       for (int pass = 0; pass < 2; pass++) {
         int carry = 0, written = 0;
         for (int base = 0; base <= wlen; base += 4 * kWave) {
-          int i0 = base + 4 * lane;
-          int32_t d[4] = {0, 0, 0, 0};
+          const int i0 = base + 4 * lane;
+          int4 dv = make_int4(0, 0, 0, 0);
+          if (i0 <= wlen) dv = *(const int4*)(L.D + i0);
+          const int32_t d[4] = {dv.x, dv.y, dv.z, dv.w};
+          bool area[4] = {false, false, false, false};
+          if (ops) {
+            const int32_t c0 = d[0], c1 = c0 + d[1], c2 = c1 + d[2], c3 = c2 + d[3];
+            int32_t tot;
+            const int32_t off = carry + wave_excl_scan(c3, &tot);
+            carry += tot;
+            const int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
 #pragma unroll
-          for (int q = 0; q < 4; q++)
-            if (i0 + q <= wlen) d[q] = L.D[i0 + q];
-          int32_t c0 = d[0], c1 = c0 + d[1], c2 = c1 + d[2], c3 = c2 + d[3];
-          int32_t tot;
-          int32_t off = carry + wave_excl_scan(c3, &tot);
-          carry += tot;
-          int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+            for (int q = 0; q < 4; q++) area[q] = i0 + q < wlen && code[q] >= kArea && code[q] < kClear;
+          }
+          if (pass == 0) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+              written += (int)__popcll(__ballot(d[q] != 0));
+              if (ops) written += (int)__popcll(__ballot(area[q]));
+            }
+            continue;
+          }
           int n = 0;
 #pragma unroll
-          for (int q = 0; q < 4; q++) {
-            if (i0 + q > wlen) continue;
-            if (d[q] != 0) n++;
-            if (kind == kRowOps && i0 + q < wlen && code[q] >= kArea && code[q] < kClear) n++;
-          }
+          for (int q = 0; q < 4; q++) n += (d[q] != 0 ? 1 : 0) + (area[q] ? 1 : 0);
           int ntot;
-          int p = wave_excl_scan(n, &ntot);
-          if (pass == 1) {
+          const int p = wave_excl_scan(n, &ntot);
+          if (i0 <= wlen) {
             int2* out = P.runs + rr.off + written + p;
             int w = 0;
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-              if (i0 + q > wlen) continue;
-              int x = C.wx0 + i0 + q;
+              const int x = C.wx0 + i0 + q;
               if (d[q] != 0) out[w++] = make_int2(x, d[q]);
-              if (kind == kRowOps && i0 + q < wlen && code[q] >= kArea && code[q] < kClear)
-                out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
-              L.D[i0 + q] = 0;
+              if (area[q]) out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
             }
+            *(int4*)(L.D + i0) = make_int4(0, 0, 0, 0);
           }
           written += ntot;
         }
