@@ -12,6 +12,7 @@
 namespace pxh {
 
 constexpr int kBlock = 256;
+constexpr int kStagePts = 4096;  // LDS points of the geometry stage (32 KB)
 constexpr int kWinA = 512;    // phase-A column window
 constexpr int kWinB = 1024;   // phase-B column window (4 chunks of 256 px)
 constexpr int kChunksB = kWinB / 256;
@@ -69,6 +70,26 @@ struct CountEmit {
   __device__ void operator()(V2 a, V2 b) {
     V2 d = a - b;
     if (d.x != 0.0f || d.y != 0.0f) n++;  // addSegment skips zero-length pieces
+  }
+};
+
+// Counts like CountEmit and keeps the first `cap` points in an LDS slice,
+// transformed like StoreEmit, so that the emit stage can copy them instead of
+// flattening the command again.
+struct StageEmit {
+  float2* buf;
+  int cap;
+  const float* mat;
+  bool xf;
+  int n = 0;
+  __device__ void operator()(V2 a, V2 b) {
+    V2 d = a - b;
+    if (d.x == 0.0f && d.y == 0.0f) return;  // addSegment skips zero-length pieces
+    if (n < cap) {
+      V2 p = xf ? mat_apply(mat, b) : b;
+      buf[n] = make_float2(p.x, p.y);
+    }
+    n++;
   }
 };
 
@@ -290,20 +311,28 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   __shared__ Bounds sBd[kBlock / 64];
   __shared__ FillInfo sF;
   __shared__ int sErr;
+  __shared__ float2 sPts[kStagePts];
   const int it = blockIdx.x;
   const int tid = threadIdx.x;
   const SceneItem I = P.items[it];
+  const bool xf = !I.stroke && !I.identity;  // fills transform points here
   if (tid == 0) sErr = kOk;
   __syncthreads();
 
-  // 1. points per command
+  // 1. points per command.  A call of at most kBlock commands keeps each
+  // command's points in an LDS slice of its own (thread tid <-> command tid
+  // in stages 1 and 4), so stage 4 copies them unless a slice overflowed.
+  const bool stage = I.cmdCount <= kBlock;
+  const int slice = stage && I.cmdCount > 0 ? kStagePts / I.cmdCount : 0;
+  bool staged = false;
   for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
     int i = I.cmdBase + c0 + tid;
     if (c0 + tid < I.cmdCount) {
-      CountEmit ce;
-      int32_t r = cmd_emit(P.cmds, i, I.errSq, ce);
+      StageEmit se{sPts + tid * slice, slice, I.mat, xf};
+      int32_t r = cmd_emit(P.cmds, i, I.errSq, se);
       if (r != kOk) atomicMax(&sErr, r);
-      P.cmdCount[i] = ce.n;
+      P.cmdCount[i] = se.n;
+      staged = stage && se.n <= slice;
     }
   }
   __syncthreads();
@@ -354,7 +383,6 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   }
   float2* pts = P.pts + ptBase;
   int32_t* ptPoly = P.ptPoly + ptBase;
-  const bool xf = !I.stroke && !I.identity;  // fills transform points here
 
   // 3. per-command point offsets, start and closing points (thread per subpath)
   for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
@@ -388,8 +416,17 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
     if (c0 + tid < I.cmdCount && P.cmdCount[i] > 0) {
       // subpath of command i: the host stored it in the Move's pad slot
       int ql = P.cmds[i].pad[0];
-      StoreEmit se{pts + P.cmdOff[i], ptPoly + P.cmdOff[i], ql, I.mat, xf};
-      cmd_emit(P.cmds, i, I.errSq, se);
+      if (staged) {
+        const int n = P.cmdCount[i], o = P.cmdOff[i];
+        const float2* src = sPts + tid * slice;
+        for (int k = 0; k < n; k++) {
+          pts[o + k] = src[k];
+          ptPoly[o + k] = ql;
+        }
+      } else {
+        StoreEmit se{pts + P.cmdOff[i], ptPoly + P.cmdOff[i], ql, I.mat, xf};
+        cmd_emit(P.cmds, i, I.errSq, se);
+      }
     }
   }
   __syncthreads();
