@@ -686,7 +686,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   }
   if (!r) {
     LaunchTimer t("scene_bands", st);
-    r = scene_launch_bands(p, g.numCUs * 16, st);
+    r = scene_launch_bands(p, g.numCUs * 64, st);
   }
   if (!r) {
     LaunchTimer t("scene_bins", st);
