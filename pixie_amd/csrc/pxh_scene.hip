@@ -1057,7 +1057,7 @@ __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
 
-__global__ void __launch_bounds__(64, 4) scene_rows_kernel(ScenePools P) {
+__global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x;
   const size_t fixedBytes = row_lds_fixed_bytes(kWinA);
