@@ -699,7 +699,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   if (!r) {
     LaunchTimer t("scene_compose_kernel", st);
     r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched,
-                             scene_queue_grid(g.numCUs * 16),
+                             scene_queue_grid(g.numCUs * 24),
                              st);
   }
   if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));

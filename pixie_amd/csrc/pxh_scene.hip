@@ -14,7 +14,7 @@ namespace pxh {
 constexpr int kBlock = 256;
 constexpr int kStagePts = 4096;  // LDS points of the geometry stage (32 KB)
 constexpr int kWinA = 512;    // phase-A column window
-constexpr int kWinB = 1024;   // phase-B column window (4 chunks of 256 px)
+constexpr int kWinB = 512;    // phase-B column window (2 chunks of 256 px)
 constexpr int kChunksB = kWinB / 256;
 
 // ---------------------------------------------------------------------------
