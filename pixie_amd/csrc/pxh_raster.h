@@ -612,7 +612,10 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   }
   int32_t pi = (y - F.startY) / F.bandH;
   if (pi > F.nParts - 1) pi = F.nParts - 1;
-  const PartInfo P = C.parts[pi];
+  PartInfo P = C.parts[pi];  // the same in every lane: scalar registers
+  P.entryBase = __builtin_amdgcn_readfirstlane(P.entryBase);
+  P.count = __builtin_amdgcn_readfirstlane(P.count);
+  P.flags = __builtin_amdgcn_readfirstlane(P.flags);
   if (P.count > L.cap) return -1;
   const bool aa = P.flags & 1, two = P.flags & 2;
   RowCtx Cp = C;

@@ -1057,6 +1057,8 @@ __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
 
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x;
@@ -1073,16 +1075,35 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   for (;;) {
     const int q = queue_next(P.queues, ntasks);
     if (q >= ntasks) break;
-    const int task = P.taskOrder[q];
-    const int it = P.taskItem[task];
-    const ItemGeom g = P.geo[it];
-    const FillInfo& F = P.fills[it];
+    // Task, call and the call's records are the same in every lane: read
+    // them into scalar registers.
+    const int task = uni(P.taskOrder[q]);
+    const int it = uni(P.taskItem[task]);
+    ItemGeom g = P.geo[it];
+    g.rowStart = uni(g.rowStart);
+    g.nWin = uni(g.nWin);
+    g.winBase = uni(g.winBase);
+    g.taskBase = uni(g.taskBase);
+    g.bandBase = uni(g.bandBase);
+    FillInfo Fu = P.fills[it];
+    Fu.startX = uni(Fu.startX);
+    Fu.startY = uni(Fu.startY);
+    Fu.pathWidth = uni(Fu.pathWidth);
+    Fu.pathHeight = uni(Fu.pathHeight);
+    Fu.nParts = uni(Fu.nParts);
+    Fu.bandH = uni(Fu.bandH);
+    Fu.mode = uni(Fu.mode);
+    Fu.rule = uni(Fu.rule);
+    Fu.xlo = uni(Fu.xlo);
+    Fu.xhi = uni(Fu.xhi);
+    Fu.canvasW = uni(Fu.canvasW);
+    const FillInfo& F = Fu;
     const int local = task - g.taskBase;
     const int y = g.rowStart + local / g.nWin;
     const int win = local % g.nWin;
     const bool mask = F.mode == kMask;
     RowCtx C;
-    C.F = &P.fills[it];
+    C.F = &Fu;
     C.parts = P.parts + g.bandBase;
     C.E = P.entries;
     C.EW = P.entW;
@@ -1094,7 +1115,7 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
     // band of this row decides where the row scratch lives
     int pi = (y - F.startY) / F.bandH;
     if (pi > F.nParts - 1) pi = F.nParts - 1;
-    const int cnt = C.parts[pi].count;
+    const int cnt = uni(C.parts[pi].count);
     RowLds& L = cnt <= P.ldsCap ? Ll : Lg;
     int32_t kind = cnt <= P.gcap ? row_program(C, L) : -1;
     RowRuns rr;
@@ -1297,6 +1318,9 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
           if (wx0 >= b1) break;
           if (wx0 + kWinA < b0) continue;
           RowRuns rr = P.tasks[tb + w];
+          rr.off = uni(rr.off);
+          rr.n = uni(rr.n);
+          rr.kind = uni(rr.kind);
           if (rr.kind > kind) kind = rr.kind;
           for (int e0 = 0; e0 < rr.n; e0 += 64) {
             int e = e0 + lane;
