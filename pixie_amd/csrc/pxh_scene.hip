@@ -185,7 +185,7 @@ struct CircleEmit {
   }
 };
 
-__device__ void circle_segments(V2 at, const SceneItem& I, SegSink& sink) {
+__device__ __forceinline__ void circle_segments(V2 at, const SceneItem& I, SegSink& sink) {
   const double K = 4.0 * (-1.0 + 1.4142135623730951) / 3.0;  // splineCircleK (float64)
   float cx = at.x, cy = at.y, rx = I.halfStroke, ry = I.halfStroke;
   double mx = K * (double)rx, my = K * (double)ry;
@@ -206,7 +206,7 @@ __device__ void circle_segments(V2 at, const SceneItem& I, SegSink& sink) {
   flatten_cubic(a, c1, c2, to, I.errSq, ce);
 }
 
-__device__ void rect_segments(V2 at, V2 to, const SceneItem& I, SegSink& sink) {  // makeRect
+__device__ __forceinline__ void rect_segments(V2 at, V2 to, const SceneItem& I, SegSink& sink) {  // makeRect
   float hs = I.halfStroke;
   V2 tangent = normalize(to - at);
   V2 normal = v2(tangent.y, tangent.x);
@@ -226,12 +226,16 @@ __device__ __forceinline__ float fix_angle_d(float a) {  // vmath fixAngle
   return a;
 }
 
-__device__ void join_segments(V2 prevPos, V2 pos, V2 nextPos, const SceneItem& I, SegSink& sink) {
+// A round join's circle is left to the caller (*circle = true): it is always
+// the last polygon a stroke point emits, so stroke_point draws every circle
+// from one place.
+__device__ __forceinline__ void join_segments(V2 prevPos, V2 pos, V2 nextPos, const SceneItem& I,
+                                              SegSink& sink, bool* circle) {
   const float PI = 3.14159265358979323846f;
   float minArea = I.minArea, hs = I.halfStroke;
   if (I.join == 1) {  // RoundJoin
     float area = PI * hs * hs;
-    if (area > minArea) circle_segments(pos, I, sink);
+    if (area > minArea) *circle = true;
     return;
   }
   V2 dn = nextPos - pos, dp = prevPos - pos;
@@ -274,32 +278,50 @@ __device__ void join_segments(V2 prevPos, V2 pos, V2 nextPos, const SceneItem& I
 
 // Stroke polygons attributed to point k of a shape with n points, in
 // strokeShapes order.
-__device__ void stroke_point(const float2* P, int k, int n, const SceneItem& I, SegSink& sink) {
+// Segments of point k of a stroked polyline P[0..n): its cap or its
+// segment rectangle and join.  A round cap or round join comes last, so the
+// one circle a point may need is drawn at the end, from a single call site
+// (which keeps I and sink in registers once inlined).
+__device__ __forceinline__ void stroke_point(const float2* P, int k, int n, const SceneItem& I,
+                                             SegSink& sink) {
   auto pt = [&](int i) { return v2(P[i].x, P[i].y); };
   bool closed = veq(pt(0), pt(n - 1));
+  bool circle = false;
+  V2 center = v2(0.0f, 0.0f);
   if (k == 0) {
     if (!closed) {
-      if (I.cap == 1) circle_segments(pt(0), I, sink);
-      else if (I.cap == 2) {
+      if (I.cap == 1) {
+        circle = true;
+        center = pt(0);
+      } else if (I.cap == 2) {
         V2 tangent = normalize(pt(1) - pt(0));
         rect_segments(pt(0) - tangent * I.halfStroke, pt(0), I, sink);
       }
     }
-    return;
-  }
-  rect_segments(pt(k - 1), pt(k), I, sink);
-  if (k < n - 1) join_segments(pt(k - 1), pt(k), pt(k + 1), I, sink);
-  if (k == n - 1) {
-    if (closed) {
-      if (n >= 2) join_segments(pt(n - 2), pt(n - 1), pt(1), I, sink);
-    } else {
-      if (I.cap == 1) circle_segments(pt(n - 1), I, sink);
-      else if (I.cap == 2) {
-        V2 tangent = normalize(pt(n - 1) - pt(n - 2));
-        rect_segments(pt(n - 1) + tangent * I.halfStroke, pt(n - 1), I, sink);
+  } else {
+    rect_segments(pt(k - 1), pt(k), I, sink);
+    if (k < n - 1) {
+      join_segments(pt(k - 1), pt(k), pt(k + 1), I, sink, &circle);
+      center = pt(k);
+    }
+    if (k == n - 1) {
+      if (closed) {
+        if (n >= 2) {
+          join_segments(pt(n - 2), pt(n - 1), pt(1), I, sink, &circle);
+          center = pt(n - 1);
+        }
+      } else {
+        if (I.cap == 1) {
+          circle = true;
+          center = pt(n - 1);
+        } else if (I.cap == 2) {
+          V2 tangent = normalize(pt(n - 1) - pt(n - 2));
+          rect_segments(pt(n - 1) + tangent * I.halfStroke, pt(n - 1), I, sink);
+        }
       }
     }
   }
+  if (circle) circle_segments(center, I, sink);
 }
 
 // ---------------------------------------------------------------------------
