@@ -13,6 +13,9 @@ namespace pxh {
 
 constexpr int kBlock = 256;
 constexpr int kStagePts = 4096;  // LDS points of the geometry stage (32 KB)
+
+// A value that is the same in every lane, moved to a scalar register.
+__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 constexpr int kWinA = 512;    // phase-A column window
 constexpr int kWinB = 512;    // phase-B column window (2 chunks of 256 px)
 constexpr int kChunksB = kWinB / 256;
@@ -663,9 +666,11 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
   const int nb = P.counters[kCntBands];
   if (nb > P.bandCap) return;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    const int it = P.bandItem[b];
-    const ItemGeom g = P.geo[it];
-    const FillInfo f = P.fills[it];
+    const int it = uni(P.bandItem[b]);
+    ItemGeom g = P.geo[it];
+    g.bandBase = uni(g.bandBase);
+    g.segBase = uni(g.segBase);
+    g.segCount = uni(g.segCount);
     const int k = b - g.bandBase;
     int cnt = 0;
     bool aa = false;
@@ -801,11 +806,21 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
   const int nb = P.counters[kCntBands];
   if (nb > P.bandCap || P.counters[kCntEntries] > P.entryCap) return;
   for (int b = blockIdx.x; b < nb; b += gridDim.x) {
-    const int it = P.bandItem[b];
-    const ItemGeom g = P.geo[it];
-    const FillInfo f = P.fills[it];
+    const int it = uni(P.bandItem[b]);
+    ItemGeom g = P.geo[it];
+    g.bandBase = uni(g.bandBase);
+    g.segBase = uni(g.segBase);
+    g.segCount = uni(g.segCount);
+    FillInfo f = P.fills[it];
+    f.startY = uni(f.startY);
+    f.bandH = uni(f.bandH);
+    f.nParts = uni(f.nParts);
+    f.pathHeight = uni(f.pathHeight);
     const int k = b - g.bandBase;
-    const PartInfo pi = P.parts[b];
+    PartInfo pi = P.parts[b];
+    pi.entryBase = uni(pi.entryBase);
+    pi.count = uni(pi.count);
+    pi.flags = uni(pi.flags);
     float top, bot;
     band_rows(f, k, &top, &bot);
     const int s0 = g.segBase, s1 = g.segBase + g.segCount;
@@ -898,9 +913,15 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
         swap = midpoint_x(a0, a1) > midpoint_x(c0, c1);
       }
     }
-    // every lane stores the same values
-    P.entries[pi.entryBase] = swap ? e1 : e0;
-    P.entries[pi.entryBase + 1] = swap ? e0 : e1;
+    // every lane stores the same values (conditional copies, not ?: on the
+    // two lvalues, which would keep both entries in memory)
+    Entry first = e0, second = e1;
+    if (swap) {
+      first = e1;
+      second = e0;
+    }
+    P.entries[pi.entryBase] = first;
+    P.entries[pi.entryBase + 1] = second;
     P.entW[pi.entryBase] = (int8_t)(swap ? w1 : w0);
     P.entW[pi.entryBase + 1] = (int8_t)(swap ? w0 : w1);
     P.parts[b].flags = flags;
@@ -1078,8 +1099,6 @@ __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   __syncthreads();
   for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
-
-__device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
 __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
