@@ -233,7 +233,11 @@ __device__ __forceinline__ int32_t key_at(uint64_t k) {
   return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
 }
 
-template <class SpanFn>
+// kPacked: the winding is the key's low byte (keys (x, index << 8 | winding))
+// instead of L.B[low word].
+__device__ __forceinline__ int32_t key_winding(uint64_t k) { return (int32_t)(int8_t)(uint8_t)k; }
+
+template <bool kPacked = false, class SpanFn>
 __device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
   int lane = lane_id();
   int32_t carryCount = 0;
@@ -244,14 +248,14 @@ __device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
     bool valid = i < n;
     uint64_t k = valid ? L.keys[i] : 0ull;
     int32_t at = valid ? key_at(k) : 0;
-    int32_t w = valid ? L.B[(uint32_t)k] : 0;
+    int32_t w = valid ? (kPacked ? key_winding(k) : L.B[(uint32_t)k]) : 0;
     int32_t tot;
     int32_t countBefore = carryCount + wave_excl_scan(w, &tot);
     bool filling = should_fill(rule, countBefore);
     bool hasNext = valid && i < n - 1;
     uint64_t kn = hasNext ? L.keys[i + 1] : 0ull;
     int32_t nextAt = hasNext ? key_at(kn) : 0;
-    int32_t nextW = hasNext ? L.B[(uint32_t)kn] : 0;
+    int32_t nextW = hasNext ? (kPacked ? key_winding(kn) : L.B[(uint32_t)kn]) : 0;
     bool canSkip = valid && at > 0 && filling && hasNext && nextAt == at && w + nextW == 0;
     uint64_t cm = ballot(canSkip);
     uint64_t skipMask = 0, consMask = carryConsumed ? 1ull : 0ull;
@@ -517,14 +521,17 @@ __device__ void wave_fix_keys(uint64_t* keys, int n, int cap) {
 // its hits in that order, which leaves them sorted except where entries cross
 // inside the row, and wave_fix_keys finishes the sort.  The hit keys carry
 // the selection index as tie-break, which orders equal x like pixie's
-// insertion sort over hits collected in selection order.
+// insertion sort over hits collected in selection order, and the winding in
+// their low byte, so the walk reads it from the key.
 __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& L, int ne, SelCache S,
                                        float yLine, float offset, int32_t rule, CovSpan& span) {
   const int lane = lane_id();
   const float Wf = (float)C.F->canvasW;
   const float yMid = (float)C.y + 0.5f;
-  int32_t* order = (int32_t*)L.tx0;  // selection index by presorted position
-  int32_t* entry = (int32_t*)L.tx1;  // entry index by presorted position
+  // By presorted position: the selection index with the winding in the low
+  // byte (the hit key's low word), and the entry index.
+  int32_t* order = (int32_t*)L.tx0;
+  int32_t* entry = (int32_t*)L.tx1;
   for (int i = lane; i < ne; i += kWave) {
     Entry e;
     int32_t w;
@@ -538,25 +545,26 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
     }
     const float yc = yMid < e.aty ? e.aty : (yMid > e.toy ? e.toy : yMid);
     L.keys[i] = ((uint64_t)float_key(solve_x(e.m, e.b, yc)) << 32) | (uint32_t)i;
-    L.B[i] = w;  // windings by selection index, read by the walk
+    L.B[i] = w;
   }
   wave_sync();
   if (ne > 1) wave_sort_keys(L.keys, ne, L.cap);
   for (int j = lane; j < ne; j += kWave) {
     const int s = (int)(uint32_t)L.keys[j];
-    order[j] = s;
+    order[j] = (s << 8) | (int32_t)(uint8_t)L.B[s];
     if (!S.inReg) entry[j] = L.A[s];
   }
   wave_sync();
   // a selection held in registers moves to its presorted lanes
   Entry er = S.e;
-  int32_t sr = lane;
+  int32_t sr = 0;
   if (S.inReg) {
     sr = lane < ne ? order[lane] : 0;
-    er.aty = __shfl(S.e.aty, sr, kWave);
-    er.toy = __shfl(S.e.toy, sr, kWave);
-    er.m = __shfl(S.e.m, sr, kWave);
-    er.b = __shfl(S.e.b, sr, kWave);
+    const int src = sr >> 8;
+    er.aty = __shfl(S.e.aty, src, kWave);
+    er.toy = __shfl(S.e.toy, src, kWave);
+    er.m = __shfl(S.e.m, src, kWave);
+    er.b = __shfl(S.e.b, src, kWave);
   }
   for (int m = 0; m < 5; m++) {
     yLine += offset;
@@ -587,7 +595,7 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
     }
     wave_sync();
     if (nh > 1) wave_fix_keys(L.keys, nh, L.cap);
-    wave_walk(L, nh, rule, span);
+    wave_walk<true>(L, nh, rule, span);
     wave_sync();
   }
 }
