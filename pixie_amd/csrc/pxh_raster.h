@@ -434,11 +434,10 @@ __device__ __forceinline__ void row_coverage_small(const RowCtx& C, RowLds& L, i
     const uint64_t segMask = (m >> base) & (SEG == 32 ? 0xffffffffull : 0xffffull);
     const int nh = __popcll(segMask);
     const int pos = __popcll(segMask & ((1ull << i) - 1ull));
-    const uint64_t key = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | (uint32_t)pos;
-    if (hit) {
-      L.keys[base + pos] = key;
-      L.B[base + pos] = ew;
-    }
+    // (x, position << 8 | winding): the winding rides in the low byte
+    const uint64_t key = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) |
+                         ((uint32_t)pos << 8) | (uint32_t)(uint8_t)ew;
+    if (hit) L.keys[base + pos] = key;
     wave_sync();
     // stable sort by (x, position): rank among the segment's hits
     if (hit) {
@@ -451,13 +450,13 @@ __device__ __forceinline__ void row_coverage_small(const RowCtx& C, RowLds& L, i
     const bool valid = i < nh;
     const uint64_t k = valid ? sorted[base + i] : 0ull;
     const int32_t hat = valid ? key_at(k) : 0;
-    const int32_t w = valid ? L.B[base + (uint32_t)k] : 0;
+    const int32_t w = valid ? key_winding(k) : 0;
     const int32_t countBefore = seg_incl_sum<SEG>(w) - w;
     const bool filling = should_fill(rule, countBefore);
     const bool hasNext = valid && i < nh - 1;
     const uint64_t kn = hasNext ? sorted[base + i + 1] : 0ull;
     const int32_t nextAt = hasNext ? key_at(kn) : 0;
-    const int32_t nextW = hasNext ? L.B[base + (uint32_t)kn] : 0;
+    const int32_t nextW = hasNext ? key_winding(kn) : 0;
     const bool canSkip = valid && hat > 0 && filling && hasNext && nextAt == hat && w + nextW == 0;
     uint64_t cm = ballot(canSkip);
     uint64_t skipMask = 0, consMask = 0;
