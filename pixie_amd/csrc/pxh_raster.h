@@ -291,6 +291,71 @@ __device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
   }
 }
 
+// wave_walk<true> for at most 64 sorted hits held in registers (hit i in
+// lane i, padding keys ~0 beyond n): the same events, with the next hit
+// taken from the next lane.
+template <class SpanFn>
+__device__ __forceinline__ void wave_walk_reg(uint64_t k, int n, int32_t rule, SpanFn& span) {
+  const int lane = lane_id();
+  const bool valid = lane < n;
+  const int32_t at = valid ? key_at(k) : 0;
+  const int32_t w = valid ? key_winding(k) : 0;
+  int32_t tot;
+  const int32_t countBefore = wave_excl_scan(w, &tot);
+  const bool filling = should_fill(rule, countBefore);
+  const bool hasNext = valid && lane < n - 1;
+  const uint64_t kn = __shfl(k, lane + 1 < kWave ? lane + 1 : lane, kWave);
+  const int32_t nextAt = hasNext ? key_at(kn) : 0;
+  const int32_t nextW = hasNext ? key_winding(kn) : 0;
+  const bool canSkip = valid && at > 0 && filling && hasNext && nextAt == at && w + nextW == 0;
+  uint64_t cm = ballot(canSkip);
+  uint64_t skipMask = 0, consMask = 0;
+  while (cm) {
+    const int b = __builtin_ctzll(cm);
+    cm &= cm - 1;
+    if (!((consMask >> b) & 1ull)) {
+      skipMask |= 1ull << b;
+      if (b + 1 < kWave) consMask |= 1ull << (b + 1);
+    }
+  }
+  const bool consumed = (consMask >> lane) & 1ull;
+  const bool skipStart = (skipMask >> lane) & 1ull;
+  int ev = 0;  // 0 none, 1 start, 2 end (yield)
+  if (valid && !consumed && !skipStart && at > 0) {
+    if (filling) {
+      const bool cont = (rule == kNonZero) && hasNext && (countBefore + w != 0);
+      ev = cont ? 0 : 2;
+    } else {
+      ev = 1;
+    }
+  }
+  const int32_t prevHere = wave_excl_max(ev ? at : INT32_MIN);
+  const int32_t prevAt = prevHere > 0 ? prevHere : 0;  // prevAt starts at 0 (walk 1300)
+  if (ev == 2) span(prevAt, at);
+}
+
+// One odd-even transposition step over the wave's keys (pairs (2m, 2m+1), or
+// with odd (2m+1, 2m+2)); the lower lane of a pair keeps the smaller key.
+// Returns whether this lane took its partner's key.
+__device__ __forceinline__ bool wave_transpose(uint64_t& k, bool odd) {
+  const int lane = lane_id();
+  bool lower, has;
+  int src;
+  if (!odd) {
+    lower = (lane & 1) == 0;
+    has = true;
+    src = lane ^ 1;
+  } else {
+    lower = (lane & 1) == 1;
+    has = lower ? lane + 1 < kWave : lane > 0;
+    src = lower ? (lane + 1 < kWave ? lane + 1 : lane) : (lane > 0 ? lane - 1 : lane);
+  }
+  const uint64_t o = __shfl(k, src, kWave);
+  const bool take = has && (lower ? o < k : o > k);
+  if (take) k = o;
+  return take;
+}
+
 // Coverage contributions of one span (computeCoverage 1390-1431).
 struct CovSpan {
   const RowCtx* C;
@@ -593,8 +658,30 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
       nh += __popcll(mk);
     }
     wave_sync();
-    if (nh > 1) wave_fix_keys(L.keys, nh, L.cap);
-    wave_walk<true>(L, nh, rule, span);
+    if (nh <= kWave) {
+      // the hits fit the wave: finish the sort and walk in registers
+      uint64_t k = lane < nh ? L.keys[lane] : ~0ull;  // padding sorts last
+      bool done = false;
+      for (int t = 0; t < kFixPasses; t++) {
+        bool sw = wave_transpose(k, false);
+        sw = wave_transpose(k, true) || sw;
+        if (ballot(sw) == 0) {
+          done = true;
+          break;
+        }
+      }
+      if (!done) {
+        wave_sync();
+        if (lane < nh) L.keys[lane] = k;
+        wave_sync();
+        wave_sort_keys(L.keys, nh, L.cap);
+        k = lane < nh ? L.keys[lane] : ~0ull;
+      }
+      wave_walk_reg(k, nh, rule, span);
+    } else {
+      wave_fix_keys(L.keys, nh, L.cap);
+      wave_walk<true>(L, nh, rule, span);
+    }
     wave_sync();
   }
 }
