@@ -988,6 +988,8 @@ __device__ __forceinline__ int queue_next(int32_t* heads, int n) {
   const int q = (int)(blockIdx.x % kQueues);
   int k = atomicAdd(heads + q * kQueueStride, lane == 0 ? 1 : 0);
   k = __shfl(k, 0, 64);
+  // the same in every lane: keep it (and what derives from it) scalar
+  k = __builtin_amdgcn_readfirstlane(k);
   return k < (n + kQueues - 1) / kQueues ? q + kQueues * k : n;
 }
 
@@ -996,7 +998,7 @@ __device__ __forceinline__ int queue_next(int32_t* heads, int n) {
 __device__ __forceinline__ int queue_take(int32_t* counter, int n) {
   const int lane = threadIdx.x & 63;
   int base = atomicAdd(counter, lane == 0 ? n : 0);
-  return __shfl(base, 0, 64);
+  return __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
 }
 
 // Call of every phase-A task (one wave per call writes its task range).
