@@ -337,6 +337,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   __shared__ FillInfo sF;
   __shared__ int sErr;
   __shared__ float2 sPts[kStagePts];
+  __shared__ int sPre[kBlock + 1];
   const int it = blockIdx.x;
   const int tid = threadIdx.x;
   const SceneItem I = P.items[it];
@@ -350,6 +351,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   const bool stage = I.cmdCount <= kBlock;
   const int slice = stage && I.cmdCount > 0 ? kStagePts / I.cmdCount : 0;
   bool staged = false;
+  int myN = 0;
   for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
     int i = I.cmdBase + c0 + tid;
     if (c0 + tid < I.cmdCount) {
@@ -357,10 +359,21 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       int32_t r = cmd_emit(P.cmds, i, I.errSq, se);
       if (r != kOk) atomicMax(&sErr, r);
       P.cmdCount[i] = se.n;
+      myN = se.n;
       staged = stage && se.n <= slice;
     }
   }
   __syncthreads();
+  // With one command per thread: points before each command of the call
+  // (sPre[c], and the total in sPre[cmdCount]), so that subpath sums and
+  // command offsets take two LDS reads instead of a loop over commands.
+  if (stage) {
+    int tot;
+    const int pre = block_excl_scan(tid < I.cmdCount ? myN : 0, &tot, sh);
+    if (tid < I.cmdCount) sPre[tid] = pre;
+    if (tid == 0) sPre[I.cmdCount] = tot;
+    __syncthreads();
+  }
   // 2. points per subpath: start + pieces (+ the implicit closing point of a fill)
   int total = 0;
   for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
@@ -369,7 +382,12 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
     if (q0 + tid < I.polyCount) {
       ScenePoly sp = P.polys[q];
       int n = 0;
-      for (int c = 0; c < sp.numCmds; c++) n += P.cmdCount[sp.firstCmd + c];
+      if (stage) {
+        const int cl = sp.firstCmd - I.cmdBase;
+        n = sPre[cl + sp.numCmds] - sPre[cl];
+      } else {
+        for (int c = 0; c < sp.numCmds; c++) n += P.cmdCount[sp.firstCmd + c];
+      }
       if (n > 0) {
         np = n + 1;
         const SceneCmd& last = P.cmds[sp.firstCmd + sp.numCmds - 1];
@@ -417,9 +435,14 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       ScenePoly sp = P.polys[q];
       int np = P.polyN[q], pb = P.polyPt[q];
       int off = pb + 1;
-      for (int c = 0; c < sp.numCmds; c++) {
-        P.cmdOff[sp.firstCmd + c] = off;
-        off += P.cmdCount[sp.firstCmd + c];
+      if (stage) {  // stage 4 derives each command's offset from sPre
+        const int cl = sp.firstCmd - I.cmdBase;
+        off += sPre[cl + sp.numCmds] - sPre[cl];
+      } else {
+        for (int c = 0; c < sp.numCmds; c++) {
+          P.cmdOff[sp.firstCmd + c] = off;
+          off += P.cmdCount[sp.firstCmd + c];
+        }
       }
       if (np > 0) {
         const SceneCmd& mv = P.cmds[sp.firstCmd];
@@ -439,17 +462,24 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
     int i = I.cmdBase + c0 + tid;
     if (c0 + tid < I.cmdCount && P.cmdCount[i] > 0) {
-      // subpath of command i: the host stored it in the Move's pad slot
+      // subpath of command i (the host stores it in pad[0] of every command)
       int ql = P.cmds[i].pad[0];
+      int o;
+      if (stage) {
+        const int q = I.polyBase + ql;
+        o = P.polyPt[q] + 1 + sPre[c0 + tid] - sPre[P.polys[q].firstCmd - I.cmdBase];
+      } else {
+        o = P.cmdOff[i];
+      }
       if (staged) {
-        const int n = P.cmdCount[i], o = P.cmdOff[i];
+        const int n = P.cmdCount[i];
         const float2* src = sPts + tid * slice;
         for (int k = 0; k < n; k++) {
           pts[o + k] = src[k];
           ptPoly[o + k] = ql;
         }
       } else {
-        StoreEmit se{pts + P.cmdOff[i], ptPoly + P.cmdOff[i], ql, I.mat, xf};
+        StoreEmit se{pts + o, ptPoly + o, ql, I.mat, xf};
         cmd_emit(P.cmds, i, I.errSq, se);
       }
     }
