@@ -686,9 +686,35 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
   }
 }
 
+// Entries row C.y selects (row_program's selection), so that a caller can
+// place the row's arrays before running it: the arrays hold the selection,
+// which is often much smaller than the band.
+__device__ __forceinline__ int row_selection_count(const RowCtx& C) {
+  const FillInfo& F = *C.F;
+  const int32_t y = C.y;
+  if (y < F.startY || y >= F.pathHeight) return 0;
+  int32_t pi = (y - F.startY) / F.bandH;
+  if (pi > F.nParts - 1) pi = F.nParts - 1;
+  const int count = __builtin_amdgcn_readfirstlane(C.parts[pi].count);
+  if (__builtin_amdgcn_readfirstlane(C.parts[pi].flags) & 2) return 2;
+  const Entry* E = C.E + __builtin_amdgcn_readfirstlane(C.parts[pi].entryBase);
+  const float scanTop = (float)y, scanBottom = (float)(y + 1);
+  int ne = 0;
+  for (int base = 0; base < count; base += kWave) {
+    const int i = base + lane_id();
+    bool s = false;
+    if (i < count) {
+      const Entry e = E[i];
+      s = !(e.toy <= scanTop || e.aty >= scanBottom);
+    }
+    ne += __popcll(ballot(s));
+  }
+  return ne;
+}
+
 // Runs the row program for (fill, row y, window) and leaves the row's codes
 // in L.D (and L.AX).  Returns the row kind; kRowNone means nothing to do and
-// -1 means the band holds more entries than L.cap (nothing was written).
+// -1 means the selection holds more entries than L.cap (nothing was written).
 __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   const FillInfo& F = *C.F;
   const int lane = lane_id();
@@ -710,7 +736,6 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   P.entryBase = __builtin_amdgcn_readfirstlane(P.entryBase);
   P.count = __builtin_amdgcn_readfirstlane(P.count);
   P.flags = __builtin_amdgcn_readfirstlane(P.flags);
-  if (P.count > L.cap) return -1;
   const bool aa = P.flags & 1, two = P.flags & 2;
   RowCtx Cp = C;
   Cp.E = C.E + P.entryBase;
@@ -718,9 +743,8 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   const Entry* E = Cp.E;
   const int8_t* EW = Cp.EW;
 
-  if (mask && lane == 0) add_run(C, L, 0, W, kClear);  // default for mask rows
-
   if (two && !aa) {  // paths.nim:1644-1668
+    if (mask && lane == 0) add_run(C, L, 0, W, kClear);  // default for mask rows
     if (lane == 0) {
       int64_t left = nim_f2i(E[0].b), right = nim_f2i(E[1].b);
       int64_t minX = left < 0 ? 0 : (left > W ? W : left);
@@ -751,12 +775,16 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
       }
       uint64_t m = ballot(s);
       int pos = ne + __popcll(m & ((1ull << lane) - 1ull));
-      if (s) L.A[pos] = i;
+      if (s && pos < L.cap) L.A[pos] = i;
       ne += __popcll(m);
       if (ballot(notSpan)) allSpan = false;
     }
   }
   wave_sync();
+  // the row's arrays hold the selection (not the whole band); D is untouched
+  // so far, so a selection too large for them leaves nothing behind
+  if (ne > L.cap) return -1;
+  if (mask && lane == 0) add_run(C, L, 0, W, kClear);  // default for mask rows
   const bool inReg = ne <= kWave;
   float sAty = 0.0f, sToy = 0.0f, sM = 0.0f, sB = 0.0f;
   int32_t sW = 0;

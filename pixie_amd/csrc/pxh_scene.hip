@@ -1189,8 +1189,11 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
     int pi = (y - F.startY) / F.bandH;
     if (pi > F.nParts - 1) pi = F.nParts - 1;
     const int cnt = uni(C.parts[pi].count);
-    RowLds& L = cnt <= P.ldsCap ? Ll : Lg;
-    int32_t kind = cnt <= P.gcap ? row_program(C, L) : -1;
+    // a large band's row whose selection fits LDS still runs there
+    bool inLds = cnt <= P.ldsCap;
+    if (!inLds && cnt <= P.gcap) inLds = row_selection_count(C) <= P.ldsCap;
+    RowLds& L = inLds ? Ll : Lg;
+    int32_t kind = (inLds || cnt <= P.gcap) ? row_program(C, L) : -1;
     RowRuns rr;
     rr.off = 0;
     rr.n = 0;
