@@ -714,19 +714,23 @@ __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
         may = k >= cr.x && k <= cr.y;
       }
       uint64_t cm = __ballot(may);
-      while (cm) {  // two candidate chunks at a time: their loads overlap
-        const int c0 = cb + __builtin_ctzll(cm);
-        cm &= cm - 1;
-        const int c1 = cm ? cb + __builtin_ctzll(cm) : -1;
-        if (cm) cm &= cm - 1;
-        const int j0 = (c0 << 6) + lane, j1 = (c1 << 6) + lane;
-        int2 sb0 = make_int2(1, 0), sb1 = make_int2(1, 0);  // empty band ranges
-        if (j0 < s1) sb0 = P.segBand[j0];
-        if (c1 >= 0 && j1 < s1) sb1 = P.segBand[j1];
-        const bool in0 = k >= sb0.x && k <= (sb0.y & ~kSegAA);
-        const bool in1 = k >= sb1.x && k <= (sb1.y & ~kSegAA);
-        cnt += __popcll(__ballot(in0)) + __popcll(__ballot(in1));
-        if (__ballot((in0 && (sb0.y & kSegAA)) || (in1 && (sb1.y & kSegAA)))) aa = true;
+      while (cm) {  // four candidate chunks at a time: their loads overlap
+        int2 sb[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          sb[u] = make_int2(1, 0);  // empty band range
+          if (cm) {
+            const int j = ((cb + __builtin_ctzll(cm)) << 6) + lane;
+            cm &= cm - 1;
+            if (j < s1) sb[u] = P.segBand[j];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const bool in = k >= sb[u].x && k <= (sb[u].y & ~kSegAA);
+          cnt += __popcll(__ballot(in));
+          if (__ballot(in && (sb[u].y & kSegAA))) aa = true;
+        }
       }
     }
     if (lane == 0) {
