@@ -700,6 +700,7 @@ __device__ __forceinline__ int row_selection_count(const RowCtx& C) {
   const Entry* E = C.E + __builtin_amdgcn_readfirstlane(C.parts[pi].entryBase);
   const float scanTop = (float)y, scanBottom = (float)(y + 1);
   int ne = 0;
+#pragma unroll 2
   for (int base = 0; base < count; base += kWave) {
     const int i = base + lane_id();
     bool s = false;
@@ -765,6 +766,7 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     }
     ne = 2;
   } else {
+#pragma unroll 2
     for (int base = 0; base < P.count; base += kWave) {
       int i = base + lane;
       bool s = false, notSpan = false;
