@@ -84,17 +84,66 @@ struct StageEmit {
   int cap;
   const float* mat;
   bool xf;
+  bool writer = true;  // lanes of a flattening group count alike; one writes
   int n = 0;
   __device__ void operator()(V2 a, V2 b) {
     V2 d = a - b;
     if (d.x == 0.0f && d.y == 0.0f) return;  // addSegment skips zero-length pieces
-    if (n < cap) {
+    if (writer && n < cap) {
       V2 p = xf ? mat_apply(mat, b) : b;
       buf[n] = make_float2(p.x, p.y);
     }
     n++;
   }
 };
+
+// flatten_cubic (paths.nim:677-721) run by a group of kFlatGroup lanes of
+// one wave (lanes gbase .. gbase + kFlatGroup - 1, u = lane - gbase).  Each
+// round lane u tries the step halved u times; the first candidate in
+// halving order that is accepted, or that reaches epsilon, decides the round,
+// which is the decision the sequential loop reaches after u halvings (each
+// halving and its reuse of the halfway point are exact, so the candidates
+// evaluate the same points).  Every lane of the group calls emit alike.
+constexpr int kFlatGroup = 8;
+template <class Emit>
+__device__ int32_t flatten_cubic_group(V2 at, V2 c1, V2 c2, V2 to, float errSq, Emit& emit, int u,
+                                       int gbase) {
+  float t = 0.0f, step = 1.0f;
+  V2 prev = at;
+  for (;;) {
+    float s = step;
+    for (int q = 0; q < u; q++) s /= 2.0f;
+    const bool eps = s <= kFltEpsilon;
+    bool ok = false;
+    V2 next = v2(0.0f, 0.0f);
+    if (!eps) {
+      next = cubic_at(at, c1, c2, to, t + s);
+      const V2 halfway = cubic_at(at, c1, c2, to, t + s / 2.0f);
+      const V2 midpoint = (prev + next) / 2.0f;
+      const V2 lineTangent = midpoint - prev;
+      const V2 curveTangent = cubic_deriv(at, c1, c2, to, t + s / 2.0f);
+      const V2 curveTangentScaled = normalize(curveTangent) * len(lineTangent);
+      const float error = lensq(midpoint - halfway);
+      const float errorTangent = lensq(lineTangent - curveTangentScaled);
+      ok = !(error + errorTangent > errSq);
+    }
+    const uint64_t bm = (__ballot(eps || ok) >> gbase) & ((1ull << kFlatGroup) - 1ull);
+    if (bm == 0) {  // every candidate was refined: go on from step / 2^kFlatGroup
+      for (int q = 0; q < kFlatGroup; q++) step /= 2.0f;
+      continue;
+    }
+    const int src = gbase + __builtin_ctzll(bm);
+    if (__shfl(eps ? 1 : 0, src, 64)) return kErrDiscretize;
+    const float sw = __shfl(s, src, 64);
+    const V2 nw = v2(__shfl(next.x, src, 64), __shfl(next.y, src, 64));
+    emit(prev, nw);
+    t += sw;
+    if (t == 1.0f) break;
+    prev = nw;
+    step = nim_minf(sw * 2.0f, 1.0f - t);
+  }
+  return kOk;
+}
 
 struct StoreEmit {
   float2* out;
@@ -352,15 +401,43 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   const int slice = stage && I.cmdCount > 0 ? kStagePts / I.cmdCount : 0;
   bool staged = false;
   int myN = 0;
-  for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
-    int i = I.cmdBase + c0 + tid;
-    if (c0 + tid < I.cmdCount) {
-      StageEmit se{sPts + tid * slice, slice, I.mat, xf};
-      int32_t r = cmd_emit(P.cmds, i, I.errSq, se);
-      if (r != kOk) atomicMax(&sErr, r);
-      P.cmdCount[i] = se.n;
-      myN = se.n;
-      staged = stage && se.n <= slice;
+  if (stage && I.cmdCount <= kBlock / kFlatGroup) {
+    // few commands: kFlatGroup lanes per command, so a cubic's adaptive
+    // steps are tried kFlatGroup halvings at a time
+    const int c = tid / kFlatGroup, u = tid % kFlatGroup;
+    if (c < I.cmdCount) {
+      const int i = I.cmdBase + c;
+      StageEmit se{sPts + c * slice, slice, I.mat, xf};
+      se.writer = u == 0;
+      int32_t r;
+      const SceneCmd& cm = P.cmds[i];
+      if ((int)cm.kind == 5)
+        r = flatten_cubic_group(cmd_end(P.cmds[i - 1]), v2(cm.p[0], cm.p[1]), v2(cm.p[2], cm.p[3]),
+                                v2(cm.p[4], cm.p[5]), I.errSq, se, u, tid & 63 & ~(kFlatGroup - 1));
+      else
+        r = cmd_emit(P.cmds, i, I.errSq, se);
+      if (u == 0) {
+        if (r != kOk) atomicMax(&sErr, r);
+        P.cmdCount[i] = se.n;
+        sPre[c] = se.n;
+      }
+    }
+    __syncthreads();
+    if (tid < I.cmdCount) {
+      myN = sPre[tid];
+      staged = myN <= slice;
+    }
+  } else {
+    for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
+      int i = I.cmdBase + c0 + tid;
+      if (c0 + tid < I.cmdCount) {
+        StageEmit se{sPts + tid * slice, slice, I.mat, xf};
+        int32_t r = cmd_emit(P.cmds, i, I.errSq, se);
+        if (r != kOk) atomicMax(&sErr, r);
+        P.cmdCount[i] = se.n;
+        myN = se.n;
+        staged = stage && se.n <= slice;
+      }
     }
   }
   __syncthreads();
