@@ -946,43 +946,44 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
           may = k >= cr.x && k <= cr.y;
         }
         uint64_t cm = __ballot(may);
-        while (cm && pos < pi.count) {  // two candidate chunks at a time
-          const int c0 = cb + __builtin_ctzll(cm);
-          cm &= cm - 1;
-          const int c1 = cm ? cb + __builtin_ctzll(cm) : -1;
-          if (cm) cm &= cm - 1;
-          const int j0 = (c0 << 6) + lane, j1 = (c1 << 6) + lane;
-          int2 sb0 = make_int2(1, 0), sb1 = make_int2(1, 0);  // empty band ranges
-          if (j0 < s1) sb0 = P.segBand[j0];
-          if (c1 >= 0 && j1 < s1) sb1 = P.segBand[j1];
-          const bool in0 = k >= sb0.x && k <= (sb0.y & ~kSegAA);
-          const bool in1 = k >= sb1.x && k <= (sb1.y & ~kSegAA);
-          float4 sg0 = make_float4(0, 0, 0, 0), sg1 = sg0;
-          int8_t w0 = 0, w1 = 0;
-          if (in0) {
-            sg0 = P.segs[j0];
-            w0 = P.segW[j0];
+        while (cm && pos < pi.count) {  // four candidate chunks at a time
+          int2 sb[4];
+          int js[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            sb[u] = make_int2(1, 0);  // empty band range
+            js[u] = 0;
+            if (cm) {
+              js[u] = ((cb + __builtin_ctzll(cm)) << 6) + lane;
+              cm &= cm - 1;
+              if (js[u] < s1) sb[u] = P.segBand[js[u]];
+            }
           }
-          if (in1) {
-            sg1 = P.segs[j1];
-            w1 = P.segW[j1];
+          bool in[4];
+          float4 sg[4];
+          int8_t sw[4];
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            in[u] = k >= sb[u].x && k <= (sb[u].y & ~kSegAA);
+            sg[u] = make_float4(0, 0, 0, 0);
+            sw[u] = 0;
+            if (in[u]) {
+              sg[u] = P.segs[js[u]];
+              sw[u] = P.segW[js[u]];
+            }
           }
-          const uint64_t m0 = __ballot(in0), m1 = __ballot(in1);
           const uint64_t below = (1ull << lane) - 1ull;
-          if (in0) {
-            const int p = pos + __popcll(m0 & below);
-            float4 cl;
-            P.entries[pi.entryBase + p] = band_entry(sg0, top, bot, &cl);
-            P.entW[pi.entryBase + p] = w0;
+#pragma unroll
+          for (int u = 0; u < 4; u++) {
+            const uint64_t m = __ballot(in[u]);
+            if (in[u]) {
+              const int p = pos + __popcll(m & below);
+              float4 cl;
+              P.entries[pi.entryBase + p] = band_entry(sg[u], top, bot, &cl);
+              P.entW[pi.entryBase + p] = sw[u];
+            }
+            pos += __popcll(m);
           }
-          pos += __popcll(m0);
-          if (in1) {
-            const int p = pos + __popcll(m1 & below);
-            float4 cl;
-            P.entries[pi.entryBase + p] = band_entry(sg1, top, bot, &cl);
-            P.entW[pi.entryBase + p] = w1;
-          }
-          pos += __popcll(m1);
         }
       }
       continue;
