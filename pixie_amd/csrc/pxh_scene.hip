@@ -1423,6 +1423,18 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
           span = P.itemSpan[it];
         }
       }
+      // each covering lane also fetches its call's first phase-A window
+      // record for this row and window (pre.w = that window, -1 if none)
+      int4 pre = make_int4(0, 0, 0, -1);
+      if (cov && (paint.y != kMask || (y >= span.x && y < span.x + span.y))) {
+        const int winBase = paint.y == kMask ? 0 : (span.z & ~3);
+        const int d = b0 - kWinA - winBase;  // windows ending before b0 are skipped
+        const int w = d > 0 ? (d + kWinA - 1) / kWinA : 0;
+        if (w < paint.w && winBase + w * kWinA < b1) {
+          const RowRuns rr = P.tasks[paint.z + (y - span.x) * paint.w + w];
+          pre = make_int4(rr.off, rr.n, rr.kind, w);
+        }
+      }
       uint64_t m = __ballot(cov);
       while (m) {
         const int src = __builtin_ctzll(m);
@@ -1435,6 +1447,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         const int rowCount = __builtin_amdgcn_readlane(span.y, src);
         const int xlo = __builtin_amdgcn_readlane(span.z, src);
         const int xhi = __builtin_amdgcn_readlane(span.w, src);
+        const int preW = __builtin_amdgcn_readlane(pre.w, src);
         if (!loaded) {  // canvas window -> LDS
 #pragma unroll
           for (int c = 0; c < kChunksB; c++) {
@@ -1495,10 +1508,17 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
           const int wx0 = winBase + w * kWinA;
           if (wx0 >= b1) break;
           if (wx0 + kWinA < b0) continue;
-          RowRuns rr = P.tasks[tb + w];
-          rr.off = uni(rr.off);
-          rr.n = uni(rr.n);
-          rr.kind = uni(rr.kind);
+          RowRuns rr;
+          if (w == preW) {
+            rr.off = __builtin_amdgcn_readlane(pre.x, src);
+            rr.n = __builtin_amdgcn_readlane(pre.y, src);
+            rr.kind = __builtin_amdgcn_readlane(pre.z, src);
+          } else {
+            rr = P.tasks[tb + w];
+            rr.off = uni(rr.off);
+            rr.n = uni(rr.n);
+            rr.kind = uni(rr.kind);
+          }
           if (rr.kind > kind) kind = rr.kind;
           for (int e0 = 0; e0 < rr.n; e0 += 64) {
             int e = e0 + lane;
