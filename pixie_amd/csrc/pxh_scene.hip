@@ -1150,7 +1150,7 @@ __device__ __forceinline__ int task_class(const ScenePools& P, const ItemGeom& g
 // global atomic per class and block) the cost class of every task.  The
 // classes of a call's bands are staged in LDS first.
 constexpr int kLdsBandClasses = 4096;
-constexpr int kTaskMapBlock = 256;
+constexpr int kTaskMapBlock = 1024;
 __global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePools P) {
   __shared__ int32_t hist[kCostClasses];
   __shared__ int8_t cls[kLdsBandClasses];
@@ -1673,7 +1673,7 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
 }
 
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
-  hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid), dim3(kTaskMapBlock), 0, s, p);
+  hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid / 4 > 0 ? grid / 4 : 1), dim3(kTaskMapBlock), 0, s, p);
   hipLaunchKernelGGL(scene_task_scan_kernel, dim3(1), dim3(64), 0, s, p.counters);
   hipLaunchKernelGGL(scene_task_order_kernel, dim3(grid >= 8 ? grid / 4 : grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<false>, dim3(grid), dim3(64), 0, s, p);
