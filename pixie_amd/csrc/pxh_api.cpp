@@ -27,6 +27,7 @@ struct pxh_image {
 
 struct pxh_scene {
   std::vector<pxh::SceneItem> items;
+  std::vector<int32_t> order;  // geometry workgroup -> call (built at upload)
   std::vector<pxh::SceneCmd> cmds;
   std::vector<pxh::ScenePoly> polys;
   bool uploaded = false;
@@ -53,7 +54,8 @@ struct pxh_scene {
   pxh::SceneItem* dItems = nullptr;
   pxh::SceneCmd* dCmds = nullptr;
   pxh::ScenePoly* dPolys = nullptr;
-  size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0;
+  int32_t* dOrder = nullptr;  // geometry workgroup -> call, longest call first
+  size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0, dOrderCap = 0;
   // per-render buffers
   char* dMem = nullptr;
   size_t dMemBytes = 0;
@@ -533,12 +535,14 @@ void scene_free_device(pxh_scene* sc) {
   if (sc->dItems) (void)hipFree(sc->dItems);
   if (sc->dCmds) (void)hipFree(sc->dCmds);
   if (sc->dPolys) (void)hipFree(sc->dPolys);
+  if (sc->dOrder) (void)hipFree(sc->dOrder);
   if (sc->dMem) (void)hipFree(sc->dMem);
   sc->dItems = nullptr;
   sc->dCmds = nullptr;
   sc->dPolys = nullptr;
+  sc->dOrder = nullptr;
   sc->dMem = nullptr;
-  sc->dItemsCap = sc->dCmdsCap = sc->dPolysCap = 0;
+  sc->dItemsCap = sc->dCmdsCap = sc->dPolysCap = sc->dOrderCap = 0;
   sc->dMemBytes = 0;
 }
 
@@ -579,6 +583,7 @@ int scene_alloc(pxh_scene* sc) {
   p.items = sc->dItems;
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
+  p.itemOrder = sc->dOrder;
   p.nItems = (int32_t)nI;
   p.geo = (ItemGeom*)(b + oGeo);
   p.fills = (FillInfo*)(b + oFill);
@@ -647,6 +652,25 @@ int scene_upload(pxh_scene* sc, int lane) {
   if (!sc->polys.empty())
     PXH_HIP(hipMemcpyAsync(sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly),
                            hipMemcpyHostToDevice, st));
+  // Geometry workgroups start longest call first, so that a long stroke does
+  // not start late and hold the kernel's end: estimated cost = commands,
+  // times the work of a stroke point (heavier with round joins or caps).
+  {
+    const size_t n = sc->items.size();
+    std::vector<int64_t> cost(n);
+    for (size_t i = 0; i < n; i++) {
+      const SceneItem& it = sc->items[i];
+      int64_t c = it.cmdCount;
+      if (it.stroke && !it.strokeNone) c *= (it.join == 1 || it.cap == 1) ? 16 : 4;
+      cost[i] = c;
+    }
+    sc->order.resize(n);
+    for (size_t i = 0; i < n; i++) sc->order[i] = (int32_t)i;
+    std::stable_sort(sc->order.begin(), sc->order.end(),
+                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
+    if (int r = scene_buf(&sc->dOrder, &sc->dOrderCap, n)) return r;
+    PXH_HIP(hipMemcpyAsync(sc->dOrder, sc->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  }
   sc->lane = lane;
   // first estimates of the pools; a render that overflows grows them
   size_t nC = sc->cmds.size();
@@ -672,6 +696,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   p.items = sc->dItems;
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
+  p.itemOrder = sc->dOrder;
   const RowShape shape = kRowShapes[sc->rowShape];
   const int rowGrid = scene_queue_grid(g.numCUs * shape.wavesPerCU);
   p.ldsCap = shape.ldsCap;

@@ -125,6 +125,7 @@ struct ScenePools {
   const SceneItem* items;
   const SceneCmd* cmds;
   const ScenePoly* polys;
+  const int32_t* itemOrder;  // geometry workgroup -> call
   int32_t nItems;
   int32_t W, H;              // target size
   // per-call outputs

@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   __shared__ int sErr;
   __shared__ float2 sPts[kStagePts];
   __shared__ int sPre[kBlock + 1];
-  const int it = blockIdx.x;
+  const int it = P.itemOrder[blockIdx.x];
   const int tid = threadIdx.x;
   const SceneItem I = P.items[it];
   const bool xf = !I.stroke && !I.identity;  // fills transform points here
