@@ -49,7 +49,7 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            raise ImportError("libpixie_hip.so is not built (run __graft_entry__.build())")
+            raise ImportError("libpixie_hip.so is not built (run make -C pixie_amd/csrc)")
         L = ctypes.CDLL(LIB_PATH)
         P = ctypes.POINTER
         vp, f32p, i32p = ctypes.c_void_p, P(ctypes.c_float), P(ctypes.c_int32)
