@@ -559,6 +559,7 @@ int scene_alloc(pxh_scene* sc) {
   size_t oGeo = carve(nI * sizeof(ItemGeom)), oFill = carve(nI * sizeof(FillInfo));
   size_t oCC = carve(nC * 4), oCO = carve(nC * 4), oPP = carve(nP * 4), oPN = carve(nP * 4);
   size_t oPts = carve((size_t)sc->ptCap * 8), oPtP = carve((size_t)sc->ptCap * 4);
+  size_t oPtS = carve((size_t)sc->ptCap * 4);
   size_t oSeg = carve((size_t)sc->segCap * 16), oSegW = carve((size_t)sc->segCap);
   size_t oSegB = carve((size_t)sc->segCap * 8), oSegC = carve(((size_t)sc->segCap / 64 + 1) * 8);
   size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
@@ -593,6 +594,7 @@ int scene_alloc(pxh_scene* sc) {
   p.polyN = (int32_t*)(b + oPN);
   p.pts = (float2*)(b + oPts);
   p.ptPoly = (int32_t*)(b + oPtP);
+  p.ptSeg = (int32_t*)(b + oPtS);
   p.ptCap = sc->ptCap;
   p.segs = (float4*)(b + oSeg);
   p.segW = (int8_t*)(b + oSegW);

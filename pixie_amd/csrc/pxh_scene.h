@@ -139,6 +139,7 @@ struct ScenePools {
   // pools
   float2* pts;
   int32_t* ptPoly;
+  int32_t* ptSeg;            // segments each point emits (geometry scratch)
   int32_t ptCap;
   float4* segs;
   int8_t* segW;

@@ -503,6 +503,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   }
   float2* pts = P.pts + ptBase;
   int32_t* ptPoly = P.ptPoly + ptBase;
+  int32_t* ptSeg = P.ptSeg + ptBase;  // segments of each point (stage 5)
 
   // 3. per-command point offsets, start and closing points (thread per subpath)
   for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
@@ -572,7 +573,10 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       int j = j0 + tid;
       SegSink sink;
       int cnt = 0;
-      if (j < total) {
+      if (pass == 1) {
+        // the counts of pass 0 (this thread wrote them)
+        if (j < total) cnt = ptSeg[j];
+      } else if (j < total) {
         int ql = ptPoly[j];
         int q = I.polyBase + ql;
         int pb = P.polyPt[q], np = P.polyN[q];
@@ -591,6 +595,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
           stroke_point(pts + pb, k, np, I, sink);
           cnt = sink.n;
         }
+        ptSeg[j] = cnt;
       }
       int t;
       int off = block_excl_scan(cnt, &t, sh);
