@@ -554,7 +554,7 @@ __device__ __forceinline__ void row_coverage_small(const RowCtx& C, RowLds& L, i
 // Finishes sorting n distinct keys that are nearly in order: odd-even
 // transposition passes until one swaps nothing, and a full sort once
 // kFixPasses passes have not been enough.
-constexpr int kFixPasses = 4;
+constexpr int kFixPasses = 8;
 __device__ void wave_fix_keys(uint64_t* keys, int n, int cap) {
   const int lane = lane_id();
   for (int pass = 0;; pass++) {
