@@ -554,7 +554,8 @@ __device__ __forceinline__ void row_coverage_small(const RowCtx& C, RowLds& L, i
 // Finishes sorting n distinct keys that are nearly in order: odd-even
 // transposition passes until one swaps nothing, and a full sort once
 // kFixPasses passes have not been enough.
-constexpr int kFixPasses = 8;
+constexpr int kFixPasses = 8;     // keys in registers
+constexpr int kFixPassesMem = 16;  // keys in LDS or global memory (more than 64 hits)
 __device__ void wave_fix_keys(uint64_t* keys, int n, int cap) {
   const int lane = lane_id();
   for (int pass = 0;; pass++) {
@@ -571,7 +572,7 @@ __device__ void wave_fix_keys(uint64_t* keys, int n, int cap) {
       wave_sync();
     }
     if (ballot(swapped) == 0) return;
-    if (pass + 1 >= kFixPasses) {
+    if (pass + 1 >= kFixPassesMem) {
       wave_sort_keys(keys, n, cap);
       return;
     }
