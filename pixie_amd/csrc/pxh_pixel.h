@@ -152,12 +152,14 @@ PXH_HD FColor clip_color_f(FColor c) {
   float l = lum_f(c);
   float n = fminf(fminf(c.r, c.g), c.b);
   float x = fmaxf(fmaxf(c.r, c.g), c.b);
-  if (n < 0.0f) {
+  /* the denominators can reach 0 through float32 rounding when every
+   * channel sits at the luminosity; the spec's limit there is no change */
+  if (n < 0.0f && l - n > 0.0f) {
     c.r = l + (c.r - l) * l / (l - n);
     c.g = l + (c.g - l) * l / (l - n);
     c.b = l + (c.b - l) * l / (l - n);
   }
-  if (x > 1.0f) {
+  if (x > 1.0f && x - l > 0.0f) {
     c.r = l + (c.r - l) * (1.0f - l) / (x - l);
     c.g = l + (c.g - l) * (1.0f - l) / (x - l);
     c.b = l + (c.b - l) * (1.0f - l) / (x - l);

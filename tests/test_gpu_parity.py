@@ -152,6 +152,7 @@ def test_star512(gpu, orc):
     got = img.download()
     _cov_check(got, want, "star512")
     assert st["touched"] > 100000
+    assert st["dropped"] == 0
 
 
 @pytest.mark.parametrize("seed", [2, 12])
@@ -160,16 +161,19 @@ def test_mixed_scene_small(gpu, orc, seed):
     want = canvas.copy()
     img = px.Image.from_array(canvas)
     px.touched_pixels(reset=True)
-    touched = 0
+    touched = dropped = 0
     for c in cmds:
         if c.stroke:
-            touched += orc.stroke_path(want, c.cmds, c.color, width=c.width, cap=c.cap,
-                                       join=c.join, miter_limit=c.miter, mode=c.mode)["touched"]
+            st = orc.stroke_path(want, c.cmds, c.color, width=c.width, cap=c.cap,
+                                 join=c.join, miter_limit=c.miter, mode=c.mode)
             img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
                            miterLimit=c.miter, blendMode=c.mode)
         else:
-            touched += orc.fill_path(want, c.cmds, c.color, winding=c.winding, mode=c.mode)["touched"]
+            st = orc.fill_path(want, c.cmds, c.color, winding=c.winding, mode=c.mode)
             img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+        touched += st["touched"]
+        dropped += st["dropped"]
+    assert dropped == 0
     _cov_check(img.download(), want, "mixed seed %d" % seed)
     # the GPU's N_touch counter (the bench's algorithmic-byte unit) matches
     assert px.touched_pixels(reset=True) == touched
@@ -182,7 +186,7 @@ def test_glyphs_into_a8(gpu, orc):
     for _ in range(8):
         p = WL.glyph_path(rng, 200, 200)
         img.fillPath(p, (255, 255, 255, 255))
-        orc.fill_path(want, p.array(), (255, 255, 255, 255))
+        assert orc.fill_path(want, p.array(), (255, 255, 255, 255))["dropped"] == 0
     got = img.download()
     assert np.array_equal(want[..., 0], want[..., 3])
     _cov_check(got, want[..., 3], "glyphs a8")

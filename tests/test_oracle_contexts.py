@@ -1,0 +1,492 @@
+"""Pins the oracle with the reference's Context masters (tests/test_contexts.nim).
+
+Context (contexts.nim) is not on the raster path, but every drawing call it
+makes funnels into the path's seams: `fill` / `stroke` call fillPath /
+strokePath with the context's matrix (contexts.nim:139-177), `clip` fills a
+mask with OverwriteBlend or MaskBlend (:384-400), layers are composited with
+`draw` (MaskBlend for the clip, Normal onto the parent, :100-137), and
+globalAlpha goes through applyOpacity (:147-155).  The small emulator below
+replays those bodies on the oracle, so each master pins the oracle's fill,
+stroke, EvenOdd winding, Exclusion blend, Overwrite, Mask, applyOpacity and
+draw on scenes the reference itself rendered.
+
+The masters are straight alpha and were written by a SIMD build of pixie,
+whose coverage multiply floors where the scalar path rounds, so colour
+channels of partially covered pixels may differ by 1; alpha must be exact.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from pixie_amd.paths import Path, mat3, mat_mul, translate, scale, sinf, cosf, f32
+
+NONZERO, EVENODD = 0, 1
+NORMAL, EXCLUSION, MASK, OVERWRITE = 0, 11, 16, 17
+BUTT, ROUND, SQUARE = 0, 1, 2
+MITER, ROUNDJ, BEVEL = 0, 1, 2
+
+# CSS colour keywords and hex forms used by the tests (chroma's
+# parseHtmlColor); all opaque, so their rgbx is the rgba itself.
+CSS = {"black": (0, 0, 0), "blue": (0, 0, 255), "green": (0, 128, 0), "red": (255, 0, 0),
+       "orange": (255, 165, 0), "pink": (255, 192, 203), "gray": (128, 128, 128),
+       "#ff6": (255, 255, 102), "#38f": (51, 136, 255)}
+
+
+def rotate(angle):
+    """vmath rotate(angle) as a Mat3 (x' = x cos - y sin, y' = x sin + y cos);
+    the rotate_simple and rotate_1 masters fix the sign (the opposite one
+    leaves the red rectangle off the canvas)."""
+    a = f32(angle)
+    s, c = sinf(a), cosf(a)
+    return np.array([c, s, 0, -s, c, 0, 0, 0, 1], np.float32)
+
+
+class Paint:
+    def __init__(self, rgbx=(0, 0, 0, 255), mode=NORMAL):
+        self.rgbx = tuple(rgbx)
+        self.mode = mode
+
+    def copy(self):
+        return Paint(self.rgbx, self.mode)
+
+
+class Ctx:
+    """contexts.nim's Context, restricted to solid paints and no text."""
+
+    def __init__(self, orc, image):
+        self.orc = orc
+        self.image = image
+        self.mat = mat3()
+        self.path = Path()
+        self.global_alpha = 1.0
+        self.line_width = 1.0
+        self.miter_limit = 10.0          # contexts.nim:59
+        self.line_cap = BUTT
+        self.line_join = MITER
+        self.fill_style = Paint()
+        self.stroke_style = Paint()
+        self.line_dash = ()
+        self.mask = None
+        self.layer = None
+        self.stack = []
+
+    # ---- styles ----
+    def set_fill(self, name):
+        self.fill_style = Paint(CSS[name] + (255,))
+
+    def set_stroke(self, name):
+        self.stroke_style = Paint(CSS[name] + (255,))
+
+    # ---- state (contexts.nim:71-137) ----
+    def _state(self, layer=None):
+        return dict(fill=self.fill_style, stroke=self.stroke_style, ga=self.global_alpha,
+                    lw=self.line_width, ml=self.miter_limit, cap=self.line_cap,
+                    join=self.line_join, dash=self.line_dash, mat=self.mat.copy(),
+                    mask=None if self.mask is None else self.mask.copy(), layer=layer)
+
+    def save(self):
+        self.stack.append(self._state())
+        self.fill_style = self.fill_style.copy()
+        self.stroke_style = self.stroke_style.copy()
+
+    def save_layer(self):
+        self.stack.append(self._state(layer=self.layer))
+        h, w = self.image.shape[:2]
+        self.layer = self.orc.new_image(w, h)
+
+    def restore(self):
+        if not self.stack:
+            return
+        popped_layer, popped_mask = self.layer, self.mask
+        st = self.stack.pop()
+        self.fill_style, self.stroke_style = st["fill"], st["stroke"]
+        self.global_alpha, self.line_width, self.miter_limit = st["ga"], st["lw"], st["ml"]
+        self.line_cap, self.line_join, self.line_dash = st["cap"], st["join"], st["dash"]
+        self.mat, self.mask, self.layer = st["mat"], st["mask"], st["layer"]
+        if popped_layer is not None:
+            if popped_mask is not None:
+                self.orc.draw(popped_layer, popped_mask, 0, 0, MASK)
+            self.orc.draw(self.layer if self.layer is not None else self.image, popped_layer)
+
+    # ---- fillPath / strokePath bodies for solid paints (paths.nim:2093-2176) ----
+    def _fill_path(self, img, path, paint, mat, rule):
+        if paint.rgbx[3] == 0 and paint.mode != OVERWRITE:
+            return
+        self.orc.fill_path(img, path.array(), paint.rgbx, winding=rule, mode=paint.mode, mat=mat)
+
+    def _fill_into(self, img, path, rule):
+        if self.global_alpha != 1:
+            self.save_layer()
+            img = self.layer
+        self._fill_path(img, path, self.fill_style, self.mat, rule)
+        if self.global_alpha != 1:
+            self.orc.apply_opacity(self.layer, self.global_alpha)
+            self.restore()
+
+    def _stroke_into(self, img, path):
+        if self.global_alpha != 1:
+            self.save_layer()
+            img = self.layer
+        p = self.stroke_style
+        if not (p.rgbx[3] == 0 and p.mode != OVERWRITE):
+            self.orc.stroke_path(img, path.array(), p.rgbx, width=self.line_width,
+                                 cap=self.line_cap, join=self.line_join,
+                                 miter_limit=self.miter_limit, dashes=self.line_dash,
+                                 mode=p.mode, mat=self.mat)
+        if self.global_alpha != 1:
+            self.orc.apply_opacity(self.layer, self.global_alpha)
+            self.restore()
+
+    def fill(self, path=None, rule=NONZERO):
+        path = self.path if path is None else path
+        if self.mask is not None and self.layer is None:
+            self.save_layer()
+            self._fill_into(self.layer, path, rule)
+            self.restore()
+        elif self.layer is not None:
+            self._fill_into(self.layer, path, rule)
+        else:
+            self._fill_into(self.image, path, rule)
+
+    def stroke(self, path=None):
+        path = self.path if path is None else path
+        if self.mask is not None and self.layer is None:
+            self.save_layer()
+            self._stroke_into(self.layer, path)
+            self.restore()
+        elif self.layer is not None:
+            self._stroke_into(self.layer, path)
+        else:
+            self._stroke_into(self.image, path)
+
+    def clip(self, path=None, rule=NONZERO):
+        path = self.path if path is None else path
+        if self.mask is None:
+            h, w = self.image.shape[:2]
+            self.mask = self.orc.new_image(w, h)
+            self._fill_path(self.mask, path, Paint((255, 255, 255, 255), OVERWRITE), mat3(), rule)
+        else:
+            self._fill_path(self.mask, path, Paint((255, 255, 255, 255), MASK), mat3(), rule)
+
+    def clear_rect(self, x, y, w, h):
+        p = Path(); p.rect(x, y, w, h)
+        self._fill_path(self.layer if self.layer is not None else self.image, p,
+                        Paint((0, 0, 0, 0), OVERWRITE), self.mat, NONZERO)
+
+    def fill_rect(self, x, y, w, h):
+        p = Path(); p.rect(x, y, w, h)
+        self.fill(p)
+
+    def stroke_rect(self, x, y, w, h):
+        p = Path(); p.rect(x, y, w, h)
+        self.stroke(p)
+
+    # ---- path and transform ----
+    def begin_path(self):
+        self.path = Path()
+
+    def move_to(self, x, y): self.path.move_to(x, y)
+    def line_to(self, x, y): self.path.line_to(x, y)
+    def close_path(self): self.path.close_path()
+    def rect(self, x, y, w, h): self.path.rect(x, y, w, h)
+    def ellipse(self, x, y, rx, ry): self.path.ellipse(x, y, rx, ry)
+    def circle(self, x, y, r): self.path.circle(x, y, r)
+
+    def bezier_curve_to(self, *a): self.path.bezier_curve_to(*a)
+    def quadratic_curve_to(self, *a): self.path.quadratic_curve_to(*a)
+
+    def set_transform(self, m): self.mat = np.asarray(m, np.float32)
+    def transform(self, m): self.mat = mat_mul(self.mat, m)
+    def translate(self, x, y): self.mat = mat_mul(self.mat, translate(x, y))
+    def scale(self, x, y): self.mat = mat_mul(self.mat, scale(x, y))
+    def rotate(self, a): self.mat = mat_mul(self.mat, rotate(a))
+    def reset_transform(self): self.mat = mat3()
+
+
+def _new(orc, w=300, h=150, fill=None):
+    img = orc.new_image(w, h)
+    if fill is not None:
+        orc.fill(img, fill)
+    return Ctx(orc, img)
+
+
+def _cmp(masters, name, img, alpha_off=0):
+    m = masters("tests/contexts/%s.png" % name)
+    assert m.shape == img.shape, name
+    d = np.abs(m.astype(int) - img.astype(int))
+    n_off = int((d[..., 3] > 0).sum())
+    assert d[..., 3].max() <= (1 if alpha_off else 0) and n_off <= alpha_off, \
+        "%s: alpha differs on %d px" % (name, n_off)
+    assert d.max() <= 1, "%s: colour differs by %d" % (name, d.max())
+    return d
+
+
+# Each scene restates one block of test_contexts.nim (line numbers cited).
+
+def s_clear_rect_1(c):                                  # :3-21
+    c.begin_path(); c.set_fill("#ff6"); c.fill_rect(0, 0, 300, 160)
+    c.begin_path(); c.set_fill("blue")
+    c.move_to(20, 20); c.line_to(180, 20); c.line_to(130, 130); c.close_path(); c.fill()
+    c.clear_rect(10, 10, 120, 100)
+
+
+def s_begin_path_1(c):                                  # :23-37
+    c.begin_path(); c.set_stroke("blue"); c.move_to(20, 20); c.line_to(200, 20); c.stroke()
+    c.begin_path(); c.set_stroke("green"); c.move_to(20, 20); c.line_to(120, 120); c.stroke()
+
+
+def s_move_to_1(c):                                     # :39-48
+    c.begin_path(); c.move_to(50, 50); c.line_to(200, 50); c.move_to(50, 90)
+    c.line_to(280, 120); c.stroke()
+
+
+def s_fill_1(c):                                        # :50-66, EvenOdd
+    r = Path()
+    r.move_to(30, 90); r.line_to(110, 20); r.line_to(240, 130); r.line_to(60, 130)
+    r.line_to(190, 20); r.line_to(270, 90); r.close_path()
+    c.set_fill("green"); c.fill(r, EVENODD)
+
+
+def s_stroke_1(c):                                      # :78-84
+    c.rect(10, 10, 150, 100); c.stroke()
+
+
+def s_stroke_2(c):                                      # :86-106, one growing path
+    c.line_width = 26; c.set_stroke("orange"); c.move_to(20, 20); c.line_to(160, 20); c.stroke()
+    c.line_width = 14; c.set_stroke("green"); c.move_to(20, 80); c.line_to(220, 80); c.stroke()
+    c.line_width = 4; c.set_stroke("pink"); c.move_to(20, 140); c.line_to(280, 140); c.stroke()
+
+
+def s_stroke_3(c):                                      # :108-123
+    c.line_width = 26; c.set_stroke("red")
+    c.begin_path(); c.rect(25, 25, 100, 100); c.fill(); c.stroke()
+    c.begin_path(); c.rect(175, 25, 100, 100); c.stroke(); c.fill()
+
+
+def s_close_path_1(c):                                  # :125-135
+    c.begin_path(); c.move_to(20, 140); c.line_to(120, 10); c.line_to(220, 140)
+    c.close_path(); c.stroke()
+
+
+def s_bezier_1(c):                                      # :137-151
+    c.begin_path(); c.move_to(50, 20); c.bezier_curve_to(230, 30, 150, 80, 250, 100); c.stroke()
+
+
+def s_bezier_2(c):                                      # :153-160
+    c.begin_path(); c.move_to(30, 30); c.bezier_curve_to(120, 160, 180, 10, 220, 140); c.stroke()
+
+
+def s_quad_1(c):                                        # :162-168
+    c.move_to(50, 20); c.quadratic_curve_to(230, 30, 50, 100); c.stroke()
+
+
+def s_quad_2(c):                                        # :170-177
+    c.begin_path(); c.move_to(20, 110); c.quadratic_curve_to(230, 150, 250, 20); c.stroke()
+
+
+def s_ellipse_1(c):                                     # :179-185
+    c.begin_path(); c.ellipse(100, 75, 75, 50); c.stroke()
+
+
+def s_stroke_rect_1(c):                                 # :187-192
+    c.set_stroke("green"); c.stroke_rect(20, 10, 160, 100)
+
+
+def s_stroke_rect_2(c):                                 # :194-201
+    c.line_join = BEVEL; c.line_width = 15; c.set_stroke("#38f"); c.stroke_rect(30, 30, 160, 90)
+
+
+SHEAR = [1, 0.2, 0, 0.8, 1, 0, 0, 0, 1]
+
+
+def s_set_transform_1(c):                               # :203-208
+    c.set_transform(SHEAR); c.fill_rect(0, 0, 100, 100)
+
+
+def s_rotate_simple(c):                                 # :217-222
+    c.rotate(45 * math.pi / 180); c.fill_rect(60, 0, 100, 30)
+
+
+def s_reset_transform_2(c):                             # :224-237
+    c.transform([1, 0, 0, 1.7, 1, 0, 0, 0, 1]); c.set_fill("gray")
+    c.fill_rect(40, 40, 50, 20); c.fill_rect(40, 90, 50, 20)
+    c.reset_transform(); c.set_fill("red")
+    c.fill_rect(40, 40, 50, 20); c.fill_rect(40, 90, 50, 20)
+
+
+def s_translate_1(c):                                   # :239-250
+    c.translate(110, 30); c.set_fill("red"); c.fill_rect(0, 0, 80, 80)
+    c.set_transform(mat3()); c.set_fill("gray"); c.fill_rect(0, 0, 80, 80)
+
+
+def s_scale_1(c):                                       # :252-263
+    c.scale(9, 3); c.set_fill("red"); c.fill_rect(10, 10, 8, 20)
+    c.set_transform(mat3()); c.set_fill("gray"); c.fill_rect(10, 10, 8, 20)
+
+
+def s_rotate_1(c):                                      # :265-274
+    c.set_fill("gray"); c.fill_rect(100, 0, 80, 20)
+    c.rotate(45 * math.pi / 180); c.set_fill("red"); c.fill_rect(100, 0, 80, 20)
+
+
+def s_save_1(c):                                        # :298-309
+    c.save(); c.set_fill("green"); c.fill_rect(10, 10, 100, 100); c.restore()
+    c.fill_rect(150, 40, 100, 100)
+
+
+def s_clip_1(c):                                        # :311-324
+    c.begin_path(); c.circle(100, 75, 50); c.clip()
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+    c.set_fill("orange"); c.fill_rect(0, 0, 100, 100)
+
+
+def s_clip_1b(c):                                       # :326-340
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+    c.begin_path(); c.circle(100, 75, 50); c.clip()
+    c.set_fill("red"); c.fill_rect(0, 0, 300, 150)
+    c.set_fill("orange"); c.fill_rect(0, 0, 100, 100)
+
+
+def s_clip_1c(c):                                       # :342-361
+    c.save(); c.begin_path(); c.circle(100, 75, 50); c.clip()
+    c.set_fill("red"); c.fill_rect(0, 0, 300, 150)
+    c.set_fill("orange"); c.fill_rect(0, 0, 100, 100)
+    c.restore(); c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+
+
+def s_clip_1d(c):                                       # :363-382
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+    c.begin_path(); c.circle(100, 75, 50); c.clip()
+    c.save_layer()
+    c.set_fill("red"); c.fill_rect(0, 0, 300, 150)
+    c.set_fill("orange"); c.fill_rect(0, 0, 100, 100)
+    c.restore()
+
+
+def s_clip_1e(c):                                       # :384-406
+    c.save(); c.begin_path(); c.circle(100, 75, 50); c.clip()
+    c.save_layer()
+    c.set_fill("red"); c.fill_rect(0, 0, 300, 150)
+    c.set_fill("orange"); c.fill_rect(0, 0, 100, 100)
+    c.restore(); c.restore()
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+
+
+def s_clip_2(c):                                        # :438-449, EvenOdd clip
+    r = Path(); r.rect(80, 10, 20, 130); r.rect(40, 50, 100, 50)
+    c.clip(r, EVENODD)
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+
+
+def s_clip_3(c):                                        # :451-467, clip of a clip
+    cp = Path(); cp.circle(150, 75, 75)
+    sp = Path(); sp.rect(85, 10, 130, 130)
+    c.clip(cp); c.clip(sp)
+    c.set_fill("blue"); c.fill_rect(0, 0, 300, 150)
+
+
+def s_set_line_dash_1(c):                               # :497-519
+    y = 15.0
+    for pat in [(), (1, 1), (10, 10), (20, 5), (15, 3, 3, 3), (20, 3, 3, 3, 3, 3, 3, 3),
+                (12, 3, 3)]:
+        c.begin_path(); c.line_dash = pat; c.move_to(0, y); c.line_to(300, y); c.stroke()
+        y += 20
+
+
+def s_blendmode_1(c):                                   # :521-537, Exclusion
+    c.fill_style = Paint((0, 0, 255, 255), EXCLUSION)
+    c.fill_rect(10, 10, 100, 100)
+
+
+def s_global_alpha_1(c):                                # :539-553, applyOpacity + layers
+    c.global_alpha = 0.5
+    c.set_fill("blue"); c.fill_rect(10, 10, 100, 100)
+    c.set_fill("red"); c.fill_rect(50, 50, 100, 100)
+
+
+def s_paint_save_restore(c):                            # :664-671 (paint copied by save)
+    c.fill_style = Paint((255, 0, 0, 255))
+    c.save(); c.fill_style.rgbx = (0, 0, 255, 255); c.restore()
+    c.fill_rect(0, 0, 100, 100)
+
+
+WHITE = (255, 255, 255, 255)
+SCENES = [
+    ("clearRect_1", s_clear_rect_1, (300, 160), None),
+    ("beginPath_1", s_begin_path_1, (300, 150), None),
+    ("moveTo_1", s_move_to_1, (300, 150), None),
+    ("fill_1", s_fill_1, (300, 150), None),
+    ("stroke_1", s_stroke_1, (300, 150), None),
+    ("stroke_2", s_stroke_2, (300, 150), None),
+    ("stroke_3", s_stroke_3, (300, 150), None),
+    ("closePath_1", s_close_path_1, (300, 150), None),
+    ("bezierCurveTo_1", s_bezier_1, (300, 150), None),
+    ("bezierCurveTo_2", s_bezier_2, (300, 150), None),
+    ("quadracticCurveTo_1", s_quad_1, (300, 150), None),
+    ("quadracticCurveTo_2", s_quad_2, (300, 150), None),
+    ("ellipse_1", s_ellipse_1, (300, 150), None),
+    ("strokeRect_1", s_stroke_rect_1, (300, 150), None),
+    ("strokeRect_2", s_stroke_rect_2, (300, 150), None),
+    ("setTransform_1", s_set_transform_1, (300, 150), None),
+    ("rotate_simple", s_rotate_simple, (300, 150), None),
+    ("resetTransform_2", s_reset_transform_2, (300, 150), None),
+    ("translate_1", s_translate_1, (300, 150), None),
+    ("scale_1", s_scale_1, (300, 150), None),
+    ("rotate_1", s_rotate_1, (300, 150), None),
+    ("save_1", s_save_1, (300, 150), None),
+    ("clip_1", s_clip_1, (300, 150), None),
+    ("clip_1b", s_clip_1b, (300, 150), None),
+    ("clip_1c", s_clip_1c, (300, 150), None),
+    ("clip_1d", s_clip_1d, (300, 150), None),
+    ("clip_1e", s_clip_1e, (300, 150), None),
+    ("clip_2", s_clip_2, (300, 150), None),
+    ("clip_3", s_clip_3, (300, 150), None),
+    ("setLineDash_1", s_set_line_dash_1, (300, 150), None),
+    ("blendmode_1", s_blendmode_1, (300, 150), WHITE),
+    ("globalAlpha_1", s_global_alpha_1, (300, 150), WHITE),
+    ("paintSaveRestore", s_paint_save_restore, (100, 100), None),
+]
+
+
+# resetTransform_1.png is left out: its block (:210-215) is the same code as
+# setTransform_1's, but the master shows a different scene (10,121 pixels
+# differ between the two masters), so it is stale.
+#
+# setTransform_1: pixel (6, 1) lies on the sheared edge x = 5y, where the
+# trapezoid sliver's exact area is 0.7 and 0.7 * 255 = 178.5.  The scalar
+# path rounds half away from zero (179); the master's SIMD build converts
+# with round-half-even (sse2.nim:6-11), giving 178.  That is the one pixel.
+ALPHA_OFF = {"setTransform_1": 1}
+
+
+@pytest.mark.parametrize("name,scene,size,bg", SCENES, ids=[s[0] for s in SCENES])
+def test_context_master(orc, masters, name, scene, size, bg):
+    c = _new(orc, size[0], size[1], bg)
+    scene(c)
+    _cmp(masters, name, c.image, ALPHA_OFF.get(name, 0))
+
+
+def test_fill_1_is_even_odd(orc, masters):
+    # the star's centre pentagon is covered twice: EvenOdd leaves it empty,
+    # NonZero (same path) fills it; the master has it empty
+    c = _new(orc)
+    s_fill_1(c)
+    m = masters("tests/contexts/fill_1.png")
+    assert tuple(m[80, 150]) == (0, 0, 0, 0) == tuple(c.image[80, 150])
+    assert tuple(c.image[60, 110]) == (0, 128, 0, 255)
+    nz = _new(orc)
+    r = Path()
+    r.move_to(30, 90); r.line_to(110, 20); r.line_to(240, 130); r.line_to(60, 130)
+    r.line_to(190, 20); r.line_to(270, 90); r.close_path()
+    nz.set_fill("green"); nz.fill(r, NONZERO)
+    assert tuple(nz.image[80, 150]) == (0, 128, 0, 255)
+
+
+def test_blendmode_1_exclusion_known_answer(orc, masters):
+    # Exclusion of opaque blue onto opaque white is yellow (255 + 0 - 0,
+    # 255 + 0 - 0, 255 + 255 - 2*255) on the non-AA rect
+    c = _new(orc, fill=WHITE)
+    s_blendmode_1(c)
+    assert tuple(c.image[50, 50]) == (255, 255, 0, 255)
+    assert tuple(c.image[5, 5]) == (255, 255, 255, 255)
