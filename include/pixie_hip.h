@@ -137,6 +137,13 @@ int pxh_scene_status(pxh_scene* scene, int32_t* status, int32_t n);
 int pxh_scene_debug_geometry(pxh_scene* scene, int32_t i, int32_t* info16, int32_t** parts4,
                              int32_t* nparts, float** entries4, int8_t** windings,
                              int32_t* nentries);
+/* Test access: lowers the run-pool capacity the next renders see to `cap`
+ * entries (never above the validated size), so that a test can force a
+ * replayed render to run out of the pool.  Such a render leaves its canvas
+ * untouched and its error surfaces at the lane's next synchronize
+ * (pxh_synchronize / pxh_lane_synchronize / pxh_scene_status), after which
+ * the scene re-validates on its next render. */
+int pxh_scene_debug_limit_runs(pxh_scene* scene, int32_t cap);
 
 /* Execution lanes (no reference counterpart: pixie is single-threaded).
  * The library owns pxh_lanes() independent device streams.  Lane 0 is the
@@ -160,6 +167,18 @@ pxh_image* pxh_image_view(void* device_ptr, int32_t width, int32_t height, int32
 int pxh_draw(pxh_image* a, const pxh_image* b, int32_t dx, int32_t dy, int32_t blend_mode);
 /* blur (images.nim:304-365); oob is the premultiplied out-of-bounds colour. */
 int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx);
+/* Allocation-free blur.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  Same result as pxh_blur, on lane 0, but
+ * the intermediate image and the Gaussian table live in a static device
+ * array the library holds on each GPU until the process ends, so it calls
+ * no hipMalloc / hipFree.  Limits: RGBA8 images of at most
+ * PXH_BLUR_NOALLOC_MAX_PIXELS pixels (w * h), radius as pxh_blur's LDS tiles
+ * allow; larger inputs fail with PXH_ERR_CAPACITY.  Every call on a GPU
+ * shares that GPU's array (each call overwrites all of it that it reads),
+ * so calls that overlap on one GPU (other threads or streams) race. */
+#define PXH_BLUR_NOALLOC_MAX_PIXELS 4194304
+int pxh_blur_noalloc(pxh_image* img, float radius, uint32_t oob_rgbx);
 /* spread (images.nim:700-758). */
 int pxh_spread(pxh_image* img, float spread);
 /* shadow (images.nim:760-776) into `out` (same size as src); integer offsets. */

@@ -110,6 +110,7 @@ def lib():
         L.pxh_scene_status.argtypes = [vp, i32p, i32]
         L.pxh_scene_debug_geometry.argtypes = [vp, i32, i32p, P(i32p), i32p, P(f32p),
                                                P(P(ctypes.c_int8)), i32p]
+        L.pxh_scene_debug_limit_runs.argtypes = [vp, i32]
         _lib = L
     return _lib
 
@@ -352,6 +353,10 @@ class Scene:
         _check(lib().pxh_scene_status(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                       n))
         return out[:n]
+
+    def debug_limit_runs(self, cap):
+        """Test access: lowers the run-pool capacity later renders see."""
+        _check(lib().pxh_scene_debug_limit_runs(self._h, cap))
 
     def debug_geometry(self, i):
         """Device geometry of call i: (info[16], parts[n,4], entries[m,4], windings[m])."""

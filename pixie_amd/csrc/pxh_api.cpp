@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <map>
+#include <set>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -100,8 +101,20 @@ struct DevBuf {
   }
 };
 
+// Static device scratch of pxh_blur_noalloc: the X pass's intermediate image
+// (at most kNoAllocBlurPixels RGBA8 pixels, 16 MiB), then the Gaussian table.
+// Each GPU holds its own copy for the life of the process.  Host code never
+// uses the name: it uses the address hipGetSymbolAddress gives for the device
+// (Ctx::scratchPtr), taken once per device at init.
+constexpr size_t kNoAllocBlurPixels = (size_t)1 << 22;
+constexpr size_t kNoAllocLutWords = kMaxLutTaps;
+static __device__ __align__(256) uint32_t scratch[kNoAllocBlurPixels + kNoAllocLutWords];
+constexpr int kMaxDevices = 16;
+
 struct Ctx {
   bool inited = false;
+  int device = 0;                          // the device the library runs on
+  uint32_t* scratchPtr[kMaxDevices] = {};  // device address of `scratch`, per device
   hipStream_t stream = nullptr;
   size_t maxLds = 65536;
   int numCUs = 256;
@@ -124,6 +137,11 @@ struct Ctx {
   DevBuf plane;      // alpha plane
   DevBuf blendBuf;   // pxh_blend_pixels
   int32_t* errFlag = nullptr;
+  // Per lane: error bits of scene renders that stopped before compose (an
+  // overflowed pool); read and cleared when the lane is synchronized.  A
+  // second row of kLanes words takes the reports of checked renders.
+  int32_t* laneErr = nullptr;
+  std::set<pxh_scene*> scenes;  // live scenes (their pools are re-validated after an error)
   unsigned long long* touched = nullptr;  // N_touch counter of fills
   hipEvent_t ev[16] = {};
   // per-launch kernel timing (pxh_profile_begin / _end)
@@ -144,6 +162,12 @@ int init() {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(kErrDevice, "no HIP device");
   PXH_HIP(hipSetDevice(0));
+  g.device = 0;
+  // the allocation-free blur's static scratch, addressed through the symbol's
+  // device address on this device (the name itself is only a host stand-in)
+  if (hipGetSymbolAddress((void**)&g.scratchPtr[g.device], HIP_SYMBOL(scratch)) != hipSuccess ||
+      !g.scratchPtr[g.device])
+    return fail(kErrDevice, "hipGetSymbolAddress(scratch) failed");
   int lds = 0, cus = 0;
   PXH_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, 0));
   PXH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
@@ -156,6 +180,9 @@ int init() {
   PXH_HIP(hipEventRecord(g.stageFree, g.stream));
   PXH_HIP(hipMalloc((void**)&g.errFlag, sizeof(int32_t)));
   PXH_HIP(hipMemsetAsync(g.errFlag, 0, sizeof(int32_t), g.stream));
+  // kLanes words read at synchronize, then kLanes that checked renders write
+  PXH_HIP(hipMalloc((void**)&g.laneErr, 2 * kLanes * sizeof(int32_t)));
+  PXH_HIP(hipMemsetAsync(g.laneErr, 0, 2 * kLanes * sizeof(int32_t), g.stream));
   PXH_HIP(hipMalloc((void**)&g.touched, sizeof(unsigned long long)));
   PXH_HIP(hipMemsetAsync(g.touched, 0, sizeof(unsigned long long), g.stream));
   for (auto& e : g.ev) PXH_HIP(hipEventCreate(&e));
@@ -315,9 +342,11 @@ const float kIdentity[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
 // gaussianKernel (internal.nim:17-34): s and the exp() argument in float32,
 // the 1/sqrt(2*PI*s^2) factor in float64 (2*PI is a float64 constant), taps
 // summed in float64, stored to float32, normalised in float64.
-void gaussian_lut(int radius, std::vector<uint32_t>* out) {
+// Writes the 2 * radius + 1 taps to out; f is caller-provided float storage
+// of the same length.
+template <class T>
+void gaussian_lut_into(int radius, float* f, T* out) {
   int n = 2 * radius + 1;
-  std::vector<float> f(n);
   double total = 0.0;
   for (int step = -radius; step <= radius; step++) {
     float s = (float)radius / 2.2f;
@@ -328,11 +357,16 @@ void gaussian_lut(int radius, std::vector<uint32_t>* out) {
     f[step + radius] = (float)a;
     total += a;
   }
-  out->resize(n);
   for (int i = 0; i < n; i++) {
     float v = (float)((double)f[i] / total);
-    (*out)[i] = (uint16_t)roundf(v * 255.0f * 256.0f);
+    out[i] = (uint16_t)roundf(v * 255.0f * 256.0f);
   }
+}
+
+void gaussian_lut(int radius, std::vector<uint32_t>* out) {
+  std::vector<float> f(2 * radius + 1);
+  out->resize(2 * radius + 1);
+  gaussian_lut_into(radius, f.data(), out->data());
 }
 
 int get_lut(int radius, uint32_t** dev) {
@@ -674,22 +708,29 @@ int scene_upload(pxh_scene* sc, int lane) {
     PXH_HIP(hipMemcpyAsync(sc->dOrder, sc->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
   }
   sc->lane = lane;
-  // first estimates of the pools; a render that overflows grows them
+  // first estimates of the pools; a render that overflows grows them.  The
+  // sizes a scene has grown to are kept across clear() and re-recording, so
+  // a scene reused for similar requests validates once, not per request.
   size_t nC = sc->cmds.size();
-  sc->ptCap = (int32_t)std::min<size_t>(64 * nC + 4096, 1u << 28);
-  sc->segCap = sc->ptCap * 2;
-  sc->bandCap = (int32_t)std::min<size_t>(64 * sc->items.size() + 4096, 1u << 26);
-  sc->entryCap = sc->segCap * 2;
-  sc->taskCap = (int32_t)std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28);
-  sc->runCap = (int32_t)std::min<int64_t>(
+  auto keep = [](int32_t prev, int64_t est) { return (int32_t)std::max<int64_t>(prev, est); };
+  sc->ptCap = keep(sc->ptCap, std::min<size_t>(64 * nC + 4096, 1u << 28));
+  sc->segCap = keep(sc->segCap, (int64_t)sc->ptCap * 2);
+  sc->bandCap = keep(sc->bandCap, std::min<size_t>(64 * sc->items.size() + 4096, 1u << 26));
+  sc->entryCap = keep(sc->entryCap, (int64_t)sc->segCap * 2);
+  sc->taskCap = keep(sc->taskCap, std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28));
+  sc->runCap = keep(sc->runCap, std::min<int64_t>(
       (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kMaxRowWavesPerCU) * kRunChunk,
-      (int64_t)1 << 30);
-  sc->binCap = (int32_t)std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28);
+      (int64_t)1 << 30));
+  sc->binCap = keep(sc->binCap, std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28));
   sc->uploaded = true;
   return scene_alloc(sc);
 }
 
-int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
+// Queues every kernel of one render.  `checked` renders are those whose
+// counters the host reads itself (pool validation, autotune timing): their
+// compose kernel reports an early stop into a word nobody reads, so that
+// only unchecked renders (replays) raise the lane's error.
+int scene_launch(pxh_scene* sc, pxh_image* img, int lane, bool checked = false) {
   ScenePools& p = sc->pools;
   hipStream_t st = g.lane[lane];
   sc->lane = lane;
@@ -726,7 +767,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane) {
   if (!r) {
     LaunchTimer t("scene_compose_kernel", st);
     r = scene_launch_compose(p, img->dev, img->fmt == PXH_FORMAT_A8, g.touched,
-                             scene_queue_grid(g.numCUs * 24),
+                             g.laneErr + (checked ? kLanes : 0) + lane, scene_queue_grid(g.numCUs * 24),
                              st);
   }
   if (r) return fail(kErrDevice, std::string("scene launch: ") + hipGetErrorString(hipGetLastError()));
@@ -768,6 +809,23 @@ int scene_replay(pxh_scene* sc, pxh_image* img, int lane) {
   return 0;
 }
 
+// Reads and clears the scene error word of a synchronized lane.  A render
+// replayed from a graph is not checked when it is issued; if one ran out of
+// a pool, its compose kernel left the canvas untouched and set this word.
+// The scenes of the lane are then re-validated (and their pools grown) on
+// their next render.
+int lane_scene_errors(int lane) {
+  int32_t e = 0;
+  PXH_HIP(hipMemcpy(&e, g.laneErr + lane, sizeof(e), hipMemcpyDeviceToHost));
+  if (!e) return 0;
+  PXH_HIP(hipMemset(g.laneErr + lane, 0, sizeof(int32_t)));
+  for (pxh_scene* sc : g.scenes)
+    if (sc->lane == lane) sc->validW = sc->validH = -1;
+  return fail(kErrCapacity, "a scene render on lane " + std::to_string(lane) +
+                                " ran out of a pool (error bits " + std::to_string(e) +
+                                ") and left its canvas untouched; render it again");
+}
+
 }  // namespace
 }  // namespace pxh
 
@@ -784,6 +842,7 @@ const char* pxh_version(void) { return "pixie_hip 0.1 (gfx950)"; }
 int pxh_synchronize(void) {
   PXH_INIT();
   PXH_HIP(hipStreamSynchronize(g.stream));
+  if (int r = lane_scene_errors(0)) return r;
   int32_t e = 0;
   PXH_HIP(hipMemcpy(&e, g.errFlag, sizeof(e), hipMemcpyDeviceToHost));
   if (e) {
@@ -982,6 +1041,37 @@ int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx) {
   return blur_impl(p, p, img->w, img->h, (int)r, oob_rgbx, false, 0);
 }
 
+// blur with no device allocation: the intermediate image and the table live
+// in the static scratch; one kernel stores the table, two run the passes, all
+// on lane 0.  Calls that overlap on one GPU (other host threads, other
+// streams) share the scratch and race.
+int pxh_blur_noalloc(pxh_image* img, float radius, uint32_t oob_rgbx) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "blur needs an RGBA8 image");
+  int64_t r = nim_f2i(roundf(radius));  // images.nim:308
+  if (r == 0) return 0;
+  if (r < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
+  if ((size_t)img->w * (size_t)img->h > kNoAllocBlurPixels)
+    return fail(kErrCapacity, "image larger than the allocation-free blur's scratch (4194304 pixels)");
+  if (2 * r + 1 > (int64_t)kNoAllocLutWords || blur_x_lds((int)r) > g.maxLds || blur_y_lds((int)r) > g.maxLds)
+    return fail(kErrCapacity, "blur radius too large for the LDS tiles");
+  uint32_t* base = g.scratchPtr[g.device];
+  if (!base) return fail(kErrDevice, "allocation-free scratch is not set up on this device");
+  LutTaps taps;
+  float f[kMaxLutTaps];
+  taps.n = (int32_t)(2 * r + 1);
+  gaussian_lut_into((int)r, f, taps.v);
+  uint32_t* tmp = base;
+  uint32_t* lut = base + kNoAllocBlurPixels;
+  if (launch_lut_store(taps, lut, g.stream) != 0) return fail(kErrDevice, "table store launch failed");
+  uint32_t* p = (uint32_t*)img->dev;
+  BlurArgs ax{p, tmp, lut, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0};
+  BlurArgs ay{tmp, p, lut, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0};
+  if (launch_blur(ax, ay, g.stream) != 0) return fail(kErrDevice, "blur launch failed");
+  return 0;
+}
+
 int pxh_blur_on(pxh_image* img, float radius, uint32_t oob_rgbx, int32_t lane) {
   PXH_INIT();
   if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
@@ -1001,7 +1091,7 @@ int pxh_lane_synchronize(int32_t lane) {
   PXH_INIT();
   if (lane < 0 || lane >= kLanes) return fail(kErrArg, "invalid lane");
   PXH_HIP(hipStreamSynchronize(g.lane[lane]));
-  return 0;
+  return lane_scene_errors(lane);
 }
 
 int pxh_image_fill_on(pxh_image* img, uint32_t rgbx, int32_t lane) {
@@ -1109,10 +1199,15 @@ int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_
   return 0;
 }
 
-pxh_scene* pxh_scene_new(void) { return new pxh_scene(); }
+pxh_scene* pxh_scene_new(void) {
+  pxh_scene* sc = new pxh_scene();
+  g.scenes.insert(sc);
+  return sc;
+}
 
 void pxh_scene_free(pxh_scene* sc) {
   if (!sc) return;
+  g.scenes.erase(sc);
   if (g.inited) (void)hipStreamSynchronize(g.lane[sc->lane]);
   scene_free_device(sc);
   delete sc;
@@ -1168,14 +1263,14 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
   // First render at this size: check the pools and grow them until the render
   // fits.  A render that overflows a pool stops before touching the canvas.
   for (int attempt = 0; attempt < 8; attempt++) {
-    if (int r = scene_launch(sc, img, lane)) return r;
+    if (int r = scene_launch(sc, img, lane, true)) return r;
     int32_t cnt[kNumCounters];
     PXH_HIP(hipMemcpyAsync(cnt, sc->pools.counters, sizeof(cnt), hipMemcpyDeviceToHost, st));
     PXH_HIP(hipStreamSynchronize(st));
-    // the run pool must also hold every wave's partly used chunk, whatever
-    // order the waves take tasks in on later renders
+    // the run pool must hold the chunks of any later render of the same
+    // scene, whatever order its waves take tasks in (scene_run_bound)
     const int64_t runNeed =
-        (int64_t)cnt[kCntRunsUsed] + (int64_t)scene_queue_grid(g.numCUs * kMaxRowWavesPerCU) * kRunChunk;
+        scene_run_bound(cnt[kCntRunsUsed], scene_queue_grid(g.numCUs * kMaxRowWavesPerCU));
     if (!cnt[kCntError]) {  // this render is complete; later ones get the slack
       if (runNeed > sc->runCap) {
         sc->runCap = (int32_t)std::min<int64_t>(runNeed + runNeed / 4, (int64_t)1 << 30);
@@ -1222,15 +1317,18 @@ int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* sha
   if (reps < 1) reps = 1;
   pxh_image* tmp = img->fmt == PXH_FORMAT_A8 ? pxh_image_new_a8(img->w, img->h) : pxh_image_new(img->w, img->h);
   if (!tmp) return kErrNoMem;
-  int best = sc->rowShape;
+  int best = -1;
   float bestMs = 0;
   for (int k = 0; k < kNumRowShapes; k++) {
     sc->rowShape = k;
-    int rc = pxh_scene_render_on(sc, tmp, 0);  // also sizes the pools for this shape
+    // validate this shape's first render (it sizes the pools and checks
+    // every counter), then time eager renders
+    sc->validW = sc->validH = -1;
+    int rc = pxh_scene_render_on(sc, tmp, 0);
     float total = 0;
     for (int r = 0; rc == 0 && r < reps; r++) {
       if (hipEventRecord(g.ev[14], g.stream) != hipSuccess) rc = kErrDevice;
-      if (!rc) rc = scene_launch(sc, tmp, 0);
+      if (!rc) rc = scene_launch(sc, tmp, 0, true);
       if (!rc && hipEventRecord(g.ev[15], g.stream) != hipSuccess) rc = kErrDevice;
       float ms = 0;
       if (!rc && (hipEventSynchronize(g.ev[15]) != hipSuccess ||
@@ -1238,16 +1336,30 @@ int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* sha
         rc = kErrDevice;
       total += ms;
     }
+    // a timed render that stopped early (an overflowed pool) did less work:
+    // such a shape is not a candidate
+    int32_t err = 0;
+    if (!rc && hipMemcpy(&err, sc->pools.counters + kCntError, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess)
+      rc = kErrDevice;
+    if (rc == kErrCapacity || err) continue;
     if (rc) {
       pxh_image_free(tmp);
-      sc->rowShape = best;
+      sc->rowShape = best < 0 ? kDefaultRowShape : best;
       return rc;
     }
-    if (k == 0 || total / reps < bestMs) {
+    if (best < 0 || total / reps < bestMs) {
       bestMs = total / reps;
       best = k;
     }
   }
+  if (best < 0) {
+    pxh_image_free(tmp);
+    sc->rowShape = kDefaultRowShape;
+    sc->validW = sc->validH = -1;
+    return fail(kErrCapacity, "no phase-A launch shape rendered this scene");
+  }
+  // the pools were last validated for the last shape tried; the chosen
+  // shape's renders fit them too (pool bounds do not depend on the shape)
   pxh_image_free(tmp);
   sc->rowShape = best;
   if (shape_out) *shape_out = best;
@@ -1266,6 +1378,15 @@ int pxh_scene_status(pxh_scene* sc, int32_t* status, int32_t n) {
   PXH_HIP(hipMemcpyAsync(geo.data(), sc->pools.geo, m * sizeof(ItemGeom), hipMemcpyDeviceToHost, st));
   PXH_HIP(hipStreamSynchronize(st));
   for (int32_t i = 0; i < m; i++) status[i] = geo[i].status;
+  return lane_scene_errors(sc->lane);
+}
+
+int pxh_scene_debug_limit_runs(pxh_scene* sc, int32_t cap) {
+  PXH_INIT();
+  if (!sc || !sc->uploaded) return fail(kErrArg, "scene not rendered");
+  if (cap < 0 || cap > sc->runCap) return fail(kErrArg, "run cap outside the validated pool");
+  sc->pools.runCap = cap;
+  sc->gen++;  // renders re-capture their graph with the lowered capacity
   return 0;
 }
 

@@ -43,6 +43,13 @@ struct BlurArgs {
   uint32_t color;
 };
 
+// Gaussian taps passed by value (kernel arguments stay under 4 KB).
+constexpr int kMaxLutTaps = 1024;
+struct LutTaps {
+  uint16_t v[kMaxLutTaps];
+  int32_t n;
+};
+
 struct SpreadArgs {
   const uint32_t* src;
   uint8_t* plane;        // alpha plane scratch (w*h)
@@ -56,6 +63,7 @@ int set_fill_lds_limit(size_t bytes);
 int launch_composite(const CompositeArgs& a, bool srcA8, hipStream_t s);
 int launch_blur(const BlurArgs& ax, const BlurArgs& ay, hipStream_t s);
 int set_blur_lds_limit(size_t bytes);
+int launch_lut_store(const LutTaps& t, uint32_t* dst, hipStream_t s);
 size_t blur_x_lds(int r);
 size_t blur_y_lds(int r);
 int launch_spread(const SpreadArgs& a, hipStream_t s);

@@ -231,6 +231,17 @@ __global__ void __launch_bounds__(256) blur_y_kernel(BlurArgs a) {
   }
 }
 
+// Writes a Gaussian table passed by value into device memory (the
+// allocation-free blur keeps its table in static device scratch).
+__global__ void __launch_bounds__(256) lut_store_kernel(LutTaps t, uint32_t* dst) {
+  for (int i = threadIdx.x; i < t.n; i += 256) dst[i] = t.v[i];
+}
+
+int launch_lut_store(const LutTaps& t, uint32_t* dst, hipStream_t s) {
+  hipLaunchKernelGGL(lut_store_kernel, dim3(1), dim3(256), 0, s, t, dst);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
 size_t blur_x_lds(int r) { return (size_t)(256 + 2 * r) * 4; }
 size_t blur_y_lds(int r) { return (size_t)64 * (kBlurTileRows + 2 * r) * 4; }
 

@@ -75,11 +75,18 @@ struct RowRuns {
 constexpr int32_t kRunArea = 1 << 30;
 constexpr int32_t kRunXMask = (1 << 28) - 1;
 
-// Phase-A waves take run-pool space in chunks of kRunChunk entries (more
-// than one task of a 512-px window can write: 513 deltas + 512 area
-// factors).  A pool of (runs written) + (waves x kRunChunk) entries is
-// always enough, whatever order the waves take tasks in.
+// Phase-A waves take run-pool space in chunks of kRunChunk entries.  One
+// task of a 512-px window writes at most kMaxTaskRuns entries (513 deltas +
+// 512 area factors), and a wave leaves a chunk only when the next task does
+// not fit, so every chunk a wave leaves holds more than kRunChunk -
+// kMaxTaskRuns written entries.  Whatever order the waves take tasks in, a
+// render of U written entries takes at most U / (kRunChunk - kMaxTaskRuns)
+// left chunks plus one open chunk per wave (scene_run_bound).
 constexpr int kRunChunk = 2048;
+constexpr int kMaxTaskRuns = 1025;
+inline int64_t scene_run_bound(int64_t used, int64_t waves) {
+  return (used / (kRunChunk - kMaxTaskRuns) + 1 + waves) * kRunChunk;
+}
 
 // segBand flag: the segment needs anti-aliasing (requiresAntiAliasing).
 constexpr int32_t kSegAA = 1 << 30;
@@ -195,6 +202,6 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
-                         int grid, hipStream_t s);
+                         int32_t* errOut, int grid, hipStream_t s);
 
 }  // namespace pxh

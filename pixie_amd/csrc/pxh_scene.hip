@@ -1393,7 +1393,8 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
 // stays small enough for the instruction cache.
 template <bool kA8>
 __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* canvas,
-                                                          unsigned long long* touchedOut) {
+                                                          unsigned long long* touchedOut,
+                                                          int32_t* errOut) {
   __shared__ __align__(16) int32_t D[kWinB + 4];
   __shared__ uint32_t AX[kWinB];
   __shared__ uint32_t WIN[kWinB];
@@ -1405,7 +1406,12 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
   const int nUnits = H * winsPerRow;
   const bool vec = ((W & 3) == 0);
   uint32_t ntouch = 0;
-  if (P.counters[kCntError]) return;
+  if (const int32_t err = P.counters[kCntError]) {
+    // an earlier stage ran out of a pool: the canvas stays untouched and the
+    // error reaches the host through the lane's error word
+    if (blockIdx.x == 0 && lane == 0) atomicOr(errOut, err);
+    return;
+  }
   for (;;) {
     const int unit = queue_next(P.queues + kQueues * kQueueStride, nUnits);
     if (unit >= nUnits) break;
@@ -1694,11 +1700,13 @@ int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s) {
 }
 
 int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
-                         int grid, hipStream_t s) {
+                         int32_t* errOut, int grid, hipStream_t s) {
   if (a8)
-    hipLaunchKernelGGL(scene_compose_kernel<true>, dim3(grid), dim3(64), 0, s, p, canvas, touched);
+    hipLaunchKernelGGL(scene_compose_kernel<true>, dim3(grid), dim3(64), 0, s, p, canvas, touched,
+                       errOut);
   else
-    hipLaunchKernelGGL(scene_compose_kernel<false>, dim3(grid), dim3(64), 0, s, p, canvas, touched);
+    hipLaunchKernelGGL(scene_compose_kernel<false>, dim3(grid), dim3(64), 0, s, p, canvas, touched,
+                       errOut);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 

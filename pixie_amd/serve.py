@@ -29,7 +29,7 @@ fillPath / strokePath launch per call.
 import threading
 import time
 
-from . import Image, Scene, lane_synchronize, lanes
+from . import Image, PixieError, Scene, lane_synchronize, lanes
 
 
 class Request:
@@ -43,12 +43,16 @@ class Request:
 
 
 class Result:
-    __slots__ = ("index", "admitted", "done", "latency", "pixels")
+    """Outcome of one request.  `error` is None, or the PixieError the
+    request raised (for example a capacity error): a failing request is
+    answered with its error and the others are served as usual."""
+    __slots__ = ("index", "admitted", "done", "latency", "pixels", "error")
 
     def __init__(self, index):
         self.index = index
         self.admitted = self.done = self.latency = 0.0
         self.pixels = None
+        self.error = None
 
 
 class _Arena:
@@ -150,17 +154,28 @@ class Server:
                     res = results[i]
                     res.admitted = time.perf_counter() - t0
                     img = Image.view(self.arena.base + off, r.side, r.side)
-                    img.fill((0, 0, 0, 0), lane=lane)
-                    scene.clear()
-                    _record(scene, r.calls)
-                    scene.render(img, lane=lane)
-                    if r.blur:
-                        img.blur(r.blur, lane=lane)
-                    lane_synchronize(lane)
-                    res.done = time.perf_counter() - t0
+                    try:
+                        img.fill((0, 0, 0, 0), lane=lane)
+                        scene.clear()
+                        _record(scene, r.calls)
+                        scene.render(img, lane=lane)
+                        if r.blur:
+                            img.blur(r.blur, lane=lane)
+                        lane_synchronize(lane)
+                        res.done = time.perf_counter() - t0
+                        if self.keep_pixels:
+                            res.pixels = img.download()
+                    except PixieError as e:
+                        # this request fails alone: its lane is drained so
+                        # that nothing of it is left on the device, and the
+                        # lane goes on with the next request
+                        res.error = e
+                        try:
+                            lane_synchronize(lane)
+                        except PixieError:
+                            pass
+                        res.done = time.perf_counter() - t0
                     res.latency = res.done - r.arrival
-                    if self.keep_pixels:
-                        res.pixels = img.download()
                     del img
                     with cv:
                         self.arena.release(off, nbytes)
@@ -192,27 +207,32 @@ def serve_plain(requests, keep_pixels=False):
         res = results[i]
         res.admitted = time.perf_counter() - t0
         img = Image(r.side, r.side)
-        for c in r.calls:
-            if c.stroke:
-                img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
-                               miterLimit=c.miter, blendMode=c.mode)
-            else:
-                img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
-        if r.blur:
-            img.blur(r.blur)
-        lane_synchronize(0)
+        try:
+            for c in r.calls:
+                if c.stroke:
+                    img.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap,
+                                   lineJoin=c.join, miterLimit=c.miter, blendMode=c.mode)
+                else:
+                    img.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+            if r.blur:
+                img.blur(r.blur)
+            lane_synchronize(0)
+        except PixieError as e:
+            res.error = e
         res.done = time.perf_counter() - t0
         res.latency = res.done - r.arrival
-        if keep_pixels:
+        if keep_pixels and res.error is None:
             res.pixels = img.download()
     return results
 
 
 def latency_stats(results, requests):
     import numpy as np
-    lat = np.array([r.latency for r in results]) * 1e3
+    ok = [r for r in results if r.error is None]
+    lat = np.array([r.latency for r in ok]) * 1e3
     span = max(r.done for r in results) - min(q.arrival for q in requests)
-    return {"scenes": len(results), "scenes_per_s": round(len(results) / span, 1),
+    return {"scenes": len(ok), "failed": len(results) - len(ok),
+            "scenes_per_s": round(len(ok) / span, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 3),
             "p99_ms": round(float(np.percentile(lat, 99)), 3),
             "max_ms": round(float(lat.max()), 3)}

@@ -372,6 +372,76 @@ def test_scene_repeated_renders_are_identical(gpu):
     _exact(img.download(), ref2.download(), "two renders")
 
 
+def _comb(x0, y0, teeth, pitch, height, slant):
+    """A comb of slanted teeth: every row of it crosses 2 * teeth sloped
+    edges, so its row windows write hundreds of area and delta runs."""
+    p = Path()
+    p.move_to(x0, y0 + height)
+    for i in range(teeth):
+        x = x0 + i * pitch
+        p.line_to(x + slant, y0)
+        p.line_to(x + slant + pitch * 0.55, y0)
+        p.line_to(x + pitch * 0.55, y0 + height)
+    p.close_path()
+    return p.array()
+
+
+def test_scene_heavy_runs_replayed(gpu):
+    # row windows that each write ~1000 runs (the most one task can write),
+    # replayed from the captured graph several times: phase-A waves take
+    # tasks in a new order on every render, and every render must fit the
+    # run pool sized by the first one and give the per-call image
+    W, H = 1536, 384
+    calls = []
+    rng = np.random.default_rng(5)
+    for k in range(24):
+        y0 = float(rng.uniform(0, H - 140))
+        calls.append(_comb(float(rng.uniform(0, 40)), y0, 240, 6.03, 120.0, 2.37))
+    ref = px.Image(W, H)
+    sc = px.Scene()
+    for i, c in enumerate(calls):
+        col = (30 + 9 * i, 200 - 5 * i, 120, 255)
+        ref.fillPath(c, col, blendMode=i % 16)
+        sc.fillPath(c, col, blendMode=i % 16)
+    want = ref.download()
+    for k in range(6):
+        img = px.Image(W, H)
+        sc.render(img)
+        px.synchronize()
+        _exact(img.download(), want, "heavy-run render %d" % k)
+    assert sc.graph_active()
+
+
+def test_scene_replay_overflow_is_reported(gpu):
+    # a replayed render that runs out of its run pool (forced from the host by
+    # lowering the capacity its kernels see) stops before compose: the canvas
+    # is untouched, the next synchronize raises a capacity error, and the
+    # next render re-validates the pools and draws the same image as before
+    cmds, canvas = WL.mixed_scene(seed=9, n_paths=40, W=256, H=256)
+    sc = px.Scene()
+    for c in cmds:
+        if c.stroke:
+            sc.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                          miterLimit=c.miter, blendMode=c.mode)
+        else:
+            sc.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    img = px.Image.from_array(canvas)
+    sc.render(img)
+    want = img.download()
+    sc.debug_limit_runs(16)
+    img2 = px.Image.from_array(canvas)
+    sc.render(img2)
+    with pytest.raises(px.PixieError) as e:
+        px.synchronize()
+    assert e.value.code == 9
+    _exact(img2.download(), canvas, "overflowed render leaves the canvas")
+    px.synchronize()  # the error word was cleared
+    img3 = px.Image.from_array(canvas)
+    sc.render(img3)
+    px.synchronize()
+    _exact(img3.download(), want, "render after re-validation")
+
+
 def test_scene_glyph_atlas(gpu, orc):
     # the glyph-atlas workload at small scale: every glyph filled NonZero into
     # its own cell of one 8-bit atlas, in one batched render
