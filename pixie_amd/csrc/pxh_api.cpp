@@ -262,6 +262,50 @@ int next_pow2(int v) {
   return p;
 }
 
+// Largest number of band entries any row of the fill selects (entries with
+// to.y > y and at.y < y + 1, the row program's selection), by a difference
+// array over each band's rows.
+int max_row_selection(const FillGeom& geo) {
+  const FillInfo& f = geo.info;
+  int best = 0;
+  std::vector<int32_t> d;
+  for (int32_t i = 0; i < f.nParts; i++) {
+    const PartInfo& P = geo.parts[i];
+    const int32_t top = f.startY + i * f.bandH;
+    const int32_t bot = i == f.nParts - 1 ? f.pathHeight : top + f.bandH;
+    if (bot <= top) continue;
+    if (P.flags & 2) {
+      best = std::max(best, 2);
+      continue;
+    }
+    d.assign((size_t)(bot - top) + 1, 0);
+    for (int32_t k = 0; k < P.count; k++) {
+      const Entry& e = geo.entries[(size_t)P.entryBase + k];
+      // rows y in [top, bot) with !(e.toy <= y || e.aty >= y + 1), the row
+      // program's test (a NaN end selects every row, as there)
+      int64_t y0 = top, y1 = bot;
+      if (e.aty == e.aty && e.aty > (float)top - 2.0f) {
+        const float a = std::min(e.aty, (float)bot + 2.0f);
+        y0 = (int64_t)floor((double)a) - 1;
+        while ((float)(y0 + 1) <= a) y0++;  // first y with a < y + 1
+      }
+      if (e.toy == e.toy && e.toy < (float)bot + 2.0f) {
+        const float t = std::max(e.toy, (float)top - 2.0f);
+        y1 = (int64_t)ceil((double)t) + 1;
+        while (y1 > y0 && !(t > (float)(y1 - 1))) y1--;  // one past the last y with t > y
+      }
+      y0 = std::max<int64_t>(y0, top);
+      y1 = std::min<int64_t>(y1, bot);
+      if (y1 <= y0) continue;
+      d[(size_t)(y0 - top)]++;
+      d[(size_t)(y1 - top)]--;
+    }
+    int run = 0;
+    for (int32_t y = 0; y < bot - top; y++) best = std::max(best, run += d[(size_t)y]);
+  }
+  return best;
+}
+
 // Launches the raster of one built fill onto img.
 int raster_fill(pxh_image* img, FillGeom& geo) {
   FillInfo& f = geo.info;
@@ -283,8 +327,11 @@ int raster_fill(pxh_image* img, FillGeom& geo) {
                  {geo.windings.data(), geo.windings.size()}},
                 &dp);
   if (r) return r;
-  const int ww = 1024;
-  const int cap = next_pow2(geo.maxCount > 2 ? geo.maxCount : 2);
+  // The row arrays hold one row's selection (the band entries that meet
+  // [y, y + 1)), not the whole band: size them for the largest selection.
+  const int maxSel = max_row_selection(geo);
+  const int ww = 512;
+  const int cap = next_pow2(maxSel > 2 ? maxSel : 2);
   const size_t fixedBytes = 8 * (size_t)ww + 16;
   const size_t capBytes = (size_t)cap * 24;
   FillArgs a;
@@ -299,7 +346,11 @@ int raster_fill(pxh_image* img, FillGeom& geo) {
   a.ww = ww;
   a.err = g.errFlag;
   a.touched = g.touched;
-  int grid = nRows < 8192 ? nRows : 8192;
+  const int32_t lo = mask ? 0 : f.xlo, hi = mask ? img->w : f.xhi;
+  const int64_t nWin = hi > (lo & ~3) ? (hi - (lo & ~3) + ww - 1) / ww : 0;
+  const int64_t nUnits = (int64_t)nRows * nWin;
+  if (nUnits == 0) return 0;
+  int grid = (int)std::min<int64_t>(nUnits, (int64_t)g.numCUs * 32);
   size_t lds = fixedBytes;
   if (fixedBytes + capBytes <= g.maxLds) {
     a.capInLds = 1;

@@ -1,7 +1,7 @@
 // pxh_kernels.hip -- HIP kernels for gfx950 (MI355X) and their launchers.
 //
 //   fill_rows_kernel   one fillShapes call (paths.nim:1593-1912): a wave per
-//                      canvas row runs the row program and the fused
+//                      (canvas row, column window) runs the row program and the fused
 //                      coverage -> paint -> blend -> write-back epilogue
 //   composite_kernel   blendRect (images.nim:468-529), all 20 blend modes
 //   blur_x / blur_y    blur (images.nim:304-365), LDS-staged halo tiles,
@@ -18,7 +18,8 @@
 namespace pxh {
 
 // ---------------------------------------------------------------------------
-// fill: one wave (64 threads) per workgroup, grid-stride over rows
+// fill: one wave (64 threads) per workgroup, grid-stride over (row, window)
+// units
 // ---------------------------------------------------------------------------
 template <bool kA8>
 __global__ void __launch_bounds__(64) fill_rows_kernel(FillArgs a) {
@@ -32,31 +33,34 @@ __global__ void __launch_bounds__(64) fill_rows_kernel(FillArgs a) {
   const bool mask = F.mode == kMask;
   const int32_t W = F.canvasW;
   const int32_t lo = mask ? 0 : F.xlo, hi = mask ? W : F.xhi;
+  const int32_t x0 = lo & ~3;
+  const int32_t nWin = hi > x0 ? (hi - x0 + a.ww - 1) / a.ww : 0;
+  const int64_t nUnits = (int64_t)a.nRows * nWin;
   uint32_t ntouch = 0;
-  for (int32_t r = blockIdx.x; r < a.nRows; r += gridDim.x) {
-    const int32_t y = a.rowBase + r;
-    for (int32_t wx = lo & ~3; wx < hi; wx += a.ww) {
-      RowCtx C;
-      C.F = a.F;
-      C.parts = a.parts;
-      C.E = a.E;
-      C.EW = a.EW;
-      C.y = y;
-      C.wx0 = wx;
-      C.wx1 = wx + a.ww < hi ? wx + a.ww : hi;
-      C.cx0 = lo;
-      C.cx1 = hi;
-      int32_t kind = row_program(C, L);
-      if (kind < 0) {
-        if (threadIdx.x == 0) atomicOr(a.err, 1);
-        break;
-      }
-      if (kind == kRowNone) break;
-      if (kA8)
-        ntouch += row_epilogue_a8(C, L, kind, (uint8_t*)a.canvas, W, F.rgbx, F.mode);
-      else
-        ntouch += row_epilogue(C, L, kind, (uint32_t*)a.canvas, W, F.rgbx, F.mode);
+  // one work unit per (row, column window), grid-stride
+  for (int64_t u = blockIdx.x; u < nUnits; u += gridDim.x) {
+    const int32_t y = a.rowBase + (int32_t)(u / nWin);
+    const int32_t wx = x0 + (int32_t)(u % nWin) * a.ww;
+    RowCtx C;
+    C.F = a.F;
+    C.parts = a.parts;
+    C.E = a.E;
+    C.EW = a.EW;
+    C.y = y;
+    C.wx0 = wx;
+    C.wx1 = wx + a.ww < hi ? wx + a.ww : hi;
+    C.cx0 = lo;
+    C.cx1 = hi;
+    int32_t kind = row_program(C, L);
+    if (kind < 0) {
+      if (threadIdx.x == 0) atomicOr(a.err, 1);
+      break;
     }
+    if (kind == kRowNone) continue;
+    if (kA8)
+      ntouch += row_epilogue_a8(C, L, kind, (uint8_t*)a.canvas, W, F.rgbx, F.mode);
+    else
+      ntouch += row_epilogue(C, L, kind, (uint32_t*)a.canvas, W, F.rgbx, F.mode);
   }
   if (a.touched) {
     // wave reduction, one atomic per workgroup
