@@ -96,8 +96,8 @@ int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, in
 
 /* Scenes (batched form) ------------------------------------------------- */
 /* A scene records an ordered sequence of fillPath / strokePath calls
- * (paths.nim:2093-2113 / 2144-2176; solid paints, no dashes, no elliptical
- * arc commands) and renders them in one device pass.  The result is the image
+ * (paths.nim:2093-2113 / 2144-2176; solid paints, dashes and elliptical arcs
+ * included) and renders them in one device pass.  The result is the image
  * the same calls leave when issued one by one in order.  Pixie has no batched
  * entry point; this one replaces a loop over fillPath / strokePath on one
  * image.  As for pxh_fill_path the caller keeps fillPath's paint checks;
@@ -105,7 +105,11 @@ int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, in
  * recorded.  The recording stays
  * resident on the device across renders; scene pools are sized on the first
  * render at a given image size (which waits for the device once) and reused
- * afterwards without host synchronisation. */
+ * afterwards without host synchronisation.  Paths with elliptical arc
+ * commands are flattened on the host when recorded (a flattening error is
+ * returned by the record call); everything else is flattened on the device.
+ * Dashes follow pxh_stroke_path (an invalid dash value is reported through
+ * pxh_scene_status). */
 typedef struct pxh_scene pxh_scene;
 pxh_scene* pxh_scene_new(void);
 void pxh_scene_free(pxh_scene* scene);
@@ -115,7 +119,8 @@ int pxh_scene_fill_path(pxh_scene* scene, const float* cmds, int32_t ncmds, cons
                         uint32_t rgbx, int32_t winding_rule, int32_t blend_mode);
 int pxh_scene_stroke_path(pxh_scene* scene, const float* cmds, int32_t ncmds, const float* mat9,
                           uint32_t rgbx, float stroke_width, int32_t line_cap,
-                          int32_t line_join, float miter_limit, int32_t blend_mode);
+                          int32_t line_join, float miter_limit, const float* dashes,
+                          int32_t ndashes, int32_t blend_mode);
 /* Renders the recorded calls onto img (queued on the library's stream). */
 int pxh_scene_render(pxh_scene* scene, pxh_image* img);
 /* Autotuner (no reference counterpart): renders the scene onto a scratch

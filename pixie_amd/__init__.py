@@ -105,7 +105,8 @@ def lib():
         L.pxh_scene_clear.argtypes = [vp]
         L.pxh_scene_count.argtypes = [vp]
         L.pxh_scene_fill_path.argtypes = [vp, f32p, i32, f32p, u32, i32, i32]
-        L.pxh_scene_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, i32]
+        L.pxh_scene_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, f32p, i32,
+                                            i32]
         L.pxh_scene_render.argtypes = [vp, vp]
         L.pxh_scene_status.argtypes = [vp, i32p, i32]
         L.pxh_scene_debug_geometry.argtypes = [vp, i32, i32p, P(i32p), i32p, P(f32p),
@@ -317,14 +318,15 @@ class Scene:
                                          windingRule, blendMode))
 
     def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
-                   lineJoin=MiterJoin, miterLimit=4.0, blendMode=0):
+                   lineJoin=MiterJoin, miterLimit=4.0, dashes=(), blendMode=0):
         if _paint_skipped(color, blendMode):
             return
         c = _cmds(path)
         m = _mat(transform)
+        d = np.ascontiguousarray(dashes, dtype=np.float32) if len(dashes) else None
         _check(lib().pxh_scene_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color),
-                                           strokeWidth, lineCap, lineJoin, miterLimit,
-                                           blendMode))
+                                           strokeWidth, lineCap, lineJoin, miterLimit, _fp(d),
+                                           0 if d is None else len(d), blendMode))
 
     def render(self, image, lane=0):
         """Renders onto `image`, queued on execution lane `lane`."""

@@ -31,6 +31,7 @@ struct pxh_scene {
   std::vector<int32_t> order;  // geometry workgroup -> call (built at upload)
   std::vector<pxh::SceneCmd> cmds;
   std::vector<pxh::ScenePoly> polys;
+  std::vector<float> dashes;   // dash patterns of the recorded strokes
   bool uploaded = false;
   int32_t validW = -1, validH = -1;  // target size the pools were validated for
   int32_t lane = 0;  // lane of the last render
@@ -56,13 +57,15 @@ struct pxh_scene {
   pxh::SceneCmd* dCmds = nullptr;
   pxh::ScenePoly* dPolys = nullptr;
   int32_t* dOrder = nullptr;  // geometry workgroup -> call, longest call first
-  size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0, dOrderCap = 0;
+  float* dDashes = nullptr;
+  size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0, dOrderCap = 0, dDashesCap = 0;
   // per-render buffers
   char* dMem = nullptr;
   size_t dMemBytes = 0;
   pxh::ScenePools pools{};
   int32_t ptCap = 0, segCap = 0, bandCap = 0, entryCap = 0, taskCap = 0, runCap = 0, binCap = 0;
   int32_t binSlots = 0;  // row bins the pools hold (height / kBinRows)
+  int32_t gcap = 8192;   // band entries a large-band wave can hold in global row scratch
 };
 
 namespace pxh {
@@ -129,6 +132,10 @@ struct Ctx {
   // own blur temporary and scene row scratch, so that work issued on
   // different lanes (by different host threads) overlaps on the device.
   hipStream_t lane[kLanes] = {};
+  // per lane: a second stream and fork / join events for phase A's two
+  // concurrent launches
+  hipStream_t laneAux[kLanes] = {};
+  hipEvent_t laneFork[kLanes] = {}, laneJoin[kLanes] = {};
   DevBuf laneTmp[kLanes];
   DevBuf laneScratch[kLanes];
   std::mutex lutMu;
@@ -176,6 +183,11 @@ int init() {
   PXH_HIP(hipStreamCreate(&g.stream));
   g.lane[0] = g.stream;
   for (int l = 1; l < kLanes; l++) PXH_HIP(hipStreamCreate(&g.lane[l]));
+  for (int l = 0; l < kLanes; l++) {
+    PXH_HIP(hipStreamCreate(&g.laneAux[l]));
+    PXH_HIP(hipEventCreateWithFlags(&g.laneFork[l], hipEventDisableTiming));
+    PXH_HIP(hipEventCreateWithFlags(&g.laneJoin[l], hipEventDisableTiming));
+  }
   PXH_HIP(hipEventCreateWithFlags(&g.stageFree, hipEventDisableTiming));
   PXH_HIP(hipEventRecord(g.stageFree, g.stream));
   PXH_HIP(hipMalloc((void**)&g.errFlag, sizeof(int32_t)));
@@ -454,7 +466,11 @@ int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius
 
 // ---- scene (batched form) ----
 
-const int kSceneGcap = 8192;  // band entries a wave can hold in global scratch
+// A row whose selection exceeds the large-band launch's LDS capacity keeps
+// its arrays in global row scratch, one slot of gcap entries per wave of that
+// launch.  gcap starts at 8192 and grows with the largest band a validated
+// render meets, up to kMaxSceneGcap and 8 GiB of slots in all.
+const int kMaxSceneGcap = 1 << 18;
 
 // Converts pixie commands to the device's canonical absolute form
 // (commandsToShapes' command semantics, paths.nim:886-1057): relative,
@@ -570,22 +586,83 @@ int canonicalize(const float* c, int32_t n, std::vector<SceneCmd>* out,
   return 0;
 }
 
+// Records flattened shapes as Move + Line subpaths that the device turns back
+// into the same points: consecutive shape points differ, so every Line emits
+// its end point, and noClose keeps the device from adding a closing point
+// (a fill's shapes already hold it).
+void canonicalize_shapes(const Shapes& sh, std::vector<SceneCmd>* out, std::vector<ScenePoly>* polys) {
+  const int32_t polyBase0 = (int32_t)polys->size();
+  for (int32_t k = 0; k < sh.count(); k++) {
+    const int32_t a = sh.ofs[k], b = sh.ofs[k + 1];
+    if (b <= a) continue;
+    ScenePoly sp = {(int32_t)out->size(), 0, 1, 0};
+    polys->push_back(sp);
+    for (int32_t i = a; i < b; i++) {
+      SceneCmd c = {};
+      c.kind = i == a ? 1.0f : 2.0f;
+      c.p[0] = sh.pts[i].x;
+      c.p[1] = sh.pts[i].y;
+      c.sx = sh.pts[a].x;
+      c.sy = sh.pts[a].y;
+      c.pad[0] = (int32_t)polys->size() - 1 - polyBase0;
+      out->push_back(c);
+      polys->back().numCmds++;
+    }
+  }
+}
+
+bool has_arc(const float* c, int32_t n) {
+  static const int pc[19] = {0, 2, 2, 1, 1, 6, 4, 4, 2, 7, 2, 2, 1, 1, 6, 4, 4, 2, 7};
+  for (int32_t i = 0; i < n;) {
+    const int32_t k = (int32_t)c[i];
+    if (k < 0 || k > 18) return false;  // canonicalize reports the bad command
+    if (k == 9 || k == 18) return true;
+    i += 1 + pc[k];
+  }
+  return false;
+}
+
 int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9, uint32_t rgbx,
                  int32_t rule, int32_t mode, bool stroke, float width, int32_t cap, int32_t join,
-                 float miterLimit) {
+                 float miterLimit, const float* dashes = nullptr, int32_t ndashes = 0) {
   if (!sc) return fail(kErrArg, "invalid scene handle");
   if (mode < 0 || mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
   if (rule != kNonZero && rule != kEvenOdd) return fail(kErrArg, "invalid winding rule");
+  if (ndashes < 0 || (ndashes > 0 && !dashes)) return fail(kErrArg, "invalid dash array");
   if (!mat9) mat9 = kIdentity;
   if (stroke && width <= 0.0f) return 0;  // strokeShapes yields nothing (paths.nim:1931-1932)
   SceneItem it = {};
   it.cmdBase = (int32_t)sc->cmds.size();
   it.polyBase = (int32_t)sc->polys.size();
   size_t polys0 = sc->polys.size(), cmds0 = sc->cmds.size();
-  if (int r = canonicalize(cmds, n, &sc->cmds, &sc->polys)) {
+  if (has_arc(cmds, n)) {
+    // Elliptical arcs are flattened here, on the host, at the call's
+    // tolerance: their flattening evaluates sinf / cosf / acosf, whose device
+    // results can differ from the host libm's in the last bit, and the points
+    // must be the ones pixie's addArc (paths.nim:764-884) emits.  The device
+    // then meets them as line subpaths.
+    Shapes sh;
+    std::string err;
+    int32_t r = commands_to_shapes(cmds, n, !stroke, pixel_scale(mat9), &sh, &err);
+    if (r != kOk) return fail(r, err);
+    canonicalize_shapes(sh, &sc->cmds, &sc->polys);
+  } else if (int r = canonicalize(cmds, n, &sc->cmds, &sc->polys)) {
     sc->cmds.resize(cmds0);
     sc->polys.resize(polys0);
     return r;
+  }
+  // dash pattern: an odd one is repeated to even length (paths.nim:2025-2027)
+  it.dashBase = (int32_t)sc->dashes.size();
+  it.dashCount = 0;
+  it.dashBad = 0;
+  if (stroke && ndashes > 0) {
+    for (int32_t k = 0; k < ndashes; k++) {
+      sc->dashes.push_back(dashes[k]);
+      if (dashes[k] <= 0.0f) it.dashBad = 1;
+    }
+    if (ndashes % 2 != 0)
+      for (int32_t k = 0; k < ndashes; k++) sc->dashes.push_back(dashes[k]);
+    it.dashCount = (int32_t)sc->dashes.size() - it.dashBase;
   }
   it.cmdCount = (int32_t)sc->cmds.size() - it.cmdBase;
   it.polyCount = (int32_t)sc->polys.size() - it.polyBase;
@@ -621,6 +698,9 @@ void scene_free_device(pxh_scene* sc) {
   if (sc->dCmds) (void)hipFree(sc->dCmds);
   if (sc->dPolys) (void)hipFree(sc->dPolys);
   if (sc->dOrder) (void)hipFree(sc->dOrder);
+  if (sc->dDashes) (void)hipFree(sc->dDashes);
+  sc->dDashes = nullptr;
+  sc->dDashesCap = 0;
   if (sc->dMem) (void)hipFree(sc->dMem);
   sc->dItems = nullptr;
   sc->dCmds = nullptr;
@@ -655,7 +735,7 @@ int scene_alloc(pxh_scene* sc) {
   size_t oRows = carve(nI * sizeof(int4)), oBinC = carve((size_t)sc->binSlots * 4);
   size_t oPaint = carve(nI * sizeof(int4)), oSpan = carve(nI * sizeof(int4));
   size_t oBinO = carve(((size_t)sc->binSlots + 1) * 4), oBinL = carve((size_t)sc->binCap * 4);
-  size_t oQ = carve(2 * kQueues * kQueueStride * 4);
+  size_t oQ = carve(kQueueSets * kQueues * kQueueStride * 4);
   size_t oCnt = carve(kCounterInts * 4);
   if (off > sc->dMemBytes) {
     PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));
@@ -670,6 +750,7 @@ int scene_alloc(pxh_scene* sc) {
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
   p.itemOrder = sc->dOrder;
+  p.dashes = sc->dDashes;
   p.nItems = (int32_t)nI;
   p.geo = (ItemGeom*)(b + oGeo);
   p.fills = (FillInfo*)(b + oFill);
@@ -709,7 +790,7 @@ int scene_alloc(pxh_scene* sc) {
   p.binSlots = sc->binSlots;
   p.queues = (int32_t*)(b + oQ);
   p.counters = (int32_t*)(b + oCnt);
-  p.gcap = kSceneGcap;
+  p.gcap = sc->gcap;
   return 0;
 }
 
@@ -738,6 +819,10 @@ int scene_upload(pxh_scene* sc, int lane) {
                          hipMemcpyHostToDevice, st));
   if (!sc->polys.empty())
     PXH_HIP(hipMemcpyAsync(sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly),
+                           hipMemcpyHostToDevice, st));
+  if (int r = scene_buf(&sc->dDashes, &sc->dDashesCap, sc->dashes.size())) return r;
+  if (!sc->dashes.empty())
+    PXH_HIP(hipMemcpyAsync(sc->dDashes, sc->dashes.data(), sc->dashes.size() * sizeof(float),
                            hipMemcpyHostToDevice, st));
   // Geometry workgroups start longest call first, so that a long stroke does
   // not start late and hold the kernel's end: estimated cost = commands,
@@ -770,7 +855,7 @@ int scene_upload(pxh_scene* sc, int lane) {
   sc->entryCap = keep(sc->entryCap, (int64_t)sc->segCap * 2);
   sc->taskCap = keep(sc->taskCap, std::min<size_t>(256 * sc->items.size() + 4096, 1u << 28));
   sc->runCap = keep(sc->runCap, std::min<int64_t>(
-      (int64_t)sc->taskCap * 8 + (int64_t)scene_queue_grid(g.numCUs * kMaxRowWavesPerCU) * kRunChunk,
+      (int64_t)sc->taskCap * 8 + (int64_t)scene_row_waves(g.numCUs) * kRunChunk,
       (int64_t)1 << 30));
   sc->binCap = keep(sc->binCap, std::min<size_t>(8 * sc->items.size() + 4096, 1u << 28));
   sc->uploaded = true;
@@ -791,11 +876,14 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane, bool checked = false) 
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
   p.itemOrder = sc->dOrder;
+  p.dashes = sc->dDashes;
   const RowShape shape = kRowShapes[sc->rowShape];
-  const int rowGrid = scene_queue_grid(g.numCUs * shape.wavesPerCU);
   p.ldsCap = shape.ldsCap;
+  // tasks of bands of more than ldsCap entries go to the large-band launch;
+  // a single-launch shape has none (class 32 is never reached)
+  p.bigClass = shape.mode ? 32 - __builtin_clz((unsigned)shape.ldsCap) : kCostClasses;
   DevBuf& scr = g.laneScratch[lane];
-  if (int e = scr.ensure(scene_gscratch_bytes(rowGrid, kSceneGcap)))
+  if (int e = scr.ensure(scene_gscratch_bytes(scene_scratch_slots(shape, g.numCUs), sc->gcap)))
     return fail(e, "scene row scratch allocation failed");
   p.gscratch = scr.p;
   int r = scene_launch_clear(p, st);
@@ -813,7 +901,7 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane, bool checked = false) 
   }
   if (!r) {
     LaunchTimer t("scene_rows_kernel", st);
-    r = scene_launch_rows(p, rowGrid, st);
+    r = scene_launch_rows(p, shape, g.numCUs, st, g.laneAux[lane], g.laneFork[lane], g.laneJoin[lane]);
   }
   if (!r) {
     LaunchTimer t("scene_compose_kernel", st);
@@ -833,7 +921,7 @@ int scene_replay(pxh_scene* sc, pxh_image* img, int lane) {
   hipStream_t st = g.lane[lane];
   const RowShape shape = kRowShapes[sc->rowShape];
   DevBuf& scr = g.laneScratch[lane];
-  if (int e = scr.ensure(scene_gscratch_bytes(scene_queue_grid(g.numCUs * shape.wavesPerCU), kSceneGcap)))
+  if (int e = scr.ensure(scene_gscratch_bytes(scene_scratch_slots(shape, g.numCUs), sc->gcap)))
     return fail(e, "scene row scratch allocation failed");
   pxh_scene::GraphKey key{img->dev, img->w, img->h, img->fmt, lane, sc->rowShape, sc->dMem, scr.p, sc->gen};
   if (!sc->exec || !(key == sc->key)) {
@@ -1271,6 +1359,7 @@ int pxh_scene_clear(pxh_scene* sc) {
   sc->items.clear();
   sc->cmds.clear();
   sc->polys.clear();
+  sc->dashes.clear();
   sc->uploaded = false;
   sc->validW = sc->validH = -1;
   return 0;
@@ -1285,9 +1374,10 @@ int pxh_scene_fill_path(pxh_scene* sc, const float* cmds, int32_t ncmds, const f
 
 int pxh_scene_stroke_path(pxh_scene* sc, const float* cmds, int32_t ncmds, const float* mat9,
                           uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
-                          float miter_limit, int32_t blend_mode) {
+                          float miter_limit, const float* dashes, int32_t ndashes,
+                          int32_t blend_mode) {
   return scene_record(sc, cmds, ncmds, mat9, rgbx, kNonZero, blend_mode, true, stroke_width,
-                      line_cap, line_join, miter_limit);
+                      line_cap, line_join, miter_limit, dashes, ndashes);
 }
 
 int pxh_scene_render(pxh_scene* sc, pxh_image* img) { return pxh_scene_render_on(sc, img, 0); }
@@ -1321,7 +1411,7 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
     // the run pool must hold the chunks of any later render of the same
     // scene, whatever order its waves take tasks in (scene_run_bound)
     const int64_t runNeed =
-        scene_run_bound(cnt[kCntRunsUsed], scene_queue_grid(g.numCUs * kMaxRowWavesPerCU));
+        scene_run_bound(cnt[kCntRunsUsed], scene_row_waves(g.numCUs));
     if (!cnt[kCntError]) {  // this render is complete; later ones get the slack
       if (runNeed > sc->runCap) {
         sc->runCap = (int32_t)std::min<int64_t>(runNeed + runNeed / 4, (int64_t)1 << 30);
@@ -1331,11 +1421,26 @@ int pxh_scene_render_on(pxh_scene* sc, pxh_image* img, int32_t lane) {
       sc->validH = img->h;
       return 0;
     }
-    if (cnt[kCntError] & 8) return fail(kErrCapacity, "a band holds more segments than the row scratch");
+    if (cnt[kCntError] & 8) {
+      // a band larger than the global row scratch slots: grow them
+      // (at most 8 GiB of slots in all)
+      const int64_t slots = scene_scratch_slots(kRowShapes[sc->rowShape], g.numCUs);
+      const int64_t maxCap = std::min<int64_t>(kMaxSceneGcap, ((int64_t)8 << 30) / (slots * 24));
+      if (sc->gcap * 2 > maxCap)
+        return fail(kErrCapacity, "a band holds more segments than the row scratch");
+      int64_t want = std::max<int64_t>((int64_t)sc->gcap * 2, cnt[kCntMaxBand]);
+      int32_t gc = sc->gcap;
+      while (gc < want && gc * 2 <= maxCap) gc *= 2;
+      sc->gcap = gc;
+    }
     auto grow = [](int32_t cap, int64_t need) {
       int64_t c = std::max<int64_t>((int64_t)cap * 2, need + need / 4 + 1024);
       return (int32_t)std::min<int64_t>(c, (int64_t)1 << 30);
     };
+    if (cnt[kCntError] == 8) {  // only the row scratch was short; every pool fit
+      if (int r = scene_alloc(sc)) return r;
+      continue;
+    }
     sc->ptCap = grow(sc->ptCap, cnt[kCntPoints]);
     sc->segCap = grow(sc->segCap, cnt[kCntSegs]);
     sc->bandCap = grow(sc->bandCap, cnt[kCntBands]);

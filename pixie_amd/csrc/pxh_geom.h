@@ -44,6 +44,12 @@ PXH_HD V2 mat_apply(const float* m, V2 v) {
 }
 
 constexpr float kFltEpsilon = 1.1920928955078125e-07f;  // epsilon(float32)
+// strokeShapes lays its dash pattern along each segment until the segment's
+// length is used up (paths.nim:2046-2059); a length that never runs out (NaN)
+// or a pattern far finer than the segment would loop without bound, so both
+// forms stop after this many passes over the pattern and report a capacity
+// error for the call.
+constexpr int kMaxDashPasses = 1 << 16;
 
 PXH_HD V2 cubic_at(V2 at, V2 c1, V2 c2, V2 to, float t) {
   float t2 = t * t, t3 = t2 * t;

@@ -594,7 +594,11 @@ int32_t stroke_shapes(const Shapes& in, const StrokeParams& sp, float pixelScale
         V2 dir = normalize(pos - prevPos);
         V2 currPos = prevPos;
         bool done = false;
-        while (!done) {
+        for (int pass = 0; !done; pass++) {
+          if (pass == kMaxDashPasses) {
+            *err = "dash pattern too fine for a segment";
+            return kErrCapacity;
+          }
           for (size_t j = 0; j < dashes.size(); j++) {
             float d = dashes[j];
             if (j % 2 == 0) {

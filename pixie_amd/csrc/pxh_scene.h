@@ -48,6 +48,8 @@ struct SceneItem {
   float mat[9];
   int32_t identity;
   int32_t strokeNone;          // strokeWidth <= 0: strokeShapes yields no shapes
+  int32_t dashBase, dashCount; // dash pattern in ScenePools::dashes (even length)
+  int32_t dashBad;             // a dash value <= 0: the stroke raises "Invalid line dash value"
   int32_t pad;
 };
 
@@ -95,7 +97,9 @@ enum SceneCounter {
   kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks,
   kCntRuns,      // run-pool entries handed out in chunks
   kCntRunsUsed,  // run entries written
-  kCntError, kCntMaxBand, kCntBinEntries, kNumCounters
+  kCntError, kCntMaxBand, kCntBinEntries,
+  kCntBigTasks,  // phase-A tasks of the large-band launch (the first in task order)
+  kNumCounters
 };
 
 // Phase-A tasks run longest first: a task's cost class is the bit length of
@@ -119,6 +123,8 @@ constexpr int kBinRows = 16;
 // every fetch.
 constexpr int kQueues = 16;
 constexpr int kQueueStride = 64;  // ints between queue heads (256 B)
+// Queue sets: phase A's large-band launch, its small-band launch, phase B.
+constexpr int kQueueSets = 3;
 
 // Grid of a queue-fed kernel: at least one wave per queue, a multiple of
 // kQueues so that every queue is served by the same number of waves.
@@ -133,6 +139,7 @@ struct ScenePools {
   const SceneCmd* cmds;
   const ScenePoly* polys;
   const int32_t* itemOrder;  // geometry workgroup -> call
+  const float* dashes;       // dash patterns of the recorded strokes
   int32_t nItems;
   int32_t W, H;              // target size
   // per-call outputs
@@ -180,19 +187,51 @@ struct ScenePools {
   char* gscratch;
   int32_t gcap;              // entries per wave slot
   int32_t ldsCap;            // entries that fit in LDS
+  int32_t bigClass;          // first cost class of the large-band launch
 };
 
-// Phase-A launch shapes: band entries a row wave keeps in LDS, and row waves
-// per CU.  The row program is latency-bound, so the grid keeps about as many
-// waves resident as LDS and registers allow for that capacity.  A scene's
-// autotuner picks one shape; run-pool sizing assumes the largest grid.
+// Phase-A launch shapes.  The row program is latency-bound, so each launch
+// keeps about as many waves resident as LDS and registers allow for the
+// band entries its waves keep in LDS.
+//   kRowsSingle      one launch of every task: ldsCap entries in LDS, larger
+//                    selections in the global row scratch
+//   kRowsConcurrent  tasks whose band holds more than ldsCap entries (their
+//                    cost class is above it; they come first in the task
+//                    order) run in a second launch of bigWaves waves per CU
+//                    with bigCap entries in LDS, beside the first launch
+//   kRowsSequential  the same two launches, one after the other
+// Only waves whose tasks may exceed their LDS capacity get global row
+// scratch slots.  A scene's autotuner picks one shape; run-pool sizing
+// assumes the most waves any shape has.
+enum RowMode { kRowsSingle = 0, kRowsConcurrent = 1, kRowsSequential = 2 };
 struct RowShape {
   int ldsCap, wavesPerCU;
+  int mode;
+  int bigCap, bigWaves;
 };
-constexpr RowShape kRowShapes[] = {{64, 24}, {128, 20}, {256, 14}, {512, 9}};
-constexpr int kNumRowShapes = 4;
-constexpr int kDefaultRowShape = 1;
-constexpr int kMaxRowWavesPerCU = 24;
+constexpr RowShape kRowShapes[] = {
+    {128, 20, kRowsSingle, 0, 0},
+    {128, 20, kRowsConcurrent, 512, 6},
+    {128, 20, kRowsSequential, 512, 9},
+    {128, 16, kRowsConcurrent, 256, 10},
+    {64, 24, kRowsSingle, 0, 0},
+    {256, 14, kRowsSingle, 0, 0},
+};
+constexpr int kNumRowShapes = 6;
+constexpr int kDefaultRowShape = 0;
+// Waves of a shape's phase-A launches together, and the most of any shape.
+inline int scene_shape_waves(const RowShape& s, int numCUs) {
+  return scene_queue_grid(numCUs * s.wavesPerCU) + (s.mode ? scene_queue_grid(numCUs * s.bigWaves) : 0);
+}
+inline int scene_row_waves(int numCUs) {
+  int m = 0;
+  for (const RowShape& s : kRowShapes) m = m > scene_shape_waves(s, numCUs) ? m : scene_shape_waves(s, numCUs);
+  return m;
+}
+// Wave slots of global row scratch a shape uses.
+inline int scene_scratch_slots(const RowShape& s, int numCUs) {
+  return s.mode ? scene_queue_grid(numCUs * s.bigWaves) : scene_queue_grid(numCUs * s.wavesPerCU);
+}
 
 int scene_set_lds_limits(size_t bytes);
 size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows
@@ -200,7 +239,10 @@ int scene_launch_clear(const ScenePools& p, hipStream_t s);
 int scene_launch_geometry(const ScenePools& p, hipStream_t s);
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s);
-int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s);
+// Phase A in the given shape; a concurrent shape forks its small-band
+// launch onto aux (fork / join through the two events).
+int scene_launch_rows(const ScenePools& p, const RowShape& shape, int numCUs, hipStream_t s,
+                      hipStream_t aux, hipEvent_t fork, hipEvent_t join);
 int scene_launch_compose(const ScenePools& p, void* canvas, bool a8, unsigned long long* touched,
                          int32_t* errOut, int grid, hipStream_t s);
 

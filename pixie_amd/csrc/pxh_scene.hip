@@ -192,6 +192,7 @@ struct SegSink {
   float4* out = nullptr;
   int8_t* outW = nullptr;
   int n = 0;
+  int err = 0;  // a dash pattern ran past kMaxDashPasses
   Bounds bd;
   __device__ void seg(V2 v1, V2 v2_) {
     Seg s;
@@ -335,7 +336,7 @@ __device__ __forceinline__ void join_segments(V2 prevPos, V2 pos, V2 nextPos, co
 // one circle a point may need is drawn at the end, from a single call site
 // (which keeps I and sink in registers once inlined).
 __device__ __forceinline__ void stroke_point(const float2* P, int k, int n, const SceneItem& I,
-                                             SegSink& sink) {
+                                             const float* dash, SegSink& sink) {
   auto pt = [&](int i) { return v2(P[i].x, P[i].y); };
   bool closed = veq(pt(0), pt(n - 1));
   bool circle = false;
@@ -351,7 +352,32 @@ __device__ __forceinline__ void stroke_point(const float2* P, int k, int n, cons
       }
     }
   } else {
-    rect_segments(pt(k - 1), pt(k), I, sink);
+    if (I.dashCount > 0) {
+      // the dash pattern restarts on every segment (paths.nim:2046-2059)
+      const V2 prevPos = pt(k - 1), pos = pt(k);
+      float distance = len(prevPos - pos);
+      const V2 dir = normalize(pos - prevPos);
+      V2 currPos = prevPos;
+      bool done = false;
+      for (int pass = 0; !done; pass++) {
+        if (pass == kMaxDashPasses) {
+          sink.err = 1;
+          break;
+        }
+        for (int j = 0; j < I.dashCount; j++) {
+          const float d = dash[j];
+          if ((j & 1) == 0) rect_segments(currPos, currPos + dir * nim_minf(distance, d), I, sink);
+          currPos = currPos + dir * d;
+          distance -= d;
+          if (distance <= 0.0f) {
+            done = true;
+            break;
+          }
+        }
+      }
+    } else {
+      rect_segments(pt(k - 1), pt(k), I, sink);
+    }
     if (k < n - 1) {
       join_segments(pt(k - 1), pt(k), pt(k + 1), I, sink, &circle);
       center = pt(k);
@@ -488,6 +514,9 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       atomicOr(&P.counters[kCntError], 1);
       sErr = kErrCapacity;
     }
+    // strokeShapes checks the dash values once it meets a shape with points
+    // (paths.nim:2029-2031)
+    if (I.stroke && !I.strokeNone && I.dashBad && total > 0 && sErr == kOk) sErr = kErrDash;
     sBase = base;
   }
   __syncthreads();
@@ -592,8 +621,9 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
             if (make_segment(v1, v2_, &s)) cnt = 1;
           }
         } else if (!I.strokeNone) {
-          stroke_point(pts + pb, k, np, I, sink);
+          stroke_point(pts + pb, k, np, I, P.dashes + I.dashBase, sink);
           cnt = sink.n;
+          if (sink.err) atomicMax(&sErr, kErrCapacity);
         }
         ptSeg[j] = cnt;
       }
@@ -622,7 +652,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
           int ql = ptPoly[j];
           int q = I.polyBase + ql;
           int pb = P.polyPt[q], np = P.polyN[q];
-          stroke_point(pts + pb, j - pb, np, I, out);
+          stroke_point(pts + pb, j - pb, np, I, P.dashes + I.dashBase, out);
           bd = out.bd;
         }
       }
@@ -1212,14 +1242,17 @@ __device__ __forceinline__ int2 task_share(int ntasks) {
   return make_int2(lo, lo + per < ntasks ? lo + per : ntasks);
 }
 
-// Start of each cost class in the task order, largest class first.
-__global__ void scene_task_scan_kernel(int32_t* counters) {
+// Start of each cost class in the task order, largest class first, and the
+// number of tasks in classes >= bigClass (the large-band launch's share).
+__global__ void scene_task_scan_kernel(int32_t* counters, int bigClass) {
   if (threadIdx.x != 0) return;
   int off = 0;
   for (int c = kCostClasses - 1; c >= 0; c--) {
+    if (c == bigClass - 1) counters[kCntBigTasks] = off;
     counters[kCntClassCursor + c] = off;
     off += counters[kCntClassHist + c];
   }
+  if (bigClass <= 0) counters[kCntBigTasks] = off;
 }
 
 // Places every task in the task order by its cost class: each block
@@ -1239,6 +1272,9 @@ __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
 
+// kBig: the large-band launch (tasks [0, nBig) of the task order), else the
+// small-band one; two instantiations so that profiles tell them apart.
+template <bool kBig>
 __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x;
@@ -1247,14 +1283,18 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   RowLds Lg = row_lds_carve(smem, P.gscratch + (size_t)blockIdx.x * row_cap_bytes(P.gcap), P.gcap, kWinA);
   for (int i = lane; i < kWinA + 4; i += 64) Ll.D[i] = 0;
   __syncthreads();
-  const int ntasks = P.counters[kCntTasks];
   if (P.counters[kCntError]) return;
+  // this launch's share of the task order: the large-band tasks come first
+  const int nBig = P.counters[kCntBigTasks];
+  const int tBegin = kBig ? 0 : nBig;
+  const int tEnd = kBig ? nBig : P.counters[kCntTasks];
+  int32_t* heads = P.queues + (kBig ? 0 : kQueues * kQueueStride);
   // Each wave appends its runs to chunks of the run pool it takes one at a
   // time (one atomic per chunk, not per task).
   int cursor = 0, chunkEnd = 0, used = 0;
   for (;;) {
-    const int q = queue_next(P.queues, ntasks);
-    if (q >= ntasks) break;
+    const int q = tBegin + queue_next(heads, tEnd - tBegin);
+    if (q >= tEnd) break;
     // Task, call and the call's records are the same in every lane: read
     // them into scalar registers.
     const int task = uni(P.taskOrder[q]);
@@ -1413,7 +1453,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
     return;
   }
   for (;;) {
-    const int unit = queue_next(P.queues + kQueues * kQueueStride, nUnits);
+    const int unit = queue_next(P.queues + 2 * kQueues * kQueueStride, nUnits);
     if (unit >= nUnits) break;
     const int y = unit / winsPerRow;
     const int b0 = (unit % winsPerRow) * kWinB;
@@ -1644,7 +1684,9 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
 // ---------------------------------------------------------------------------
 int scene_set_lds_limits(size_t bytes) {
   // the kernel's static LDS (its task slot) counts against the same limit
-  if (hipFuncSetAttribute((const void*)scene_rows_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+  if (hipFuncSetAttribute((const void*)scene_rows_kernel<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)(bytes - 1024)) != hipSuccess ||
+      hipFuncSetAttribute((const void*)scene_rows_kernel<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
                           (int)(bytes - 1024)) != hipSuccess)
     return kErrDevice;
   return 0;
@@ -1662,7 +1704,7 @@ __global__ void __launch_bounds__(256) scene_clear_kernel(int32_t* counters, int
 
 int scene_launch_clear(const ScenePools& p, hipStream_t s) {
   hipLaunchKernelGGL(scene_clear_kernel, dim3(1), dim3(256), 0, s, p.counters, (int)kCounterInts,
-                     p.queues, 2 * kQueues * kQueueStride);
+                     p.queues, kQueueSets * kQueues * kQueueStride);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 
@@ -1685,7 +1727,7 @@ int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s) {
 
 int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
   hipLaunchKernelGGL(scene_task_map_kernel, dim3(grid / 4 > 0 ? grid / 4 : 1), dim3(kTaskMapBlock), 0, s, p);
-  hipLaunchKernelGGL(scene_task_scan_kernel, dim3(1), dim3(64), 0, s, p.counters);
+  hipLaunchKernelGGL(scene_task_scan_kernel, dim3(1), dim3(64), 0, s, p.counters, p.bigClass);
   hipLaunchKernelGGL(scene_task_order_kernel, dim3(grid >= 8 ? grid / 4 : grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_kernel<false>, dim3(grid), dim3(64), 0, s, p);
   hipLaunchKernelGGL(scene_bin_scan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
@@ -1693,9 +1735,36 @@ int scene_launch_bins(const ScenePools& p, int grid, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 
-int scene_launch_rows(const ScenePools& p, int grid, hipStream_t s) {
-  size_t lds = row_lds_fixed_bytes(kWinA) + row_cap_bytes(p.ldsCap);
-  hipLaunchKernelGGL(scene_rows_kernel, dim3(grid), dim3(64), lds, s, p);
+int scene_launch_rows(const ScenePools& p, const RowShape& shape, int numCUs, hipStream_t s,
+                      hipStream_t aux, hipEvent_t fork, hipEvent_t join) {
+  const int gridSmall = scene_queue_grid(numCUs * shape.wavesPerCU);
+  ScenePools small = p;
+  small.ldsCap = shape.ldsCap;
+  const size_t smallLds = row_lds_fixed_bytes(kWinA) + row_cap_bytes(shape.ldsCap);
+  if (shape.mode == kRowsSingle) {  // every task; big ones use the row scratch
+    hipLaunchKernelGGL(scene_rows_kernel<false>, dim3(gridSmall), dim3(64), smallLds, s, small);
+    return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+  }
+  const int gridBig = scene_queue_grid(numCUs * shape.bigWaves);
+  ScenePools big = p;
+  big.ldsCap = shape.bigCap;
+  const size_t bigLds = row_lds_fixed_bytes(kWinA) + row_cap_bytes(shape.bigCap);
+  // the small-band launch never needs global row scratch (its bands hold
+  // fewer entries than its LDS capacity); with no slots, a row that did would
+  // report a capacity error instead of reaching another wave's slot
+  small.gscratch = nullptr;
+  small.gcap = 0;
+  if (shape.mode == kRowsSequential) {
+    hipLaunchKernelGGL(scene_rows_kernel<true>, dim3(gridBig), dim3(64), bigLds, s, big);
+    hipLaunchKernelGGL(scene_rows_kernel<false>, dim3(gridSmall), dim3(64), smallLds, s, small);
+    return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+  }
+  if (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(aux, fork, 0) != hipSuccess)
+    return kErrDevice;
+  hipLaunchKernelGGL(scene_rows_kernel<false>, dim3(gridSmall), dim3(64), smallLds, aux, small);
+  hipLaunchKernelGGL(scene_rows_kernel<true>, dim3(gridBig), dim3(64), bigLds, s, big);
+  if (hipEventRecord(join, aux) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess)
+    return kErrDevice;
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 

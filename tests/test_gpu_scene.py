@@ -132,7 +132,7 @@ def _oracle(orc, canvas, calls):
         if stroke:
             st = orc.stroke_path(want, c, kw["color"], width=kw["strokeWidth"], cap=kw["lineCap"],
                                  join=kw["lineJoin"], miter_limit=kw["miterLimit"],
-                                 mode=kw["blendMode"], mat=xf)
+                                 dashes=kw.get("dashes", ()), mode=kw["blendMode"], mat=xf)
         else:
             st = orc.fill_path(want, c, kw["color"], winding=kw["windingRule"],
                                mode=kw["blendMode"], mat=xf)
@@ -292,7 +292,7 @@ def test_scene_wide_bands_and_pool_growth(gpu, orc):
 def test_scene_status_and_errors(gpu):
     sc = px.Scene()
     with pytest.raises(px.PixieError):
-        sc.fillPath("M 10 10 A 5 5 0 0 1 20 20 Z", (255, 0, 0, 255))
+        sc.fillPath("M 0 0 L 10 0 L 10 10 Z", (255, 0, 0, 255), blendMode=25)
     assert len(sc) == 0
     sc.fillPath("M 0 0 L 10 0 L 10 10 Z", (0, 0, 0, 0))            # transparent: skipped
     sc.strokePath("M 0 0 L 10 0", (255, 0, 0, 255), strokeWidth=0)  # zero width: skipped
@@ -313,22 +313,75 @@ def test_scene_status_and_errors(gpu):
     _exact(img.download(), ref.download(), "status scene")
 
 
-def test_scene_band_over_capacity_is_an_error(gpu):
-    # more than 8192 segments crossing one row: the batched form reports a
-    # capacity error and leaves the image untouched
+def test_scene_band_over_capacity_renders(gpu):
+    # more than 8,192 segments crossing one row: the batched form grows its
+    # global row scratch and draws what the per-call form draws
     zig = Path()
     zig.move_to(0, 10)
     for i in range(9000):
         zig.line_to(i * 0.25, 20 if i % 2 == 0 else 10)
     zig.line_to(0, 30)
     zig.close_path()
+    calls = [(False, zig.array(), None, dict(color=(255, 0, 0, 255), windingRule=0, blendMode=0)),
+             (False, "M 5 2 L 60 2 L 60 38 Z", None,
+              dict(color=(0, 0, 255, 160), windingRule=1, blendMode=11))]
+    canvas = np.zeros((40, 2300, 4), np.uint8)
+    _compare(gpu, calls, canvas, "band of 9,000 entries")
+
+
+# Elliptical arcs (flattened when recorded) in every form pixie's tests use:
+# small / large arc, sweep, rotation, relative arcs, arcs inside longer paths.
+ARC_PATHS = [
+    "M30 60 A 20 20 0 0 0 90 60 L 30 60",
+    "M 10,30 A 20,20 0,0,1 50,30 A 20,20 0,0,1 90,30 Q 90,60 50,90 Q 10,60 10,30 z",
+    "M 20 50 A 20 10 45 1 1 80 50 L 20 50",
+    "M 0 50 A 50 50 0 0 0 50 0 L 50 50 L 0 50",
+    "M 10 50 a 30 15 -30 1 0 60 10 l -20 30 z m 70 -30 a 8 16 60 0 1 10 10",
+]
+DASHES = [(), (2, 2), (4, 1, 1), (5,), (1, 2, 3, 4, 5, 6, 7, 8, 9), (0.75, 0.5)]
+
+
+def test_scene_arcs_and_dashes(gpu, orc):
+    rng = np.random.default_rng(8)
+    W, H = 360, 280
+    calls = []
+    for i in range(48):
+        path = ARC_PATHS[i % len(ARC_PATHS)] if i % 3 else PATHS[i % len(PATHS)]
+        xf = XFORMS[i % len(XFORMS)]
+        off = translate(float(rng.uniform(-10, W - 90)), float(rng.uniform(-10, H - 90)))
+        xf = off if xf is None else mat_mul(off, xf)
+        rgb = rng.integers(0, 256, 3)
+        color = WL.premul(int(rgb[0]), int(rgb[1]), int(rgb[2]), int(rng.integers(60, 256)))
+        mode = int(rng.integers(0, 16))
+        if i % 2:
+            calls.append((True, path, xf, dict(color=color, strokeWidth=float(rng.uniform(0.8, 7)),
+                                               lineCap=i % 3, lineJoin=(i // 3) % 3,
+                                               miterLimit=4.0, dashes=DASHES[i % len(DASHES)],
+                                               blendMode=mode)))
+        else:
+            calls.append((False, path, xf, dict(color=color, windingRule=i % 4 // 2, blendMode=mode)))
+    canvas = WL.random_premultiplied_image(np.random.default_rng(9), W, H)
+    _compare(gpu, calls, canvas, "arcs and dashes", orc)
+
+
+def test_scene_invalid_dash_is_a_call_status(gpu):
+    # a dash value <= 0 makes strokePath raise (paths.nim:2029-2031); in a
+    # scene that call draws nothing and reports the error, the others draw
     sc = px.Scene()
-    sc.fillPath(zig.array(), (255, 0, 0, 255))
-    img = px.Image(2300, 40)
+    sc.fillPath("M 1 1 L 30 1 L 30 30 Z", (255, 0, 0, 255))
+    sc.strokePath("M 5 20 L 35 20", (0, 255, 0, 255), strokeWidth=4, dashes=(2, 0))
+    sc.strokePath("M 5 30 L 35 30", (0, 0, 255, 255), strokeWidth=4, dashes=(3,))
+    img = px.Image(40, 40)
+    sc.render(img)
+    st = sc.status()
+    ref = px.Image(40, 40)
+    ref.fillPath("M 1 1 L 30 1 L 30 30 Z", (255, 0, 0, 255))
     with pytest.raises(px.PixieError) as e:
-        sc.render(img)
-    assert e.value.code == 9
-    assert not img.download().any()
+        ref.strokePath("M 5 20 L 35 20", (0, 255, 0, 255), strokeWidth=4, dashes=(2, 0))
+    ref.strokePath("M 5 30 L 35 30", (0, 0, 255, 255), strokeWidth=4, dashes=(3,))
+    assert e.value.code == 3
+    assert list(st) == [0, 3, 0]
+    _exact(img.download(), ref.download(), "invalid dash scene")
 
 
 def test_scene_repeated_renders_are_identical(gpu):
@@ -412,6 +465,35 @@ def test_scene_heavy_runs_replayed(gpu):
     assert sc.graph_active()
 
 
+@pytest.mark.parametrize("shape", range(6))
+def test_scene_every_phase_a_shape(gpu, shape):
+    # every phase-A launch shape the autotuner may pick (one launch, or the
+    # large-band tasks in a second launch beside or before the rest) draws
+    # the per-call image, on a mixed scene with large bands and on combs
+    cmds, canvas = WL.mixed_scene(seed=13, n_paths=120, W=768, H=512)
+    ref = px.Image.from_array(canvas)
+    sc = px.Scene()
+    for c in cmds:
+        if c.stroke:
+            for t in (ref, sc):
+                t.strokePath(c.cmds, c.color, strokeWidth=c.width, lineCap=c.cap, lineJoin=c.join,
+                             miterLimit=c.miter, blendMode=c.mode)
+        else:
+            for t in (ref, sc):
+                t.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
+    for k in range(4):
+        comb = _comb(5.0 + 3 * k, 40.0 + 100 * k, 110, 6.03, 90.0, 2.37)
+        ref.fillPath(comb, (40 * k, 90, 200, 220), blendMode=k)
+        sc.fillPath(comb, (40 * k, 90, 200, 220), blendMode=k)
+    want = ref.download()
+    sc.set_shape(shape)
+    for rep in range(3):
+        img = px.Image.from_array(canvas)
+        sc.render(img)
+        px.synchronize()
+        _exact(img.download(), want, "shape %d render %d" % (shape, rep))
+
+
 def test_scene_replay_overflow_is_reported(gpu):
     # a replayed render that runs out of its run pool (forced from the host by
     # lowering the capacity its kernels see) stops before compose: the canvas
@@ -480,7 +562,7 @@ def test_scene_autotune_keeps_results(gpu):
             ref.fillPath(c.cmds, c.color, windingRule=c.winding, blendMode=c.mode)
     img = px.Image.from_array(canvas)
     shape, ms = sc.autotune(img, reps=2)
-    assert 0 <= shape < 4 and ms > 0
+    assert 0 <= shape < 6 and ms > 0
     _exact(img.download(), canvas, "autotune leaves the image alone")
     sc.render(img)
     _exact(img.download(), ref.download(), "render after autotune")
