@@ -105,12 +105,13 @@ struct DevBuf {
 };
 
 // Static device scratch of pxh_blur_noalloc: the X pass's intermediate image
-// (at most kNoAllocBlurPixels RGBA8 pixels, 16 MiB), then the Gaussian table.
+// (at most kNoAllocBlurPixels RGBA8 pixels, 16 MiB), then the Gaussian taps
+// and the weight table built from them.
 // Each GPU holds its own copy for the life of the process.  Host code never
 // uses the name: it uses the address hipGetSymbolAddress gives for the device
 // (Ctx::scratchPtr), taken once per device at init.
 constexpr size_t kNoAllocBlurPixels = (size_t)1 << 22;
-constexpr size_t kNoAllocLutWords = kMaxLutTaps;
+constexpr size_t kNoAllocLutWords = 4096;  // kMaxLutTaps taps + a weight table of up to 3072 dwords
 static __device__ __align__(256) uint32_t scratch[kNoAllocBlurPixels + kNoAllocLutWords];
 constexpr int kMaxDevices = 16;
 
@@ -160,7 +161,7 @@ struct Ctx {
   };
   std::vector<Timed> evPairs;
   std::map<std::string, std::pair<double, int64_t>> kernelTimes;  // last profile: ms, launches
-  std::map<int, uint32_t*> luts;  // gaussian LUT per radius (device, uint32 taps)
+  std::map<int, uint32_t*> luts;  // per radius: device weight table of the blur kernels
 };
 Ctx g;
 
@@ -439,13 +440,18 @@ int get_lut(int radius, uint32_t** dev) {
     *dev = it->second;
     return 0;
   }
+  // taps, then the weight table the blur kernels read (built on the device)
   std::vector<uint32_t> lut;
   gaussian_lut(radius, &lut);
+  const size_t tabWords = 8 * (size_t)blur_table_dwords(radius);
   uint32_t* p = nullptr;
-  PXH_HIP(hipMalloc((void**)&p, lut.size() * 4));
+  PXH_HIP(hipMalloc((void**)&p, (lut.size() + tabWords) * 4));
   PXH_HIP(hipMemcpy(p, lut.data(), lut.size() * 4, hipMemcpyHostToDevice));
-  g.luts[radius] = p;
-  *dev = p;
+  if (launch_blur_table(p, radius, p + lut.size(), g.stream) != 0)
+    return fail(kErrDevice, "blur table launch failed");
+  PXH_HIP(hipStreamSynchronize(g.stream));
+  g.luts[radius] = p + lut.size();
+  *dev = p + lut.size();
   return 0;
 }
 
@@ -453,12 +459,13 @@ int blur_impl(uint32_t* src_dst, uint32_t* out, int32_t w, int32_t h, int radius
               bool shadowColor, uint32_t color, int lane = 0) {
   if (blur_x_lds(radius) > g.maxLds || blur_y_lds(radius) > g.maxLds)
     return fail(kErrCapacity, "blur radius too large for the LDS tiles");
-  uint32_t* lut = nullptr;
-  if (int r = get_lut(radius, &lut)) return r;
+  uint32_t* wtab = nullptr;
+  if (int r = get_lut(radius, &wtab)) return r;
   DevBuf& tmp = lane == 0 ? g.tmp : g.laneTmp[lane];
   if (int e = tmp.ensure((size_t)w * h * 4)) return fail(e, "blur temporary allocation failed");
-  BlurArgs ax{src_dst, (uint32_t*)tmp.p, lut, w, h, radius, oob, 0, 0};
-  BlurArgs ay{(uint32_t*)tmp.p, out, lut, w, h, radius, oob, shadowColor ? 1 : 0, color};
+  const int nd = blur_table_dwords(radius);
+  BlurArgs ax{src_dst, (uint32_t*)tmp.p, wtab, w, h, radius, oob, 0, 0, nd};
+  BlurArgs ay{(uint32_t*)tmp.p, out, wtab, w, h, radius, oob, shadowColor ? 1 : 0, color, nd};
   if (launch_blur(ax, ay, g.lane[lane]) != 0) return fail(kErrDevice, "blur launch failed");
   return 0;
 }
@@ -1193,7 +1200,9 @@ int pxh_blur_noalloc(pxh_image* img, float radius, uint32_t oob_rgbx) {
   if (r < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
   if ((size_t)img->w * (size_t)img->h > kNoAllocBlurPixels)
     return fail(kErrCapacity, "image larger than the allocation-free blur's scratch (4194304 pixels)");
-  if (2 * r + 1 > (int64_t)kNoAllocLutWords || blur_x_lds((int)r) > g.maxLds || blur_y_lds((int)r) > g.maxLds)
+  if (2 * r + 1 > (int64_t)kMaxLutTaps ||
+      2 * r + 1 + 8 * (int64_t)blur_table_dwords((int)r) > (int64_t)kNoAllocLutWords ||
+      blur_x_lds((int)r) > g.maxLds || blur_y_lds((int)r) > g.maxLds)
     return fail(kErrCapacity, "blur radius too large for the LDS tiles");
   uint32_t* base = g.scratchPtr[g.device];
   if (!base) return fail(kErrDevice, "allocation-free scratch is not set up on this device");
@@ -1203,10 +1212,13 @@ int pxh_blur_noalloc(pxh_image* img, float radius, uint32_t oob_rgbx) {
   gaussian_lut_into((int)r, f, taps.v);
   uint32_t* tmp = base;
   uint32_t* lut = base + kNoAllocBlurPixels;
-  if (launch_lut_store(taps, lut, g.stream) != 0) return fail(kErrDevice, "table store launch failed");
+  uint32_t* wtab = lut + taps.n;
+  if (launch_lut_store(taps, lut, g.stream) != 0 || launch_blur_table(lut, (int)r, wtab, g.stream) != 0)
+    return fail(kErrDevice, "table store launch failed");
   uint32_t* p = (uint32_t*)img->dev;
-  BlurArgs ax{p, tmp, lut, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0};
-  BlurArgs ay{tmp, p, lut, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0};
+  const int nd = blur_table_dwords((int)r);
+  BlurArgs ax{p, tmp, wtab, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0, nd};
+  BlurArgs ay{tmp, p, wtab, img->w, img->h, (int32_t)r, oob_rgbx, 0, 0, nd};
   if (launch_blur(ax, ay, g.stream) != 0) return fail(kErrDevice, "blur launch failed");
   return 0;
 }

@@ -36,11 +36,12 @@ struct CompositeArgs {
 struct BlurArgs {
   const uint32_t* src;
   uint32_t* dst;
-  const uint32_t* lut;   // 2r+1 taps (device)
+  const uint32_t* wtab;  // weight table of blur_table_kernel (device)
   int32_t w, h, radius;
   uint32_t oob;
   int32_t shadowColor;   // Y pass only: write blendMask(color, result)
   uint32_t color;
+  int32_t nd;            // blur_table_dwords(radius)
 };
 
 // Gaussian taps passed by value (kernel arguments stay under 4 KB).
@@ -64,6 +65,10 @@ int launch_composite(const CompositeArgs& a, bool srcA8, hipStream_t s);
 int launch_blur(const BlurArgs& ax, const BlurArgs& ay, hipStream_t s);
 int set_blur_lds_limit(size_t bytes);
 int launch_lut_store(const LutTaps& t, uint32_t* dst, hipStream_t s);
+// Weight table of a blur of radius r (2 * 4 * blur_table_dwords(r) dwords)
+// built on the device from its 2r + 1 taps.
+int blur_table_dwords(int r);
+int launch_blur_table(const uint32_t* lut, int r, uint32_t* wtab, hipStream_t s);
 size_t blur_x_lds(int r);
 size_t blur_y_lds(int r);
 int launch_spread(const SpreadArgs& a, hipStream_t s);

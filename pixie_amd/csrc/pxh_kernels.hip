@@ -157,81 +157,176 @@ int launch_composite(const CompositeArgs& a, bool srcA8, hipStream_t s) {
 }
 
 // ---------------------------------------------------------------------------
-// blur: separable gaussian on premultiplied RGBA8 (or an alpha plane).
-// Each pass computes sum(c*k) in uint32 and stores (sum div 256 div 255).
-// The LUT is symmetric, so taps are paired: k[r+j]*(c[-j] + c[+j]).
+// blur: separable gaussian on premultiplied RGBA8 (images.nim:304-365).
+//
+// Each pass computes, per channel, sum(c * k) over 2r + 1 taps in uint32 and
+// stores (sum div 256 div 255).  The sums are formed with v_dot4_u32_u8
+// (four byte products per instruction): the pixels are byte-transposed into
+// channel planes (byte i of a plane dword = one channel of pixel i), and each
+// 16-bit tap k = 256 * kh + kl is split into its bytes, so
+//   sum(c * k) = 256 * sum(c * kh) + sum(c * kl)
+// exactly (every partial sum stays below 2^32).  A thread computes four
+// consecutive outputs (m = 0..3); output m of a thread whose first input
+// dword is d0 weighs byte i of dword d0 + d by the tap 4d + i - m - a, where
+// a = R - r and R is r rounded up to a multiple of 4, so one plane dword
+// feeds four outputs through four shifted weight tables (zero outside the
+// window).  The table is built once per radius by blur_table_kernel.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void acc_rgba(uint32_t* acc, uint32_t p, uint32_t k) {
-  acc[0] += (p & 255u) * k;
-  acc[1] += ((p >> 8) & 255u) * k;
-  acc[2] += ((p >> 16) & 255u) * k;
-  acc[3] += (p >> 24) * k;
-}
-__device__ __forceinline__ void acc_pair(uint32_t* acc, uint32_t p, uint32_t q, uint32_t k) {
-  uint32_t rb = (p & 0x00FF00FFu) + (q & 0x00FF00FFu);
-  uint32_t ga = ((p >> 8) & 0x00FF00FFu) + ((q >> 8) & 0x00FF00FFu);
-  acc[0] += (rb & 0xFFFFu) * k;
-  acc[2] += (rb >> 16) * k;
-  acc[1] += (ga & 0xFFFFu) * k;
-  acc[3] += (ga >> 16) * k;
-}
-__device__ __forceinline__ uint32_t acc_out(const uint32_t* acc) {
-  return pack(acc[0] / 256u / 255u, acc[1] / 256u / 255u, acc[2] / 256u / 255u, acc[3] / 256u / 255u);
+
+// Weight dwords of one blur: wtab[(d * 4 + m) * 2 + h], h = 0 the low bytes
+// of the taps, h = 1 the high bytes (the eight dwords of input dword d are
+// contiguous, so a wave reads them as two broadcast 16-byte LDS reads).
+int blur_table_dwords(int r) { return (2 * r + 10) / 4; }
+
+__global__ void __launch_bounds__(256) blur_table_kernel(const uint32_t* lut, int r, int nd,
+                                                         uint32_t* wtab) {
+  const int a = ((r + 3) & ~3) - r;
+  for (int e = threadIdx.x; e < 4 * nd; e += 256) {
+    const int d = e / 4, m = e % 4;
+    uint32_t lo = 0, hi = 0;
+    for (int i = 0; i < 4; i++) {
+      const int j = 4 * d + i - m - a;
+      const uint32_t k = (j >= 0 && j <= 2 * r) ? lut[j] : 0u;
+      lo |= (k & 255u) << (8 * i);
+      hi |= (k >> 8) << (8 * i);
+    }
+    wtab[2 * e] = lo;
+    wtab[2 * e + 1] = hi;
+  }
 }
 
-// X pass over rows: a block produces 256 outputs of one row.  LDS holds the
-// row segment [x0 - r, x0 + 256 + r) with out-of-bounds samples replaced by
-// the oob colour.
+int launch_blur_table(const uint32_t* lut, int r, uint32_t* wtab, hipStream_t s) {
+  hipLaunchKernelGGL(blur_table_kernel, dim3(1), dim3(256), 0, s, lut, r, blur_table_dwords(r), wtab);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+// Four RGBA8 pixels -> four channel planes (plane c, byte i = channel c of
+// pixel i), eight byte permutes.
+__device__ __forceinline__ uint4 planes4(uint32_t p0, uint32_t p1, uint32_t p2, uint32_t p3) {
+  const uint32_t t0 = __builtin_amdgcn_perm(p1, p0, 0x05010400u);  // r0 r1 g0 g1
+  const uint32_t t1 = __builtin_amdgcn_perm(p1, p0, 0x07030602u);  // b0 b1 a0 a1
+  const uint32_t t2 = __builtin_amdgcn_perm(p3, p2, 0x05010400u);  // r2 r3 g2 g3
+  const uint32_t t3 = __builtin_amdgcn_perm(p3, p2, 0x07030602u);  // b2 b3 a2 a3
+  return make_uint4(__builtin_amdgcn_perm(t2, t0, 0x05040100u), __builtin_amdgcn_perm(t2, t0, 0x07060302u),
+                    __builtin_amdgcn_perm(t3, t1, 0x05040100u), __builtin_amdgcn_perm(t3, t1, 0x07060302u));
+}
+
+// The four outputs' channel sums of one thread.
+struct BlurAcc {
+  uint32_t lo[4][4], hi[4][4];  // [m][channel]
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) lo[m][c] = hi[m][c] = 0;
+  }
+  // v: the four channel planes of one input dword; w: its eight weight
+  // dwords (LDS, the same for every lane)
+  __device__ __forceinline__ void add(const uint4 v, const uint4* w) {
+    const uint4 w01 = w[0], w23 = w[1];
+    const uint32_t ws[8] = {w01.x, w01.y, w01.z, w01.w, w23.x, w23.y, w23.z, w23.w};
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const uint32_t wl = ws[2 * m], wh = ws[2 * m + 1];
+      lo[m][0] = __builtin_amdgcn_udot4(v.x, wl, lo[m][0], false);
+      hi[m][0] = __builtin_amdgcn_udot4(v.x, wh, hi[m][0], false);
+      lo[m][1] = __builtin_amdgcn_udot4(v.y, wl, lo[m][1], false);
+      hi[m][1] = __builtin_amdgcn_udot4(v.y, wh, hi[m][1], false);
+      lo[m][2] = __builtin_amdgcn_udot4(v.z, wl, lo[m][2], false);
+      hi[m][2] = __builtin_amdgcn_udot4(v.z, wh, hi[m][2], false);
+      lo[m][3] = __builtin_amdgcn_udot4(v.w, wl, lo[m][3], false);
+      hi[m][3] = __builtin_amdgcn_udot4(v.w, wh, hi[m][3], false);
+    }
+  }
+  __device__ __forceinline__ uint32_t out(int m) const {
+    uint32_t c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = (lo[m][k] + (hi[m][k] << 8)) / 256u / 255u;
+    return pack(c[0], c[1], c[2], c[3]);
+  }
+};
+
+// X pass: a block produces 1024 outputs of one row, four per thread.  LDS
+// holds the row segment from x0 - R in channel planes, out-of-bounds pixels
+// replaced by the oob colour.
 __global__ void __launch_bounds__(256) blur_x_kernel(BlurArgs a) {
-  extern __shared__ uint32_t srow[];
+  extern __shared__ uint4 splane[];
   const int32_t y = blockIdx.y;
-  const int32_t x0 = blockIdx.x * 256;
-  const int32_t r = a.radius;
-  const int32_t n = 256 + 2 * r;
+  const int32_t x0 = blockIdx.x * 1024;
+  const int nd = a.nd;
+  uint4* sw = splane + 256 + nd;  // the weight table, 2 uint4 per input dword
+  for (int i = threadIdx.x; i < 2 * nd; i += 256) sw[i] = ((const uint4*)a.wtab)[i];
+  const int32_t s0 = x0 - ((a.radius + 3) & ~3);
   const uint32_t* in = a.src + (size_t)y * a.w;
-  for (int i = threadIdx.x; i < n; i += 256) {
-    int32_t x = x0 - r + i;
-    srow[i] = (x >= 0 && x < a.w) ? in[x] : a.oob;
+  const bool vec = (a.w & 3) == 0;
+  for (int q = threadIdx.x; q < 256 + nd; q += 256) {
+    const int32_t xs = s0 + 4 * q;
+    uint32_t p[4];
+    if (vec && xs >= 0 && xs + 3 < a.w) {
+      const uint4 v = *(const uint4*)(in + xs);
+      p[0] = v.x; p[1] = v.y; p[2] = v.z; p[3] = v.w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++) p[i] = (xs + i >= 0 && xs + i < a.w) ? in[xs + i] : a.oob;
+    }
+    splane[q] = planes4(p[0], p[1], p[2], p[3]);
   }
   __syncthreads();
-  const int32_t x = x0 + threadIdx.x;
-  if (x >= a.w) return;
-  const int32_t c = threadIdx.x + r;
-  uint32_t acc[4] = {0, 0, 0, 0};
-  acc_rgba(acc, srow[c], a.lut[r]);
-  for (int32_t j = 1; j <= r; j++) acc_pair(acc, srow[c - j], srow[c + j], a.lut[r + j]);
-  a.dst[(size_t)y * a.w + x] = acc_out(acc);
+  const int t = threadIdx.x;
+  const int32_t xo = x0 + 4 * t;
+  if (xo >= a.w) return;
+  BlurAcc acc;
+  acc.zero();
+  for (int d = 0; d < nd; d++) acc.add(splane[t + d], sw + 2 * d);
+  uint32_t* outp = a.dst + (size_t)y * a.w;
+  if (vec && xo + 3 < a.w) {
+    *(uint4*)(outp + xo) = make_uint4(acc.out(0), acc.out(1), acc.out(2), acc.out(3));
+  } else {
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      if (xo + m < a.w) outp[xo + m] = acc.out(m);
+  }
 }
 
-// Y pass: a block produces a 64-column x kTileRows tile; LDS holds the
-// columns' [y0 - r, y0 + kTileRows + r) rows.
+// Y pass: a block produces a 64-column x 64-row tile; thread (column, g)
+// computes the row quads g, g + 4, g + 8, g + 12.  LDS holds the columns'
+// rows from y0 - R, four rows to a plane dword.
 constexpr int kBlurTileRows = 64;
 __global__ void __launch_bounds__(256) blur_y_kernel(BlurArgs a) {
-  extern __shared__ uint32_t stile[];
+  extern __shared__ uint4 stile[];
   const int32_t x0 = blockIdx.x * 64, y0 = blockIdx.y * kBlurTileRows;
-  const int32_t r = a.radius;
-  const int32_t rows = kBlurTileRows + 2 * r;
-  const int32_t tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+  const int nd = a.nd;
+  const int32_t s0 = y0 - ((a.radius + 3) & ~3);
+  const int nq = kBlurTileRows / 4 - 1 + nd;  // row quads the tile reads
+  uint4* sw = stile + nq * 64;                 // the weight table
+  for (int i = threadIdx.x; i < 2 * nd; i += 256) sw[i] = ((const uint4*)a.wtab)[i];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   const int32_t x = x0 + tx;
-  for (int32_t i = ty; i < rows; i += 4) {
-    int32_t yy = y0 - r + i;
-    uint32_t v = a.oob;
-    if (yy >= 0 && yy < a.h && x < a.w) v = a.src[(size_t)yy * a.w + x];
-    stile[i * 64 + tx] = v;
+  for (int q = ty; q < nq; q += 4) {
+    const int32_t ys = s0 + 4 * q;
+    uint32_t p[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int32_t yy = ys + i;
+      p[i] = (yy >= 0 && yy < a.h && x < a.w) ? a.src[(size_t)yy * a.w + x] : a.oob;
+    }
+    stile[q * 64 + tx] = planes4(p[0], p[1], p[2], p[3]);
   }
   __syncthreads();
   if (x >= a.w) return;
-  for (int32_t o = ty; o < kBlurTileRows; o += 4) {
-    int32_t y = y0 + o;
-    if (y >= a.h) break;
-    const int32_t c = o + r;
-    uint32_t acc[4] = {0, 0, 0, 0};
-    acc_rgba(acc, stile[c * 64 + tx], a.lut[r]);
-    for (int32_t j = 1; j <= r; j++)
-      acc_pair(acc, stile[(c - j) * 64 + tx], stile[(c + j) * 64 + tx], a.lut[r + j]);
-    uint32_t out = acc_out(acc);
-    if (a.shadowColor) out = blend(kMask, a.color, out);  // shadow: fill(color).draw(mask, MaskBlend)
-    a.dst[(size_t)y * a.w + x] = out;
+  for (int Q = ty; Q < kBlurTileRows / 4; Q += 4) {
+    const int32_t yq = y0 + 4 * Q;
+    if (yq >= a.h) break;
+    BlurAcc acc;
+    acc.zero();
+    for (int d = 0; d < nd; d++) acc.add(stile[(Q + d) * 64 + tx], sw + 2 * d);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      if (yq + m >= a.h) break;
+      uint32_t out = acc.out(m);
+      if (a.shadowColor) out = blend(kMask, a.color, out);  // shadow: fill(color).draw(mask, MaskBlend)
+      a.dst[(size_t)(yq + m) * a.w + x] = out;
+    }
   }
 }
 
@@ -246,11 +341,13 @@ int launch_lut_store(const LutTaps& t, uint32_t* dst, hipStream_t s) {
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }
 
-size_t blur_x_lds(int r) { return (size_t)(256 + 2 * r) * 4; }
-size_t blur_y_lds(int r) { return (size_t)64 * (kBlurTileRows + 2 * r) * 4; }
+size_t blur_x_lds(int r) { return (size_t)(256 + 3 * blur_table_dwords(r)) * 16; }
+size_t blur_y_lds(int r) {
+  return (size_t)(kBlurTileRows / 4 - 1 + blur_table_dwords(r)) * 64 * 16 + (size_t)blur_table_dwords(r) * 32;
+}
 
 int launch_blur(const BlurArgs& ax, const BlurArgs& ay, hipStream_t s) {
-  dim3 gx((ax.w + 255) / 256, ax.h);
+  dim3 gx((ax.w + 1023) / 1024, ax.h);
   hipLaunchKernelGGL(blur_x_kernel, gx, dim3(256), blur_x_lds(ax.radius), s, ax);
   dim3 gy((ay.w + 63) / 64, (ay.h + kBlurTileRows - 1) / kBlurTileRows);
   hipLaunchKernelGGL(blur_y_kernel, gy, dim3(256), blur_y_lds(ay.radius), s, ay);
