@@ -193,24 +193,30 @@ def image_ops_extra(px, WL):
     return out
 
 
-def serve_extra(px, WL, rates=(50.0, 200.0, 800.0), n=60, plain_n=24):
+def serve_extra(px, WL, rates=(50.0, 200.0, 800.0), n=500, pool_n=120):
     """Serving loop: Poisson arrivals of scenes (1-500 paths, side 256-4096,
-    blur 0 or 8), continuous batching over the execution lanes vs the plain
-    per-call form serving the same arrivals one at a time."""
+    blur 0 or 8), n requests per rate cycled from a pool of pool_n distinct
+    scenes.  Three forms serve the same arrivals: continuous batching over
+    the execution lanes driven from Python threads, the library's own C++
+    server (pxh_server_*), and the plain per-call form one request at a
+    time."""
     from pixie_amd import serve
     out = {"workload": "serving: Poisson arrivals, scenes of 1-500 paths (log-uniform) as in the mixed "
-                       "scene, canvas side log-uniform 256-4096 px RGBA8, blur 0 or 8 (seed 5)",
+                       "scene, canvas side log-uniform 256-4096 px RGBA8, blur 0 or 8 (seed 5); "
+                       "%d requests per rate from %d distinct scenes" % (n, pool_n),
            "lanes": px.lanes() - 1, "hbm_budget_gib": 4}
+    pool = WL.serve_requests(seed=5, n=pool_n, rate=0)
     server = serve.Server(hbm_budget=4 << 30)
+    native = serve.NativeServer(hbm_budget=4 << 30)
     warm = WL.serve_requests(seed=99, n=8, rate=0)
     server.run(warm)
+    native.run(warm)
     for rate in rates:
-        reqs = WL.serve_requests(seed=5, n=n, rate=rate)
-        res = server.run(reqs)
-        out["batched_rate_%d" % rate] = serve.latency_stats(res, reqs)
-    reqs = WL.serve_requests(seed=5, n=plain_n, rate=rates[0])
-    res = serve.serve_plain(reqs)
-    out["plain_rate_%d" % rates[0]] = serve.latency_stats(res, reqs)
+        reqs = WL.poisson_arrivals(pool, n, rate)
+        out["batched_rate_%d" % rate] = serve.latency_stats(server.run(reqs), reqs)
+        out["native_rate_%d" % rate] = serve.latency_stats(native.run(reqs), reqs)
+        out["plain_rate_%d" % rate] = serve.latency_stats(serve.serve_plain(reqs), reqs)
+        log("serving at %g/s: %s" % (rate, out["native_rate_%d" % rate]))
     return out
 
 

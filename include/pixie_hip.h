@@ -125,8 +125,12 @@ int pxh_scene_stroke_path(pxh_scene* scene, const float* cmds, int32_t ncmds, co
 int pxh_scene_render(pxh_scene* scene, pxh_image* img);
 /* Autotuner (no reference counterpart): renders the scene onto a scratch
  * image of img's size with each phase-A launch shape (band entries kept in
- * LDS x row waves per CU), `reps` times each, and keeps the fastest for
- * later renders.  *shape receives its index, *ms its time per render. */
+ * LDS x row waves per CU, and whether tasks of large bands run in a launch
+ * of their own), `reps` timed renders each, and keeps the fastest for later
+ * renders.  *shape receives its index, *ms its time per render.  Results are
+ * cached for the process, keyed by device arch, target size and format, and
+ * the scene's call count and command count (to a power of two): a scene of
+ * that key takes the cached shape without timing. */
 int pxh_scene_autotune(pxh_scene* scene, pxh_image* img, int32_t reps, int32_t* shape,
                        float* ms);
 /* Sets the launch shape directly (0..3; 1 is the default). */
@@ -165,6 +169,41 @@ int pxh_image_fill_on(pxh_image* img, uint32_t rgbx, int32_t lane);
 /* A non-owning image over caller-owned device memory (width*height*bpp
  * bytes); pxh_image_free releases only the handle. */
 pxh_image* pxh_image_view(void* device_ptr, int32_t width, int32_t height, int32_t format);
+
+/* Serving (no reference counterpart: pixie is a library with no server).
+ * A server renders scene requests with continuous batching: requests queue
+ * in arrival order, and each of n_lanes worker threads (lanes 1..n_lanes,
+ * so 1 <= n_lanes < pxh_lanes(); lane 0 stays the caller's) takes the oldest
+ * queued request as soon as its canvas fits in the server's resident HBM
+ * arena of hbm_budget bytes, fills the canvas transparent, renders the scene
+ * on its lane and blurs it (images.nim:304-365) if blur_radius != 0.
+ * Requests admitted while others run overlap on the device.  A request's
+ * scene must stay alive and unchanged, and in no other unfinished request,
+ * until the request is done.  A failing request (for example a negative
+ * blur radius) is done with its PXH_ERR_* status and does not affect the
+ * others.  A finished request's canvas (RGBA8, width x height) stays in the
+ * arena until pxh_server_release: release requests to keep admitting. */
+typedef struct pxh_server pxh_server;
+pxh_server* pxh_server_new(int32_t n_lanes, int64_t hbm_budget);
+/* Waits for running requests, drops queued ones and frees the arena. */
+void pxh_server_free(pxh_server* server);
+/* Queues a request; returns its ticket (>= 0), or -PXH_ERR_ARG /
+ * -PXH_ERR_CAPACITY (canvas larger than the arena). */
+int64_t pxh_server_enqueue(pxh_server* server, pxh_scene* scene, int32_t width, int32_t height,
+                           float blur_radius);
+/* *done = 1 once the request finished, with its status in *status. */
+int pxh_server_poll(pxh_server* server, int64_t ticket, int32_t* done, int32_t* status);
+int pxh_server_wait(pxh_server* server, int64_t ticket, int32_t* status);
+/* The finished request's canvas (a view into the arena, valid until
+ * release), its error text, and its enqueue / admission / completion times
+ * in seconds on the server's clock (pxh_server_now). */
+pxh_image* pxh_server_image(pxh_server* server, int64_t ticket);
+const char* pxh_server_message(pxh_server* server, int64_t ticket);
+int pxh_server_times(pxh_server* server, int64_t ticket, double* enqueued, double* admitted,
+                     double* done);
+double pxh_server_now(pxh_server* server);
+/* Returns a finished request's canvas to the arena and forgets the ticket. */
+int pxh_server_release(pxh_server* server, int64_t ticket);
 
 /* Image operations ------------------------------------------------------ */
 /* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.

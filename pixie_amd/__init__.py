@@ -112,6 +112,22 @@ def lib():
         L.pxh_scene_debug_geometry.argtypes = [vp, i32, i32p, P(i32p), i32p, P(f32p),
                                                P(P(ctypes.c_int8)), i32p]
         L.pxh_scene_debug_limit_runs.argtypes = [vp, i32]
+        L.pxh_server_new.restype = vp
+        L.pxh_server_new.argtypes = [i32, ctypes.c_int64]
+        L.pxh_server_free.argtypes = [vp]
+        L.pxh_server_enqueue.restype = ctypes.c_int64
+        L.pxh_server_enqueue.argtypes = [vp, vp, i32, i32, f]
+        L.pxh_server_poll.argtypes = [vp, ctypes.c_int64, i32p, i32p]
+        L.pxh_server_wait.argtypes = [vp, ctypes.c_int64, i32p]
+        L.pxh_server_image.restype = vp
+        L.pxh_server_image.argtypes = [vp, ctypes.c_int64]
+        L.pxh_server_message.restype = ctypes.c_char_p
+        L.pxh_server_message.argtypes = [vp, ctypes.c_int64]
+        d = ctypes.POINTER(ctypes.c_double)
+        L.pxh_server_times.argtypes = [vp, ctypes.c_int64, d, d, d]
+        L.pxh_server_now.restype = ctypes.c_double
+        L.pxh_server_now.argtypes = [vp]
+        L.pxh_server_release.argtypes = [vp, ctypes.c_int64]
         _lib = L
     return _lib
 

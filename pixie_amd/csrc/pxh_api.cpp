@@ -150,6 +150,7 @@ struct Ctx {
   // second row of kLanes words takes the reports of checked renders.
   int32_t* laneErr = nullptr;
   std::set<pxh_scene*> scenes;  // live scenes (their pools are re-validated after an error)
+  std::mutex scenesMu;
   unsigned long long* touched = nullptr;  // N_touch counter of fills
   hipEvent_t ev[16] = {};
   // per-launch kernel timing (pxh_profile_begin / _end)
@@ -162,6 +163,11 @@ struct Ctx {
   std::vector<Timed> evPairs;
   std::map<std::string, std::pair<double, int64_t>> kernelTimes;  // last profile: ms, launches
   std::map<int, uint32_t*> luts;  // per radius: device weight table of the blur kernels
+  std::string arch;               // gcnArchName of the device
+  // Autotuner results: (arch, target w, h, format, calls, commands bucket)
+  // -> (phase-A shape, ms per render)
+  std::map<std::string, std::pair<int, float>> tuned;
+  std::mutex tunedMu;
 };
 Ctx g;
 
@@ -181,6 +187,10 @@ int init() {
   PXH_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0));
   g.maxLds = (size_t)lds;
   g.numCUs = cus;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess) g.arch = prop.gcnArchName;
+  }
   PXH_HIP(hipStreamCreate(&g.stream));
   g.lane[0] = g.stream;
   for (int l = 1; l < kLanes; l++) PXH_HIP(hipStreamCreate(&g.lane[l]));
@@ -965,8 +975,11 @@ int lane_scene_errors(int lane) {
   PXH_HIP(hipMemcpy(&e, g.laneErr + lane, sizeof(e), hipMemcpyDeviceToHost));
   if (!e) return 0;
   PXH_HIP(hipMemset(g.laneErr + lane, 0, sizeof(int32_t)));
-  for (pxh_scene* sc : g.scenes)
-    if (sc->lane == lane) sc->validW = sc->validH = -1;
+  {
+    std::lock_guard<std::mutex> lk(g.scenesMu);
+    for (pxh_scene* sc : g.scenes)
+      if (sc->lane == lane) sc->validW = sc->validH = -1;
+  }
   return fail(kErrCapacity, "a scene render on lane " + std::to_string(lane) +
                                 " ran out of a pool (error bits " + std::to_string(e) +
                                 ") and left its canvas untouched; render it again");
@@ -1352,13 +1365,17 @@ int pxh_blend_pixels(int32_t blend_mode, const uint32_t* backdrop, const uint32_
 
 pxh_scene* pxh_scene_new(void) {
   pxh_scene* sc = new pxh_scene();
+  std::lock_guard<std::mutex> lk(g.scenesMu);
   g.scenes.insert(sc);
   return sc;
 }
 
 void pxh_scene_free(pxh_scene* sc) {
   if (!sc) return;
-  g.scenes.erase(sc);
+  {
+    std::lock_guard<std::mutex> lk(g.scenesMu);
+    g.scenes.erase(sc);
+  }
   if (g.inited) (void)hipStreamSynchronize(g.lane[sc->lane]);
   scene_free_device(sc);
   delete sc;
@@ -1475,14 +1492,34 @@ int pxh_scene_set_shape(pxh_scene* sc, int32_t shape) {
 // 1 when the scene's last render replayed a captured graph.
 int32_t pxh_scene_graph_active(const pxh_scene* sc) { return sc && sc->exec && !sc->graphsOff ? 1 : 0; }
 
-// Times every phase-A launch shape on a scratch image of img's size and
-// keeps the fastest for later renders of this scene.
+// Times every phase-A launch shape on a scratch image of img's size (graph
+// replays, as renders are issued) and keeps the fastest for later renders
+// of this scene; the result is cached per device arch, target and size of
+// the scene.
 int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* shape_out,
                        float* ms_out) {
   PXH_INIT();
   if (!sc) return fail(kErrArg, "invalid scene handle");
   if (int r = check_image(img)) return r;
   if (reps < 1) reps = 1;
+  // a scene of the same shape of work on the same device and target reuses
+  // the shape found before
+  size_t ncmd = sc->cmds.size(), bucket = 0;
+  while (ncmd >>= 1) bucket++;
+  const std::string key = g.arch + "|" + std::to_string(img->w) + "x" + std::to_string(img->h) + "|" +
+                          std::to_string(img->fmt) + "|" + std::to_string(sc->items.size()) + "|" +
+                          std::to_string(bucket);
+  {
+    std::lock_guard<std::mutex> lk(g.tunedMu);
+    auto it = g.tuned.find(key);
+    if (it != g.tuned.end()) {
+      if (sc->rowShape != it->second.first) sc->validW = sc->validH = -1;
+      sc->rowShape = it->second.first;
+      if (shape_out) *shape_out = it->second.first;
+      if (ms_out) *ms_out = it->second.second;
+      return 0;
+    }
+  }
   pxh_image* tmp = img->fmt == PXH_FORMAT_A8 ? pxh_image_new_a8(img->w, img->h) : pxh_image_new(img->w, img->h);
   if (!tmp) return kErrNoMem;
   int best = -1;
@@ -1490,26 +1527,22 @@ int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* sha
   for (int k = 0; k < kNumRowShapes; k++) {
     sc->rowShape = k;
     // validate this shape's first render (it sizes the pools and checks
-    // every counter), then time eager renders
+    // every counter) and capture its graph, then time renders as a caller
+    // issues them (graph replays on lane 0)
     sc->validW = sc->validH = -1;
     int rc = pxh_scene_render_on(sc, tmp, 0);
+    if (!rc) rc = pxh_scene_render_on(sc, tmp, 0);
     float total = 0;
-    for (int r = 0; rc == 0 && r < reps; r++) {
-      if (hipEventRecord(g.ev[14], g.stream) != hipSuccess) rc = kErrDevice;
-      if (!rc) rc = scene_launch(sc, tmp, 0, true);
-      if (!rc && hipEventRecord(g.ev[15], g.stream) != hipSuccess) rc = kErrDevice;
-      float ms = 0;
-      if (!rc && (hipEventSynchronize(g.ev[15]) != hipSuccess ||
-                  hipEventElapsedTime(&ms, g.ev[14], g.ev[15]) != hipSuccess))
-        rc = kErrDevice;
-      total += ms;
-    }
+    if (!rc && hipEventRecord(g.ev[14], g.stream) != hipSuccess) rc = kErrDevice;
+    for (int r = 0; rc == 0 && r < reps; r++) rc = pxh_scene_render_on(sc, tmp, 0);
+    if (!rc && hipEventRecord(g.ev[15], g.stream) != hipSuccess) rc = kErrDevice;
+    if (!rc && (hipEventSynchronize(g.ev[15]) != hipSuccess ||
+                hipEventElapsedTime(&total, g.ev[14], g.ev[15]) != hipSuccess))
+      rc = kErrDevice;
     // a timed render that stopped early (an overflowed pool) did less work:
     // such a shape is not a candidate
-    int32_t err = 0;
-    if (!rc && hipMemcpy(&err, sc->pools.counters + kCntError, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess)
-      rc = kErrDevice;
-    if (rc == kErrCapacity || err) continue;
+    const bool stopped = !rc && lane_scene_errors(0) != 0;
+    if (rc == kErrCapacity || stopped) continue;
     if (rc) {
       pxh_image_free(tmp);
       sc->rowShape = best < 0 ? kDefaultRowShape : best;
@@ -1526,10 +1559,13 @@ int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* sha
     sc->validW = sc->validH = -1;
     return fail(kErrCapacity, "no phase-A launch shape rendered this scene");
   }
-  // the pools were last validated for the last shape tried; the chosen
-  // shape's renders fit them too (pool bounds do not depend on the shape)
   pxh_image_free(tmp);
+  if (sc->rowShape != best) sc->validW = sc->validH = -1;
   sc->rowShape = best;
+  {
+    std::lock_guard<std::mutex> lk(g.tunedMu);
+    g.tuned[key] = {best, bestMs};
+  }
   if (shape_out) *shape_out = best;
   if (ms_out) *ms_out = bestMs;
   return 0;

@@ -26,10 +26,13 @@ Canvases and scene buffers stay resident:
 The plain baseline serves the same arrivals one request at a time, with one
 fillPath / strokePath launch per call.
 """
+import ctypes
 import threading
 import time
 
-from . import Image, PixieError, Scene, lane_synchronize, lanes
+import numpy as np
+
+from . import Image, PixieError, Scene, lane_synchronize, lanes, lib
 
 
 class Request:
@@ -192,6 +195,94 @@ class Server:
             t.join()
         if state["error"] is not None:
             raise state["error"]
+        return results
+
+
+class NativeServer:
+    """The same continuous batching, run by the library's own server
+    (pxh_server_*, the C-ABI served form): worker threads per lane in C++,
+    FIFO admission by lane and arena budget.  Requests are recorded into
+    scenes as they arrive and then enqueued; results are read back and
+    released as they finish."""
+
+    def __init__(self, n_lanes=None, hbm_budget=1 << 30, keep_pixels=False):
+        self.n_lanes = n_lanes or (lanes() - 1)
+        h = lib().pxh_server_new(self.n_lanes, int(hbm_budget))
+        if not h:
+            raise PixieError(6, "pxh_server_new failed (lanes 1..%d, a budget that fits HBM)"
+                             % (lanes() - 1))
+        self._h = ctypes.c_void_p(h)
+        self.budget = int(hbm_budget)
+        self.keep_pixels = keep_pixels
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None:
+            lib().pxh_server_free(h)
+            self._h = None
+
+    def _finish(self, ticket, res, offset, arrival):
+        L = lib()
+        st = ctypes.c_int32()
+        L.pxh_server_wait(self._h, ticket, ctypes.byref(st))
+        e, a, d = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        L.pxh_server_times(self._h, ticket, ctypes.byref(e), ctypes.byref(a), ctypes.byref(d))
+        res.admitted = a.value + offset
+        res.done = d.value + offset
+        res.latency = res.done - arrival
+        if st.value:
+            res.error = PixieError(st.value, L.pxh_server_message(self._h, ticket).decode())
+        elif self.keep_pixels:
+            img = L.pxh_server_image(self._h, ticket)
+            w = L.pxh_image_width(img)
+            h = L.pxh_image_height(img)
+            out = np.empty((h, w, 4), np.uint8)
+            L.pxh_image_download(img, out.ctypes.data)
+            res.pixels = out
+        L.pxh_server_release(self._h, ticket)
+
+    def run(self, requests):
+        """Serves `requests` (arrival times in seconds from the start) and
+        returns one Result per request, in request order.  Each request
+        arrives as a recorded scene, as a C caller would hand it over: the
+        scenes are recorded before the clock starts (recording through
+        ctypes from one Python thread would otherwise bound the arrival
+        rate), so latency runs from arrival to the finished canvas."""
+        L = lib()
+        n = len(requests)
+        results = [Result(i) for i in range(n)]
+        if any(r.side * r.side * 4 > self.budget for r in requests):
+            raise ValueError("a canvas is larger than the HBM budget")
+        recorded = []
+        for r in requests:
+            sc = Scene()
+            _record(sc, r.calls)
+            recorded.append(sc)
+        t0 = time.perf_counter() + 0.01
+        offset = time.perf_counter() - L.pxh_server_now(self._h) - t0
+        pending = []   # (ticket, index, scene): a scene lives until its request is done
+        for i, r in enumerate(requests):
+            wait = t0 + r.arrival - time.perf_counter()
+            if wait > 0:
+                # finish what is done while waiting for the next arrival
+                while pending and wait > 0.0005:
+                    done = ctypes.c_int32()
+                    L.pxh_server_poll(self._h, pending[0][0], ctypes.byref(done), None)
+                    if not done.value:
+                        break
+                    tk, j, _sc = pending.pop(0)
+                    self._finish(tk, results[j], offset, requests[j].arrival)
+                    wait = t0 + r.arrival - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+            sc = recorded[i]
+            tk = L.pxh_server_enqueue(self._h, sc._h, r.side, r.side, float(r.blur))
+            if tk < 0:
+                raise PixieError(-tk, "pxh_server_enqueue failed")
+            results[i].admitted = 0.0
+            pending.append((tk, i, sc))
+        for tk, j, _sc in pending:
+            self._finish(tk, results[j], offset, requests[j].arrival)
         return results
 
 

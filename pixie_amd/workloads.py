@@ -236,6 +236,20 @@ def serve_scene(rng, max_paths=500):
     return cmds, side, blur
 
 
+def poisson_arrivals(pool, n, rate, seed=6):
+    """n requests cycled from `pool` (serve.Request objects) with fresh
+    Poisson arrival times at `rate` per second."""
+    from .serve import Request
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate)) if rate > 0 else 0.0
+        r = pool[i % len(pool)]
+        out.append(Request(r.calls, r.side, r.blur, t))
+    return out
+
+
 def serve_requests(seed=5, n=100, rate=100.0, max_paths=500, max_side=4096):
     """Poisson arrivals at `rate` scenes/s of serving requests (serve_scene),
     as serve.Request objects."""

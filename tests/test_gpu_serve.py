@@ -121,3 +121,35 @@ def test_serve_failing_request_is_answered_alone(gpu):
         for k, i in enumerate([0, 1, 2, 4, 5, 6]):
             assert res[i].error is None
             _same(res[i].pixels, alone[k].pixels, "request %d around a failing one" % i)
+
+
+def test_native_server_alone_vs_mix(gpu):
+    # the library's own server (pxh_server_*, the C-ABI served form): every
+    # result equals the request served alone and the per-call form
+    reqs = WL.serve_requests(seed=26, n=24, rate=0, max_paths=200, max_side=1536)
+    solo = serve.Server(n_lanes=1, hbm_budget=16 << 20, keep_pixels=True)
+    alone = [solo.run([serve.Request(r.calls, r.side, r.blur, 0.0)])[0] for r in reqs]
+    # 12 MiB of arena: canvases wait for memory as well as for lanes
+    mix = serve.NativeServer(hbm_budget=12 << 20, keep_pixels=True)
+    res = mix.run(reqs)
+    for i, r in enumerate(res):
+        assert r.error is None, r.error
+        _same(r.pixels, alone[i].pixels, "request %d in the native mix" % i)
+    assert all(r.latency > 0 for r in res)
+
+
+def test_native_server_failing_request_and_arrivals(gpu):
+    # Poisson arrivals with one failing request (negative blur): it is done
+    # with its status, the others are served and equal their alone images
+    good = WL.serve_requests(seed=27, n=10, rate=300.0, max_paths=80, max_side=700)
+    bad = serve.Request(good[0].calls, 256, -2.0, good[4].arrival)
+    reqs = good[:5] + [bad] + good[5:]
+    solo = serve.Server(n_lanes=1, hbm_budget=8 << 20, keep_pixels=True)
+    alone = [solo.run([serve.Request(r.calls, r.side, r.blur, 0.0)])[0] for r in good]
+    res = serve.NativeServer(n_lanes=2, hbm_budget=32 << 20, keep_pixels=True).run(reqs)
+    assert res[5].error is not None and res[5].error.code == 4
+    for k, i in enumerate([0, 1, 2, 3, 4, 6, 7, 8, 9, 10]):
+        assert res[i].error is None
+        _same(res[i].pixels, alone[k].pixels, "request %d (native, Poisson)" % i)
+    st = serve.latency_stats(res, reqs)
+    assert st["failed"] == 1 and st["scenes"] == 10
