@@ -11,8 +11,12 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+_LIB_PATHS = {"scalar": os.path.join(_HERE, "liboracle.so"),
+              # pixie's SIMD coverage path, -O3 -mavx2: the CPU baseline's
+              # second form, not a checker
+              "avx2": os.path.join(_HERE, "liboracle_avx2.so")}
+_libs = {}
+_variant = "scalar"
 
 NONZERO, EVENODD = 0, 1
 
@@ -40,12 +44,27 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+class simd_mode:
+    """Context manager: the functions below run on liboracle_avx2.so (the
+    model of pixie's AVX2 build) inside the block."""
+
+    def __enter__(self):
+        global _variant
+        self._prev, _variant = _variant, "avx2"
+        return self
+
+    def __exit__(self, *exc):
+        global _variant
+        _variant = self._prev
+        return False
+
+
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+    if _variant not in _libs:
+        path = _LIB_PATHS[_variant]
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         P = ctypes.POINTER
         f32p, i32p, u8p = P(ctypes.c_float), P(ctypes.c_int), P(ctypes.c_uint8)
         L.orc_last_error.restype = ctypes.c_char_p
@@ -93,8 +112,8 @@ def lib():
                                      P(i32p), i32p, P(f32p), i32p, i32p]
         L.orc_fill_overlaps.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_int, i32p]
-        _lib = L
-    return _lib
+        _libs[_variant] = L
+    return _libs[_variant]
 
 
 def _check(rc):

@@ -935,6 +935,139 @@ static void clear_row(canvas_t* c, int64_t x0, int64_t x1, int64_t y) {
   }
 }
 
+#ifdef ORC_SIMD_MODE
+/* ---- AVX2-mode build: pixie's SIMD coverage path (simd/avx2.nim) -------
+ * Built into liboracle_avx2.so only (the CPU baseline of pixie's SIMD
+ * build; the scalar oracle above is the checker).  It follows the reference
+ * where the SIMD path departs from the scalar one:
+ *   A1 the coverage multiply of a vector block floors, c * cov / 255 via
+ *      mulhi and 0x8081 (avx2.nim:418-434), where the scalar path rounds;
+ *   A2 an Overwrite block that is neither all 0 nor all 255 writes every
+ *      pixel, zeroing those of coverage 0 (avx2.nim:469-481);
+ *   A3 a Mask block of coverage 255 is skipped only for an opaque paint, and
+ *      multiplied by the paint's alpha otherwise (avx2.nim:707-724);
+ *   A4 trapezoid edge pixels convert c * area with round-half-even
+ *      (sse2.nim:6-11) instead of round(area * 255) then (c*x+127) div 255;
+ *   A5 each line runs scalar until its pixel address is 32-byte aligned,
+ *      for a 32-byte aligned image base (avx2.nim:443-447 and the like).
+ * Vector blocks are 32 pixels, taken while i < len - 32, as in the
+ * reference.  The reference's Mask loop tests i < len - 16 and can read
+ * coverage past the line; this build stops at len - 32 instead. */
+#include <immintrin.h>
+
+/* floor(c * f / 255) for each channel of 8 pixels; f8 holds f << 8 in each
+ * 16-bit lane */
+static inline __m256i mul_div255_8(__m256i px, __m256i f8) {
+  const __m256i oddMask = _mm256_set1_epi16((short)0xff00);
+  const __m256i div255 = _mm256_set1_epi16((short)0x8081);
+  __m256i even = _mm256_slli_epi16(px, 8);
+  __m256i odd = _mm256_and_si256(px, oddMask);
+  even = _mm256_mulhi_epu16(even, f8);
+  odd = _mm256_mulhi_epu16(odd, f8);
+  even = _mm256_srli_epi16(_mm256_mulhi_epu16(even, div255), 7);
+  odd = _mm256_srli_epi16(_mm256_mulhi_epu16(odd, div255), 7);
+  return _mm256_or_si256(even, _mm256_slli_epi16(odd, 8));
+}
+static inline __m256i lanes_of(__m256i v32) { /* per pixel: v in both 16-bit lanes, << 8 */
+  return _mm256_or_si256(_mm256_slli_epi32(v32, 8), _mm256_slli_epi32(v32, 24));
+}
+static inline __m256i cov8(const uint8_t* cov) {
+  return lanes_of(_mm256_cvtepu8_epi32(_mm_loadl_epi64((const __m128i*)cov)));
+}
+static inline __m256i blend_normal8(__m256i b, __m256i s) {
+  const __m256i k = _mm256_sub_epi32(_mm256_set1_epi32(255), _mm256_srli_epi32(s, 24));
+  return _mm256_add_epi8(s, mul_div255_8(b, lanes_of(k)));
+}
+static inline __m256i blend_mask8(__m256i b, __m256i s) {
+  return mul_div255_8(b, lanes_of(_mm256_srli_epi32(s, 24)));
+}
+
+/* fillCoverage of one antialiased row for Overwrite / Normal / Mask
+ * (blendLineCoverage*Avx2, avx2.nim:436-733).  Returns the pixels written. */
+static int64_t coverage_line_simd(canvas_t* c, int64_t y, int64_t startX, const uint8_t* cov,
+                                  int64_t len, orc_rgbx rgbx, int mode) {
+  uint32_t* line = (uint32_t*)(c->img + (y * c->W + startX) * 4);
+  const uint32_t rgbxU = (uint32_t)rgbx.r | ((uint32_t)rgbx.g << 8) | ((uint32_t)rgbx.b << 16) |
+                         ((uint32_t)rgbx.a << 24);
+  int64_t i = 0, touched = 0;
+#define SCALAR_PX(k)                                                              \
+  do {                                                                            \
+    uint8_t cv_ = cov[k];                                                         \
+    int64_t x_ = startX + (k);                                                    \
+    if (mode == ORC_OVERWRITE) {                                                  \
+      if (cv_ != 0) { px_set(c, x_, y, orc_mul_cov(rgbx, cv_)); touched++; }      \
+    } else if (mode == ORC_MASK) {                                                \
+      if (cv_ != 255) px_blend(c, x_, y, orc_mul_cov(rgbx, cv_));                 \
+    } else if (cv_ != 0) {                                                        \
+      px_blend(c, x_, y, orc_mul_cov(rgbx, cv_));                                 \
+    }                                                                             \
+  } while (0)
+  while (i < len && ((y * c->W + startX + i) & 7) != 0) {
+    SCALAR_PX(i);
+    i++;
+  }
+  const __m256i rgbxVec = _mm256_set1_epi32((int)rgbxU);
+  const __m256i zero = _mm256_setzero_si256(), all255 = _mm256_set1_epi8((char)255);
+  while (i < len - 32) {
+    const __m256i cv = _mm256_loadu_si256((const __m256i*)(cov + i));
+    const int allZero = _mm256_movemask_epi8(_mm256_cmpeq_epi8(cv, zero)) == -1;
+    const int allFull = _mm256_movemask_epi8(_mm256_cmpeq_epi8(cv, all255)) == -1;
+    if (mode == ORC_OVERWRITE) {
+      if (!allZero) {
+        for (int j = 0; j < 32; j += 8) {
+          const __m256i v = allFull ? rgbxVec : mul_div255_8(rgbxVec, cov8(cov + i + j));
+          _mm256_storeu_si256((__m256i*)(line + i + j), v);
+        }
+        touched += 32;
+      }
+    } else if (mode == ORC_MASK) {
+      if (allZero) {
+        for (int j = 0; j < 32; j += 8) _mm256_storeu_si256((__m256i*)(line + i + j), zero);
+      } else if (!(allFull && rgbx.a == 255)) {
+        for (int j = 0; j < 32; j += 8) {
+          const __m256i b = _mm256_loadu_si256((const __m256i*)(line + i + j));
+          _mm256_storeu_si256((__m256i*)(line + i + j),
+                              blend_mask8(b, mul_div255_8(rgbxVec, cov8(cov + i + j))));
+        }
+      }
+      touched += 32;
+    } else {
+      if (allFull && rgbx.a == 255) {
+        for (int j = 0; j < 32; j += 8) _mm256_storeu_si256((__m256i*)(line + i + j), rgbxVec);
+        touched += 32;
+      } else if (!allZero) {
+        for (int j = 0; j < 32; j += 8) {
+          const __m256i b = _mm256_loadu_si256((const __m256i*)(line + i + j));
+          _mm256_storeu_si256((__m256i*)(line + i + j),
+                              blend_normal8(b, mul_div255_8(rgbxVec, cov8(cov + i + j))));
+        }
+        touched += 32;
+      }
+    }
+    i += 32;
+  }
+  for (; i < len; i++) SCALAR_PX(i);
+#undef SCALAR_PX
+  return touched;
+}
+
+/* applyOpacity(M128, float32) (sse2.nim:6-11): c * area per channel,
+ * converted with round-half-even and saturated to 0..255 */
+static orc_rgbx mul_area_simd(orc_rgbx c, float area) {
+  float v[4] = {(float)c.r * area, (float)c.g * area, (float)c.b * area, (float)c.a * area};
+  uint8_t o[4];
+  for (int k = 0; k < 4; k++) {
+    float r = rintf(v[k]);
+    o[k] = !(r > 0.0f) ? 0 : (r > 255.0f ? 255 : (uint8_t)r);
+  }
+  orc_rgbx out = {o[0], o[1], o[2], o[3]};
+  return out;
+}
+#define ORC_MUL_AREA(c, a) mul_area_simd((c), (a))
+#else
+#define ORC_MUL_AREA(c, a) orc_mul_opacity((c), (a))
+#endif
+
 /* fillHits 1540-1591 with walkInteger 1336-1348. */
 static void fill_hits(canvas_t* c, int64_t startX, int64_t y, const hit_t* hits, int nh,
                       int rule, int maskClears, span_t* tmp) {
@@ -1125,7 +1258,7 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
                 float rectArea = inverted ? (prevPenY - (float)y) * run
                                           : ((float)(y + 1) - prevPenY) * run;
                 float area = triangleArea + rectArea + rightRectArea;
-                px_blend(&cv, x, y, orc_mul_opacity(rgbx, area));
+                px_blend(&cv, x, y, ORC_MUL_AREA(rgbx, area));
               }
             }
             { /* right sliver 1811-1847 */
@@ -1150,7 +1283,7 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
                 float rectArea = inverted ? (penY - (float)y) * run
                                           : ((float)(y + 1) - penY) * run;
                 float area = leftRectArea + triangleArea + rectArea;
-                px_blend(&cv, x, y, orc_mul_opacity(rgbx, area));
+                px_blend(&cv, x, y, ORC_MUL_AREA(rgbx, area));
               }
             }
             int64_t fillBegin = leftCoverEnd < 0 ? 0 : (leftCoverEnd > W ? W : leftCoverEnd);
@@ -1230,6 +1363,12 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
     }
 
     if (aa) { /* fillCoverage 1479-1526 */
+#ifdef ORC_SIMD_MODE
+      if (mode == ORC_OVERWRITE || mode == ORC_NORMAL || mode == ORC_MASK) {
+        int64_t t = coverage_line_simd(&cv, y, startX, cov, pathWidth, rgbx, mode);
+        if (st) st->touched += t;
+      } else
+#endif
       for (int64_t i = 0; i < pathWidth; i++) {
         uint8_t cv_ = cov[i];
         int64_t x = startX + i;
