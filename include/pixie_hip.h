@@ -89,6 +89,33 @@ int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const floa
                     float miter_limit, const float* dashes, int32_t ndashes,
                     int32_t blend_mode);
 
+/* Non-solid paints: gradients (paints.nim:12-31, 72-248).  A paint is its
+ * PaintKind ordinal, blend mode, opacity, handle positions (image space:
+ * 2 points for a linear gradient, 3 for radial and angular) and colour stops
+ * (straight float r, g, b, a and the stop position, in order). */
+enum {
+  PXH_LINEAR_GRADIENT = 3,
+  PXH_RADIAL_GRADIENT = 4,
+  PXH_ANGULAR_GRADIENT = 5
+};
+typedef struct {
+  int32_t kind;
+  int32_t blend_mode;
+  float opacity;
+  float handles[6];
+  int32_t nstops;
+  const float* stops; /* nstops x (r, g, b, a, position) */
+} pxh_paint;
+#define PXH_MAX_STOPS 32
+
+/* fillPath with a gradient paint (paths.nim:2115-2142).  Pixie builds a
+ * full-size coverage mask, a full-size gradient image, masks one with the
+ * other and draws the result; here the mask is an 8-bit plane and one pass
+ * evaluates the gradient per pixel, masks it and blends it.  Opacity 0
+ * draws nothing; no stops raises "Gradient must have at least 1 color stop". */
+int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                        const pxh_paint* paint, int32_t winding_rule);
+
 /* fillShapes (paths.nim:1593-1912): flattened, transformed polygons given as
  * xy pairs and polygon start offsets (npolys + 1 entries, in points). */
 int pxh_fill_shapes(pxh_image* img, const float* xy, const int32_t* poly_ofs, int32_t npolys,

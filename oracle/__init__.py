@@ -34,6 +34,29 @@ class Stats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Stop(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_float) for n in ("r", "g", "b", "a", "pos")]
+
+
+class Paint(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("mode", ctypes.c_int), ("opacity", ctypes.c_float),
+                ("h", ctypes.c_float * 6), ("nstops", ctypes.c_int),
+                ("stops", ctypes.POINTER(Stop))]
+
+
+LINEAR, RADIAL, ANGULAR = 3, 4, 5
+
+
+def make_paint(kind, handles, stops, opacity=1.0, mode=0):
+    """kind: LINEAR / RADIAL / ANGULAR; handles: [(x, y), ...]; stops:
+    [((r, g, b, a) floats, position), ...]."""
+    arr = (Stop * max(1, len(stops)))(*[Stop(c[0], c[1], c[2], c[3], p) for c, p in stops])
+    h = (ctypes.c_float * 6)(*[v for xy in handles for v in xy] + [0.0] * (6 - 2 * len(handles)))
+    p = Paint(kind, mode, opacity, h, len(stops), arr)
+    p._keep = arr
+    return p
+
+
 class RGBX(ctypes.Structure):
     _fields_ = [("r", ctypes.c_uint8), ("g", ctypes.c_uint8),
                 ("b", ctypes.c_uint8), ("a", ctypes.c_uint8)]
@@ -112,6 +135,10 @@ def lib():
                                      P(i32p), i32p, P(f32p), i32p, i32p]
         L.orc_fill_overlaps.argtypes = [f32p, ctypes.c_int, f32p, ctypes.c_float, ctypes.c_float,
                                         ctypes.c_int, i32p]
+        L.orc_fill_gradient.argtypes = [u8p, ctypes.c_int, ctypes.c_int, P(Paint)]
+        L.orc_fill_path_paint.argtypes = [u8p, ctypes.c_int, ctypes.c_int, f32p, ctypes.c_int, f32p,
+                                          P(Paint), ctypes.c_int, P(Stats)]
+        L.orc_radial_matrix.argtypes = [P(Paint), f32p]
         _libs[_variant] = L
     return _libs[_variant]
 
@@ -323,6 +350,24 @@ def stroke_path(img, cmds, color, width=1.0, cap=0, join=0, miter_limit=4.0, das
     _check(lib().orc_stroke_path(_u8(img), w, h, cp, len(c), m[1] if m else None, rgbx(color),
                                  width, cap, join, miter_limit, d[1] if d else None,
                                  len(dashes), mode, ctypes.byref(st)))
+    return st.as_dict()
+
+
+def fill_gradient(img, paint):
+    h, w = img.shape[:2]
+    _check(lib().orc_fill_gradient(_u8(img), w, h, ctypes.byref(paint)))
+
+
+def fill_path_paint(img, cmds, paint, winding=NONZERO, mat=None):
+    """fillPath with a gradient paint (paths.nim:2115-2142)."""
+    h, w = img.shape[:2]
+    if isinstance(cmds, str):
+        cmds = parse_path(cmds)
+    c, cp = _f32(cmds)
+    m = _f32(mat)
+    st = Stats()
+    _check(lib().orc_fill_path_paint(_u8(img), w, h, cp, len(c), m[1] if m else None,
+                                     ctypes.byref(paint), winding, ctypes.byref(st)))
     return st.as_dict()
 
 

@@ -144,6 +144,22 @@ int orc_partitions(const float* xy, const int* poly_ofs, int npolys, int w,
 int orc_fill_overlaps(const float* cmds, int ncmds, const float* mat9,
                       float x, float y, int winding, int* result);
 
+/* ---- non-solid paints (paints.nim:12-248, paths.nim:2115-2142) ---- */
+enum { ORC_LINEAR_GRADIENT = 3, ORC_RADIAL_GRADIENT = 4, ORC_ANGULAR_GRADIENT = 5 };
+typedef struct { float r, g, b, a, pos; } orc_stop;  /* straight float colour, position */
+typedef struct {
+  int kind;              /* PaintKind ordinal (paints.nim:4-10) */
+  int mode;              /* blendMode */
+  float opacity;
+  float h[6];            /* gradientHandlePositions: 2 (linear) or 3 points */
+  int nstops;
+  const orc_stop* stops;
+} orc_paint;
+int orc_fill_gradient(uint8_t* img, int w, int h, const orc_paint* p);
+void orc_radial_matrix(const orc_paint* p, float* m9);
+int orc_fill_path_paint(uint8_t* img, int w, int h, const float* cmds, int ncmds,
+                        const float* mat9, const orc_paint* p, int winding, orc_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

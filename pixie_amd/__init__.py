@@ -16,7 +16,8 @@ import numpy as np
 from .paths import Path, mat3, translate, scale, f32  # noqa: F401
 
 __all__ = ["Image", "Path", "PixieError", "parse_path", "lib", "blend_pixels",
-           "BlendMode", "NonZero", "EvenOdd", "mat3", "translate", "scale"]
+           "BlendMode", "NonZero", "EvenOdd", "mat3", "translate", "scale", "Paint",
+           "PaintKind", "ColorStop"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpixie_hip.so")
@@ -31,6 +32,52 @@ class BlendMode:
     Normal, Darken, Multiply, ColorBurn, Lighten, Screen, ColorDodge, Overlay, SoftLight, \
         HardLight, Difference, Exclusion, Hue, Saturation, Color, Luminosity, Mask, Overwrite, \
         SubtractMask, ExcludeMask = range(20)
+
+
+class PaintKind:
+    """pixie's PaintKind ordinals (paints.nim:6-12)."""
+    SolidPaint, ImagePaint, TiledImagePaint, LinearGradientPaint, RadialGradientPaint, \
+        AngularGradientPaint = range(6)
+
+
+class ColorStop:
+    """A gradient stop (paints.nim:24-27): straight float colour (r, g, b, a)
+    in 0..1 and its position."""
+    __slots__ = ("color", "position")
+
+    def __init__(self, color, position):
+        self.color = tuple(float(v) for v in color)
+        self.position = float(position)
+
+
+class Paint:
+    """pixie's Paint (paints.nim:14-31) for the gradient kinds: blend mode,
+    opacity, gradient handle positions (image space; 2 for linear, 3 for
+    radial and angular) and colour stops."""
+
+    def __init__(self, kind, blendMode=0, opacity=1.0, gradientHandlePositions=(),
+                 gradientStops=()):
+        self.kind = kind
+        self.blendMode = blendMode
+        self.opacity = float(opacity)
+        self.gradientHandlePositions = [tuple(p) for p in gradientHandlePositions]
+        self.gradientStops = list(gradientStops)
+
+    def _native(self):
+        hp = [float(v) for xy in self.gradientHandlePositions[:3] for v in xy]
+        hp += [0.0] * (6 - len(hp))
+        stops = np.array([list(s.color) + [s.position] for s in self.gradientStops],
+                         np.float32).reshape(-1, 5)
+        stops = np.ascontiguousarray(stops)
+        p = _PxhPaint(self.kind, self.blendMode, self.opacity, (ctypes.c_float * 6)(*hp),
+                      len(stops), _fp(stops) if len(stops) else None)
+        return p, stops
+
+
+class _PxhPaint(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("blend_mode", ctypes.c_int32),
+                ("opacity", ctypes.c_float), ("handles", ctypes.c_float * 6),
+                ("nstops", ctypes.c_int32), ("stops", ctypes.POINTER(ctypes.c_float))]
 
 
 class PixieError(RuntimeError):
@@ -74,6 +121,7 @@ def lib():
         L.pxh_parse_path.argtypes = [ctypes.c_char_p, P(f32p), i32p]
         L.pxh_free.argtypes = [vp]
         L.pxh_fill_path.argtypes = [vp, f32p, i32, f32p, u32, i32, i32]
+        L.pxh_fill_path_paint.argtypes = [vp, f32p, i32, f32p, P(_PxhPaint), i32]
         L.pxh_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, f32p, i32, i32]
         L.pxh_fill_shapes.argtypes = [vp, f32p, i32p, i32, u32, i32, i32]
         L.pxh_draw.argtypes = [vp, vp, i32, i32, i32]
@@ -254,6 +302,16 @@ class Image:
             _check(lib().pxh_image_fill(self._h, pack(color)))
 
     def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
+        """`color` is premultiplied (r, g, b, a) bytes, or a gradient Paint
+        (whose own blend mode applies, as in pixie)."""
+        if isinstance(color, Paint):
+            c = _cmds(path)
+            m = _mat(transform)
+            p, keep = color._native()
+            _check(lib().pxh_fill_path_paint(self._h, _fp(c), len(c), _fp(m), ctypes.byref(p),
+                                             windingRule))
+            del keep
+            return
         if _paint_skipped(color, blendMode):
             return
         c = _cmds(path)

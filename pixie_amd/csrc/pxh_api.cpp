@@ -1120,6 +1120,100 @@ int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float*
   return fill_shapes_impl(img, sh, rgbx, winding_rule, blend_mode);
 }
 
+// The inverse gradient matrix of a radial paint, (translate(center) *
+// rotate(angle) * scale(distanceX, distanceY)).inverse() (paints.nim:185-195),
+// in float32 with vmath's Mat3 layout m[3 * col + row].
+static void radial_matrix(const float* hp, float* out) {
+  auto len2 = [](float x, float y) { return sqrtf(x * x + y * y); };
+  auto fix = [](float a) {
+    const float PI = 3.14159265358979323846f;
+    while (a > PI) a -= PI * 2.0f;
+    while (a < -PI) a += PI * 2.0f;
+    return a;
+  };
+  const float cx = hp[0], cy = hp[1];
+  const float dX = len2(cx - hp[2], cy - hp[3]), dY = len2(cx - hp[4], cy - hp[5]);
+  const float ln = len2(cx - hp[2], cy - hp[3]);
+  const float ga = fix(atan2f((cy - hp[3]) / ln, (cx - hp[2]) / ln));
+  const float c = cosf(ga), s = sinf(ga);
+  const float T[9] = {1, 0, 0, 0, 1, 0, cx, cy, 1};
+  const float R[9] = {c, s, 0, -s, c, 0, 0, 0, 1};
+  const float S[9] = {dX, 0, 0, 0, dY, 0, 0, 0, 1};
+  auto mul = [](const float* a, const float* b, float* o) {
+    float r[9];
+    for (int col = 0; col < 3; col++)
+      for (int row = 0; row < 3; row++)
+        r[3 * col + row] = a[row] * b[3 * col] + a[3 + row] * b[3 * col + 1] + a[6 + row] * b[3 * col + 2];
+    memcpy(o, r, sizeof(r));
+  };
+  float TR[9], M[9];
+  mul(T, R, TR);
+  mul(TR, S, M);
+  const float a = M[0], b = M[3], cc = M[6], d = M[1], e = M[4], f = M[7], g2 = M[2], h = M[5], i = M[8];
+  const float A = e * i - f * h, B = -(d * i - f * g2), C = d * h - e * g2;
+  const float inv = 1.0f / (a * A + b * B + cc * C);
+  out[0] = A * inv;
+  out[1] = B * inv;
+  out[2] = C * inv;
+  out[3] = -(b * i - cc * h) * inv;
+  out[4] = (a * i - cc * g2) * inv;
+  out[5] = -(a * h - b * g2) * inv;
+  out[6] = (b * f - cc * e) * inv;
+  out[7] = -(a * f - cc * d) * inv;
+  out[8] = (a * e - b * d) * inv;
+}
+
+int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                        const pxh_paint* paint, int32_t winding_rule) {
+  PXH_INIT();
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "paints need an RGBA8 image");
+  if (!paint) return fail(kErrArg, "invalid paint");
+  if (paint->kind < PXH_LINEAR_GRADIENT || paint->kind > PXH_ANGULAR_GRADIENT)
+    return fail(kErrArg, "Paint must be a gradient");
+  if (paint->blend_mode < 0 || paint->blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (paint->nstops <= 0 || !paint->stops) return fail(kErrArg, "Gradient must have at least 1 color stop");
+  if (paint->nstops > PXH_MAX_STOPS) return fail(kErrCapacity, "too many gradient stops");
+  float opacity = paint->opacity;  // paths.nim:2101: clamp(opacity, 0, 1)
+  if (!(opacity > 0.0f)) opacity = 0.0f;
+  if (opacity > 1.0f) opacity = 1.0f;
+  if (opacity == 0.0f) return 0;
+  // the path's coverage: a white NonZero / EvenOdd fill onto a transparent
+  // 8-bit plane, which holds the alpha pixie's RGBA mask would hold
+  const size_t npx = (size_t)img->w * img->h;
+  if (int e = g.plane.ensure(npx)) return fail(e, "mask allocation failed");
+  PXH_HIP(hipMemsetAsync(g.plane.p, 0, npx, g.stream));
+  pxh_image* mask = pxh_image_view(g.plane.p, img->w, img->h, PXH_FORMAT_A8);
+  if (!mask) return kErrArg;
+  int r = pxh_fill_path(mask, cmds, ncmds, mat9, 0xFFFFFFFFu, winding_rule, kNormal);
+  pxh_image_free(mask);
+  if (r) return r;
+  PaintArgs a = {};
+  a.dst = (uint32_t*)img->dev;
+  a.mask = (const uint8_t*)g.plane.p;
+  a.w = img->w;
+  a.h = img->h;
+  a.kind = paint->kind;
+  a.mode = paint->blend_mode;
+  a.useOpacity = opacity != 1.0f;
+  a.o255 = (uint16_t)nim_f2u32(roundf(255.0f * opacity));  // images.nim:263
+  memcpy(a.hx, paint->handles, sizeof(a.hx));
+  if (paint->kind == PXH_RADIAL_GRADIENT) radial_matrix(paint->handles, a.m);
+  if (paint->kind == PXH_ANGULAR_GRADIENT) {
+    const float PI = 3.14159265358979323846f;
+    const float vx = paint->handles[2] - paint->handles[0], vy = paint->handles[3] - paint->handles[1];
+    const float l = sqrtf(vx * vx + vy * vy);
+    float ga = atan2f(vy / l, vx / l);
+    while (ga > PI) ga -= PI * 2.0f;
+    while (ga < -PI) ga += PI * 2.0f;
+    a.ga = ga;
+  }
+  a.nstops = paint->nstops;
+  memcpy(a.stops, paint->stops, sizeof(float) * 5 * (size_t)paint->nstops);
+  if (launch_paint(a, g.stream) != 0) return fail(kErrDevice, "paint launch failed");
+  return 0;
+}
+
 int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
                     uint32_t rgbx, float stroke_width, int32_t line_cap, int32_t line_join,
                     float miter_limit, const float* dashes, int32_t ndashes, int32_t blend_mode) {

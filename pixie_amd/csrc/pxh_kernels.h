@@ -59,6 +59,25 @@ struct SpreadArgs {
   int32_t dx, dy;        // read src translated by (dx, dy)
 };
 
+// Gradient paint over the whole canvas (fillPath's route for non-solid
+// paints, paths.nim:2115-2142): dst = blend(mode, dst, blendMask(gradient,
+// applyOpacity(mask))), the mask being the coverage of a white fill.
+constexpr int kMaxStops = 32;
+struct PaintArgs {
+  uint32_t* dst;
+  const uint8_t* mask;      // A8 coverage of the path
+  int32_t w, h;
+  int32_t kind, mode;       // PaintKind ordinal (3..5), blend mode
+  int32_t useOpacity;       // opacity != 1: mask *= o255 / 255
+  uint32_t o255;            // round(255 * opacity)
+  float hx[6];              // gradient handles
+  float m[9];               // radial: the inverse gradient matrix
+  float ga;                 // angular: fixAngle(angle(normalize(edge - center)))
+  int32_t nstops;
+  float stops[kMaxStops * 5];  // r, g, b, a (straight), position
+};
+int launch_paint(const PaintArgs& a, hipStream_t s);
+
 int launch_fill_rows(const FillArgs& a, int grid, size_t ldsBytes, bool a8, hipStream_t s);
 int set_fill_lds_limit(size_t bytes);
 int launch_composite(const CompositeArgs& a, bool srcA8, hipStream_t s);

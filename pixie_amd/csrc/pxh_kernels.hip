@@ -88,6 +88,85 @@ int set_fill_lds_limit(size_t bytes) {
 }
 
 // ---------------------------------------------------------------------------
+// gradient paints (paints.nim:72-248) through fillPath's mask route
+// (paths.nim:2115-2142), one pass: each pixel takes the path's coverage from
+// the A8 mask, the gradient colour at its integer coordinates, masks the
+// colour with the coverage as fill.draw(mask, MaskBlend) does, and blends it
+// as image.draw(fill, blendMode) does.  Float math is pixie's float32 order.
+// ---------------------------------------------------------------------------
+// gradientColor (paints.nim:72-97), then Color -> ColorRGBX (round to 8 bits,
+// premultiply with (c * a + 127) div 255)
+__device__ uint32_t gradient_color(const PaintArgs& a, float t) {
+  int index = -1;
+  for (int i = 0; i < a.nstops; i++) {
+    const float pos = a.stops[5 * i + 4];
+    if (pos < t) index = i;
+    if (pos > t) break;
+  }
+  float c[4];
+  if (index == -1 || index + 1 >= a.nstops) {
+    const int k = index == -1 ? 0 : index;
+    for (int j = 0; j < 4; j++) c[j] = a.stops[5 * k + j];
+  } else {
+    const float* s1 = a.stops + 5 * index;
+    const float* s2 = a.stops + 5 * (index + 1);
+    const float v = (t - s1[4]) / (s2[4] - s1[4]);
+    for (int j = 0; j < 4; j++) c[j] = s1[j] * (1.0f - v) + s2[j] * v;
+  }
+  const FColor f = {c[0], c[1], c[2], c[3]};
+  return from_fcolor(f);
+}
+
+__device__ __forceinline__ float fix_angle_p(float x) {
+  const float PI = 3.14159265358979323846f;
+  while (x > PI) x -= PI * 2.0f;
+  while (x < -PI) x += PI * 2.0f;
+  return x;
+}
+
+__global__ void __launch_bounds__(256) paint_kernel(PaintArgs a) {
+  const int32_t y = blockIdx.y;
+  const int32_t x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= a.w) return;
+  const size_t i = (size_t)y * a.w + x;
+  uint32_t m = a.mask[i];
+  if (a.useOpacity) m = m * a.o255 / 255u;  // applyOpacity on the mask (images.nim:261-277)
+  float t;
+  const float fx = (float)x, fy = (float)y;
+  if (a.kind == 3) {  // linear: toLineSpace (paints.nim:113-118)
+    const float dx = a.hx[2] - a.hx[0], dy = a.hx[3] - a.hx[1];
+    const float det = dx * dx + dy * dy;
+    t = (dy * (fy - a.hx[1]) + dx * (fx - a.hx[0])) / det;
+  } else if (a.kind == 4) {  // radial: |inverse gradient matrix * xy|
+    const float px = a.m[0] * fx + a.m[3] * fy + a.m[6];
+    const float py = a.m[1] * fx + a.m[4] * fy + a.m[7];
+    t = sqrtf(px * px + py * py);
+  } else {  // angular (paints.nim:232-236)
+    const float PI = 3.14159265358979323846f;
+    const float vx = fx - a.hx[0], vy = fy - a.hx[1];
+    const float l = sqrtf(vx * vx + vy * vy);
+    const float angle = atan2f(vy / l, vx / l);
+    t = fix_angle_p(angle + a.ga + PI / 2.0f) / 2.0f / PI + 0.5f;
+  }
+  const uint32_t g = gradient_color(a, t);
+  // fill.draw(mask, MaskBlend): every channel times the mask's alpha
+  const uint32_t src = pack(ch_r(g) * m / 255u, ch_g(g) * m / 255u, ch_b(g) * m / 255u, ch_a(g) * m / 255u);
+  // a transparent source leaves the backdrop of these modes bit for bit
+  const int md = a.mode;
+  const bool keepOnZero = md == kNormal || md == kDarken || md == kMultiply || md == kLighten ||
+                          md == kScreen || md == kOverlay || md == kHardLight || md == kDifference ||
+                          md == kExclusion;
+  if (src == 0u && keepOnZero) return;
+  a.dst[i] = blend(md, a.dst[i], src);
+}
+
+int launch_paint(const PaintArgs& a, hipStream_t s) {
+  dim3 g((a.w + 255) / 256, a.h);
+  hipLaunchKernelGGL(paint_kernel, g, dim3(256), 0, s, a);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
+// ---------------------------------------------------------------------------
 // composite (blendRect): each thread handles 4 consecutive destination pixels
 // ---------------------------------------------------------------------------
 template <bool kSrcA8>
