@@ -40,20 +40,34 @@ class Stop(ctypes.Structure):
 
 class Paint(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("mode", ctypes.c_int), ("opacity", ctypes.c_float),
-                ("h", ctypes.c_float * 6), ("nstops", ctypes.c_int),
-                ("stops", ctypes.POINTER(Stop))]
+                ("nhandles", ctypes.c_int), ("h", ctypes.c_float * 6), ("nstops", ctypes.c_int),
+                ("stops", ctypes.POINTER(Stop)), ("image", ctypes.POINTER(ctypes.c_uint8)),
+                ("iw", ctypes.c_int), ("ih", ctypes.c_int), ("image_mat", ctypes.c_float * 9)]
 
 
-LINEAR, RADIAL, ANGULAR = 3, 4, 5
+IMAGE, TILED, LINEAR, RADIAL, ANGULAR = 1, 2, 3, 4, 5
 
 
 def make_paint(kind, handles, stops, opacity=1.0, mode=0):
     """kind: LINEAR / RADIAL / ANGULAR; handles: [(x, y), ...]; stops:
     [((r, g, b, a) floats, position), ...]."""
     arr = (Stop * max(1, len(stops)))(*[Stop(c[0], c[1], c[2], c[3], p) for c, p in stops])
-    h = (ctypes.c_float * 6)(*[v for xy in handles for v in xy] + [0.0] * (6 - 2 * len(handles)))
-    p = Paint(kind, mode, opacity, h, len(stops), arr)
+    hv = [float(v) for xy in handles[:3] for v in xy]
+    h = (ctypes.c_float * 6)(*(hv + [0.0] * (6 - len(hv))))
+    p = Paint(kind, mode, opacity, len(handles), h, len(stops), arr)
     p._keep = arr
+    return p
+
+
+def make_image_paint(image, mat, tiled=False, opacity=1.0, mode=0):
+    """ImagePaint / TiledImagePaint over a premultiplied RGBA8 array, with
+    imageMat `mat` (9 floats, m[3 * col + row])."""
+    img = np.ascontiguousarray(image, dtype=np.uint8)
+    ih, iw = img.shape[:2]
+    m = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(mat, np.float32).ravel()])
+    p = Paint(TILED if tiled else IMAGE, mode, opacity, 0, (ctypes.c_float * 6)(), 0, None,
+              _u8(img), iw, ih, m)
+    p._keep = img
     return p
 
 
@@ -139,6 +153,18 @@ def lib():
         L.orc_fill_path_paint.argtypes = [u8p, ctypes.c_int, ctypes.c_int, f32p, ctypes.c_int, f32p,
                                           P(Paint), ctypes.c_int, P(Stats)]
         L.orc_radial_matrix.argtypes = [P(Paint), f32p]
+        L.orc_minify_size.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, i32p]
+        L.orc_minify_by2.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, u8p]
+        L.orc_magnify_by2.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_int, u8p]
+        L.orc_mix.restype = RGBX
+        L.orc_mix.argtypes = [RGBX, RGBX, ctypes.c_float]
+        L.orc_get_rgba_smooth.restype = RGBX
+        L.orc_get_rgba_smooth.argtypes = [u8p, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_int]
+        L.orc_draw.argtypes = [u8p, ctypes.c_int, ctypes.c_int, u8p, ctypes.c_int, ctypes.c_int,
+                               f32p, ctypes.c_int]
+        L.orc_draw_tiled.argtypes = [u8p, ctypes.c_int, ctypes.c_int, u8p, ctypes.c_int,
+                                     ctypes.c_int, f32p, ctypes.c_int]
         _libs[_variant] = L
     return _libs[_variant]
 
@@ -258,6 +284,57 @@ def draw(a, b, dx=0, dy=0, mode=0):
     bh, bw = b.shape[:2]
     b = np.ascontiguousarray(b)
     lib().orc_draw_rect(_u8(a), aw, ah, _u8(b), bw, bh, int(dx), int(dy), mode)
+
+
+def minify_by2(img, power=1):
+    """minifyBy2 (images.nim:168-228): a new array."""
+    h, w = img.shape[:2]
+    ow, oh = ctypes.c_int(), ctypes.c_int()
+    lib().orc_minify_size(w, h, max(power, 0), ctypes.byref(ow), ctypes.byref(oh))
+    out = np.zeros((oh.value, ow.value, 4), np.uint8)
+    src = np.ascontiguousarray(img, dtype=np.uint8)
+    _check(lib().orc_minify_by2(_u8(src), w, h, power, _u8(out)))
+    return out
+
+
+def magnify_by2(img, power=1):
+    """magnifyBy2 (images.nim:230-259): a new array."""
+    h, w = img.shape[:2]
+    s = 1 << max(power, 0)
+    out = np.zeros((h * s, w * s, 4), np.uint8)
+    src = np.ascontiguousarray(img, dtype=np.uint8)
+    _check(lib().orc_magnify_by2(_u8(src), w, h, power, _u8(out)))
+    return out
+
+
+def mix(a, b, t):
+    r = lib().orc_mix(rgbx(a), rgbx(b), t)
+    return (r.r, r.g, r.b, r.a)
+
+
+def get_rgba_smooth(img, x, y, wrapped=False):
+    h, w = img.shape[:2]
+    src = np.ascontiguousarray(img, dtype=np.uint8)
+    r = lib().orc_get_rgba_smooth(_u8(src), w, h, x, y, int(wrapped))
+    return (r.r, r.g, r.b, r.a)
+
+
+def draw_transform(a, b, mat, mode=0):
+    """draw(a, b, transform, blendMode) (images.nim:636-678)."""
+    ah, aw = a.shape[:2]
+    bh, bw = b.shape[:2]
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    m = _f32(np.asarray(mat, np.float32).ravel())
+    _check(lib().orc_draw(_u8(a), aw, ah, _u8(b), bw, bh, m[1], mode))
+
+
+def draw_tiled(a, b, mat, mode=0):
+    """drawTiled (images.nim:680-683)."""
+    ah, aw = a.shape[:2]
+    bh, bw = b.shape[:2]
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    m = _f32(np.asarray(mat, np.float32).ravel())
+    _check(lib().orc_draw_tiled(_u8(a), aw, ah, _u8(b), bw, bh, m[1], mode))
 
 
 def diff_score(master, img):

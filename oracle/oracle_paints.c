@@ -38,38 +38,6 @@ static float pfix_angle(float a) {                        /* vmath fixAngle */
   return a;
 }
 
-/* Mat3 with m[3 * col + row], as the matrices of the C ABI */
-static void m3_mul(const float* a, const float* b, float* o) {
-  float r[9];
-  for (int c = 0; c < 3; c++)
-    for (int rr = 0; rr < 3; rr++)
-      r[3 * c + rr] = a[0 * 3 + rr] * b[3 * c + 0] + a[1 * 3 + rr] * b[3 * c + 1] + a[2 * 3 + rr] * b[3 * c + 2];
-  memcpy(o, r, sizeof(r));
-}
-static void m3_inverse(const float* m, float* o) {
-  /* cofactor expansion, m[col][row] */
-  const float a = m[0], b = m[3], c = m[6];
-  const float d = m[1], e = m[4], f = m[7];
-  const float g = m[2], h = m[5], i = m[8];
-  const float A = e * i - f * h, B = -(d * i - f * g), C = d * h - e * g;
-  const float det = a * A + b * B + c * C;
-  const float inv = 1.0f / det;
-  float r[9];
-  r[0] = A * inv;
-  r[1] = B * inv;
-  r[2] = C * inv;
-  r[3] = -(b * i - c * h) * inv;
-  r[4] = (a * i - c * g) * inv;
-  r[5] = -(a * h - b * g) * inv;
-  r[6] = (b * f - c * e) * inv;
-  r[7] = -(a * f - c * d) * inv;
-  r[8] = (a * e - b * d) * inv;
-  memcpy(o, r, sizeof(r));
-}
-static pv2 m3_apply(const float* m, pv2 p) {
-  return pv(m[0] * p.x + m[3] * p.y + m[6], m[1] * p.x + m[4] * p.y + m[7]);
-}
-
 /* gradientColor (paints.nim:72-97): the last stop below t and the next one,
  * mixed linearly in straight float colour; alpha times the opacity. */
 static orc_rgbx gradient_color(const orc_paint* p, float t) {
@@ -97,6 +65,14 @@ static orc_rgbx gradient_color(const orc_paint* p, float t) {
 /* fillGradient (paints.nim:99-248) with the paint's opacity (fillPath calls
  * it at opacity 1 and applies the opacity to the mask instead). */
 int orc_fill_gradient(uint8_t* img, int w, int h, const orc_paint* p) {
+  if (p->kind == ORC_LINEAR_GRADIENT && p->nhandles != 2)
+    return orc_fail(ORC_ERR_ARG, "Linear gradient requires 2 handles");
+  if (p->kind == ORC_RADIAL_GRADIENT && p->nhandles != 3)
+    return orc_fail(ORC_ERR_ARG, "Radial gradient requires 3 handles");
+  if (p->kind == ORC_ANGULAR_GRADIENT && p->nhandles != 3)
+    return orc_fail(ORC_ERR_ARG, "Angular gradient requires 2 handles"); /* sic: 3 are read */
+  if (p->kind < ORC_LINEAR_GRADIENT || p->kind > ORC_ANGULAR_GRADIENT)
+    return orc_fail(ORC_ERR_ARG, "Paint must be a gradient");
   if (p->nstops == 0) return orc_fail(ORC_ERR_ARG, "Gradient must have at least 1 color stop");
   orc_paint q = *p;
   q.opacity = q.opacity < 0 ? 0 : (q.opacity > 1 ? 1 : q.opacity);
@@ -121,7 +97,9 @@ int orc_fill_gradient(uint8_t* img, int w, int h, const orc_paint* p) {
     orc_radial_matrix(p, m);
     for (int y = 0; y < h; y++)
       for (int x = 0; x < w; x++) {
-        const float t = plen(m3_apply(m, pv((float)x, (float)y)));
+        float px, py;
+        orc_m3_apply(m, (float)x, (float)y, &px, &py);
+        const float t = plen(pv(px, py));
         orc_rgbx c = gradient_color(&q, t);
         uint8_t* o = img + ((size_t)y * w + x) * 4;
         o[0] = c.r; o[1] = c.g; o[2] = c.b; o[3] = c.a;
@@ -155,12 +133,13 @@ void orc_radial_matrix(const orc_paint* p, float* m) {
   const float R[9] = {c, s, 0, -s, c, 0, 0, 0, 1};
   const float S[9] = {dx, 0, 0, 0, dy, 0, 0, 0, 1};
   float TR[9], TRS[9];
-  m3_mul(T, R, TR);
-  m3_mul(TR, S, TRS);
-  m3_inverse(TRS, m);
+  orc_m3_mul(T, R, TR);
+  orc_m3_mul(TR, S, TRS);
+  orc_m3_inverse(TRS, m);
 }
 
-/* fillPath with a gradient paint (paths.nim:2115-2142). */
+/* fillPath with a non-solid paint (paths.nim:2115-2142): image, tiled
+ * image or gradient. */
 int orc_fill_path_paint(uint8_t* img, int w, int h, const float* cmds, int n, const float* mat9,
                         const orc_paint* p, int winding, orc_stats* st) {
   float opacity = p->opacity < 0 ? 0 : (p->opacity > 1 ? 1 : p->opacity);
@@ -178,7 +157,12 @@ int orc_fill_path_paint(uint8_t* img, int w, int h, const float* cmds, int n, co
   if (rc == ORC_OK) {
     orc_paint q = *p;
     q.opacity = 1.0f;
-    rc = orc_fill_gradient(fill, w, h, &q);
+    if (p->kind == ORC_IMAGE_PAINT)
+      rc = orc_draw(fill, w, h, p->image, p->iw, p->ih, p->image_mat, ORC_NORMAL);
+    else if (p->kind == ORC_TILED_IMAGE_PAINT)
+      rc = orc_draw_tiled(fill, w, h, p->image, p->iw, p->ih, p->image_mat, ORC_NORMAL);
+    else
+      rc = orc_fill_gradient(fill, w, h, &q);
   }
   if (rc == ORC_OK) {
     if (opacity != 1.0f) orc_apply_opacity(mask, w, h, opacity);

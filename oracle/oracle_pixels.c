@@ -83,10 +83,14 @@ void orc_to_straight_alpha(uint8_t* d, int64_t n) {
   for (int64_t i = 0; i < n; i++) {
     uint8_t* p = d + 4 * i;
     uint32_t a = p[3];
-    float mult = a > 0 ? (255.0f / (float)a) : 0.0f;
+    /* straightAlphaTable is a compile-time constant: Nim's VM evaluates
+     * its float32 arithmetic in float64, so the table holds the float64
+     * results (101 * 255 / 202 = 127.5000...01 rounds to 128, 39 * 255 /
+     * 234 = 42.4999...9 to 42; float32 gives 127 and 43) */
+    double mult = a > 0 ? (255.0 / (double)a) : 0.0;
     for (int k = 0; k < 3; k++) {
-      float v = roundf((float)p[k] * mult);
-      if (v > 255.0f) v = 255.0f;
+      double v = round((double)p[k] * mult);
+      if (v > 255.0) v = 255.0;
       p[k] = (uint8_t)v;
     }
   }

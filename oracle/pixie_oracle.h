@@ -144,16 +144,37 @@ int orc_partitions(const float* xy, const int* poly_ofs, int npolys, int w,
 int orc_fill_overlaps(const float* cmds, int ncmds, const float* mat9,
                       float x, float y, int winding, int* result);
 
+/* ---- transformed draws (images.nim:168-259, 367-445, 531-683) ---- */
+orc_rgbx orc_mix(orc_rgbx a, orc_rgbx b, float t);   /* common.nim:59-65 */
+void orc_minify_size(int w, int h, int power, int* ow, int* oh);
+int orc_minify_by2(const uint8_t* src, int w, int h, int power, uint8_t* dst);
+int orc_magnify_by2(const uint8_t* src, int w, int h, int power, uint8_t* dst);
+orc_rgbx orc_get_rgba_smooth(const uint8_t* img, int w, int h, float x, float y, int wrapped);
+/* draw(a, b, transform, blendMode) and drawTiled (drawCorrect, tiled) */
+int orc_draw(uint8_t* a, int aw, int ah, const uint8_t* b, int bw, int bh, const float* mat9,
+             int mode);
+int orc_draw_correct(uint8_t* a, int aw, int ah, const uint8_t* b, int bw, int bh,
+                     const float* mat9, int mode, int tiled);
+int orc_draw_tiled(uint8_t* a, int aw, int ah, const uint8_t* b, int bw, int bh, const float* mat9,
+                   int mode);
+
 /* ---- non-solid paints (paints.nim:12-248, paths.nim:2115-2142) ---- */
-enum { ORC_LINEAR_GRADIENT = 3, ORC_RADIAL_GRADIENT = 4, ORC_ANGULAR_GRADIENT = 5 };
+enum {
+  ORC_IMAGE_PAINT = 1, ORC_TILED_IMAGE_PAINT = 2,
+  ORC_LINEAR_GRADIENT = 3, ORC_RADIAL_GRADIENT = 4, ORC_ANGULAR_GRADIENT = 5
+};
 typedef struct { float r, g, b, a, pos; } orc_stop;  /* straight float colour, position */
 typedef struct {
   int kind;              /* PaintKind ordinal (paints.nim:4-10) */
   int mode;              /* blendMode */
   float opacity;
+  int nhandles;          /* gradientHandlePositions.len */
   float h[6];            /* gradientHandlePositions: 2 (linear) or 3 points */
   int nstops;
   const orc_stop* stops;
+  const uint8_t* image;  /* ImagePaint / TiledImagePaint: premultiplied RGBA8 */
+  int iw, ih;
+  float image_mat[9];    /* imageMat */
 } orc_paint;
 int orc_fill_gradient(uint8_t* img, int w, int h, const orc_paint* p);
 void orc_radial_matrix(const orc_paint* p, float* m9);
