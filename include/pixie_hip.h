@@ -89,11 +89,30 @@ int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const floa
                     float miter_limit, const float* dashes, int32_t ndashes,
                     int32_t blend_mode);
 
-/* Non-solid paints: gradients (paints.nim:12-31, 72-248).  A paint is its
- * PaintKind ordinal, blend mode, opacity, handle positions (image space:
- * 2 points for a linear gradient, 3 for radial and angular) and colour stops
- * (straight float r, g, b, a and the stop position, in order). */
+/* Transformed image draws.
+ * draw(a, b, transform, blendMode) (images.nim:636-678): minifyBy2 /
+ * magnifyBy2 until the sampling step is within (1/2, 2), then the integer
+ * composite when the result is a whole-pixel translation, else drawSmooth's
+ * bilinear samples (images.nim:531-634).  drawTiled (images.nim:680-683):
+ * drawCorrect with wrapped samples over every pixel.  Both images RGBA8 and
+ * distinct.  pixie loops forever on a transform whose sampling step is 0 or
+ * infinite; here that returns PXH_ERR_ARG. */
+int pxh_draw_transform(pxh_image* a, const pxh_image* b, const float* mat9, int32_t blend_mode);
+int pxh_draw_tiled(pxh_image* a, const pxh_image* b, const float* mat9, int32_t blend_mode);
+/* minifyBy2 / magnifyBy2 (images.nim:168-259): new images (NULL on error,
+ * pxh_last_error() says why). */
+pxh_image* pxh_minify_by2(const pxh_image* img, int32_t power);
+pxh_image* pxh_magnify_by2(const pxh_image* img, int32_t power);
+
+/* Non-solid paints (paints.nim:4-29): the PaintKind ordinal, blend mode and
+ * opacity; image paints (1, 2) carry the image and imageMat, gradients
+ * (3-5) the handle positions (image space: 2 points for linear, 3 for
+ * radial and angular) and colour stops (straight float r, g, b, a and the
+ * stop position, in order).  Solid paints go through pxh_fill_path /
+ * pxh_stroke_path with their colour premultiplied. */
 enum {
+  PXH_IMAGE_PAINT = 1,
+  PXH_TILED_IMAGE_PAINT = 2,
   PXH_LINEAR_GRADIENT = 3,
   PXH_RADIAL_GRADIENT = 4,
   PXH_ANGULAR_GRADIENT = 5
@@ -102,19 +121,30 @@ typedef struct {
   int32_t kind;
   int32_t blend_mode;
   float opacity;
+  int32_t nhandles;        /* gradientHandlePositions.len */
   float handles[6];
   int32_t nstops;
-  const float* stops; /* nstops x (r, g, b, a, position) */
+  const float* stops;      /* nstops x (r, g, b, a, position) */
+  const pxh_image* image;  /* ImagePaint / TiledImagePaint, RGBA8 */
+  float image_mat[9];      /* imageMat, m[3 * col + row] */
 } pxh_paint;
 #define PXH_MAX_STOPS 32
 
-/* fillPath with a gradient paint (paths.nim:2115-2142).  Pixie builds a
- * full-size coverage mask, a full-size gradient image, masks one with the
- * other and draws the result; here the mask is an 8-bit plane and one pass
- * evaluates the gradient per pixel, masks it and blends it.  Opacity 0
- * draws nothing; no stops raises "Gradient must have at least 1 color stop". */
+/* fillPath / strokePath with a non-solid paint (paths.nim:2115-2142,
+ * 2176-2214).  Pixie builds a full-size coverage mask, a full-size fill
+ * (the image drawn with imageMat, tiled or not, or the gradient), masks one
+ * with the other and draws the result; here the mask is an 8-bit plane and
+ * one pass masks the fill (a gradient evaluated per pixel, or the drawn
+ * image) and blends it.  Opacity 0 draws nothing; gradient errors are
+ * pixie's ("Gradient must have at least 1 color stop", "Linear gradient
+ * requires 2 handles", ...), raised after the mask is built and before the
+ * image changes. */
 int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
                         const pxh_paint* paint, int32_t winding_rule);
+int pxh_stroke_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                          const pxh_paint* paint, float stroke_width, int32_t line_cap,
+                          int32_t line_join, float miter_limit, const float* dashes,
+                          int32_t ndashes);
 
 /* fillShapes (paths.nim:1593-1912): flattened, transformed polygons given as
  * xy pairs and polygon start offsets (npolys + 1 entries, in points). */

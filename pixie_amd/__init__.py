@@ -9,6 +9,7 @@ There is no CPU fallback: importing this package fails if the native
 library has not been built, and every call reports device errors.
 """
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -51,17 +52,37 @@ class ColorStop:
 
 
 class Paint:
-    """pixie's Paint (paints.nim:14-31) for the gradient kinds: blend mode,
-    opacity, gradient handle positions (image space; 2 for linear, 3 for
-    radial and angular) and colour stops."""
+    """pixie's Paint (paints.nim:12-24) for the non-solid kinds: blend mode,
+    opacity; `image` (an Image) and `imageMat` for ImagePaint /
+    TiledImagePaint; gradient handle positions (image space; 2 for linear, 3
+    for radial and angular) and colour stops for the gradients."""
 
     def __init__(self, kind, blendMode=0, opacity=1.0, gradientHandlePositions=(),
-                 gradientStops=()):
+                 gradientStops=(), image=None, imageMat=None, color=(0.0, 0.0, 0.0, 0.0)):
         self.kind = kind
+        self.color = tuple(float(v) for v in color)  # SolidPaint: straight float rgba
         self.blendMode = blendMode
         self.opacity = float(opacity)
         self.gradientHandlePositions = [tuple(p) for p in gradientHandlePositions]
         self.gradientStops = list(gradientStops)
+        self.image = image
+        self.imageMat = mat3() if imageMat is None else np.asarray(imageMat, np.float32)
+
+    def solid_rgbx(self):
+        """A SolidPaint's colour as fillPath hands it to fillShapes
+        (paths.nim:2101-2112): opacity clamped and multiplied into alpha, then
+        converted to premultiplied bytes (round to 8 bits, then (c * a + 127)
+        div 255)."""
+        o = min(max(self.opacity, 0.0), 1.0)
+        r, g, b, a = self.color
+        a = float(np.float32(a) * np.float32(o))
+        # roundf of the float32 product: half away from zero
+        byte = [int(math.floor(float(np.float32(min(max(v, 0.0), 1.0)) * np.float32(255)) + 0.5))
+                for v in (r, g, b, a)]
+        r, g, b, a = byte
+        if a != 255:
+            r, g, b = [(c * a + 127) // 255 for c in (r, g, b)]
+        return (r, g, b, a)
 
     def _native(self):
         hp = [float(v) for xy in self.gradientHandlePositions[:3] for v in xy]
@@ -69,15 +90,19 @@ class Paint:
         stops = np.array([list(s.color) + [s.position] for s in self.gradientStops],
                          np.float32).reshape(-1, 5)
         stops = np.ascontiguousarray(stops)
-        p = _PxhPaint(self.kind, self.blendMode, self.opacity, (ctypes.c_float * 6)(*hp),
-                      len(stops), _fp(stops) if len(stops) else None)
+        m = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(self.imageMat).ravel()])
+        p = _PxhPaint(self.kind, self.blendMode, self.opacity, len(self.gradientHandlePositions),
+                      (ctypes.c_float * 6)(*hp), len(stops), _fp(stops) if len(stops) else None,
+                      self.image._h if self.image is not None else None, m)
         return p, stops
 
 
 class _PxhPaint(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("blend_mode", ctypes.c_int32),
-                ("opacity", ctypes.c_float), ("handles", ctypes.c_float * 6),
-                ("nstops", ctypes.c_int32), ("stops", ctypes.POINTER(ctypes.c_float))]
+                ("opacity", ctypes.c_float), ("nhandles", ctypes.c_int32),
+                ("handles", ctypes.c_float * 6), ("nstops", ctypes.c_int32),
+                ("stops", ctypes.POINTER(ctypes.c_float)), ("image", ctypes.c_void_p),
+                ("image_mat", ctypes.c_float * 9)]
 
 
 class PixieError(RuntimeError):
@@ -122,6 +147,14 @@ def lib():
         L.pxh_free.argtypes = [vp]
         L.pxh_fill_path.argtypes = [vp, f32p, i32, f32p, u32, i32, i32]
         L.pxh_fill_path_paint.argtypes = [vp, f32p, i32, f32p, P(_PxhPaint), i32]
+        L.pxh_stroke_path_paint.argtypes = [vp, f32p, i32, f32p, P(_PxhPaint), f, i32, i32, f,
+                                            f32p, i32]
+        L.pxh_draw_transform.argtypes = [vp, vp, f32p, i32]
+        L.pxh_draw_tiled.argtypes = [vp, vp, f32p, i32]
+        L.pxh_minify_by2.restype = vp
+        L.pxh_minify_by2.argtypes = [vp, i32]
+        L.pxh_magnify_by2.restype = vp
+        L.pxh_magnify_by2.argtypes = [vp, i32]
         L.pxh_stroke_path.argtypes = [vp, f32p, i32, f32p, u32, f, i32, i32, f, f32p, i32, i32]
         L.pxh_fill_shapes.argtypes = [vp, f32p, i32p, i32, u32, i32, i32]
         L.pxh_draw.argtypes = [vp, vp, i32, i32, i32]
@@ -304,6 +337,10 @@ class Image:
     def fillPath(self, path, color, transform=None, windingRule=NonZero, blendMode=0):
         """`color` is premultiplied (r, g, b, a) bytes, or a gradient Paint
         (whose own blend mode applies, as in pixie)."""
+        if isinstance(color, Paint) and color.kind == PaintKind.SolidPaint:
+            if color.opacity <= 0:
+                return
+            color, blendMode = color.solid_rgbx(), color.blendMode
         if isinstance(color, Paint):
             c = _cmds(path)
             m = _mat(transform)
@@ -321,14 +358,27 @@ class Image:
 
     def strokePath(self, path, color, transform=None, strokeWidth=1.0, lineCap=ButtCap,
                    lineJoin=MiterJoin, miterLimit=4.0, dashes=(), blendMode=0):
-        if _paint_skipped(color, blendMode):
-            return
+        """`color` is premultiplied (r, g, b, a) bytes, or a non-solid Paint
+        (whose own blend mode applies, as in pixie)."""
         c = _cmds(path)
         m = _mat(transform)
         d = np.ascontiguousarray(dashes, dtype=np.float32) if len(dashes) else None
+        nd = 0 if d is None else len(d)
+        if isinstance(color, Paint) and color.kind == PaintKind.SolidPaint:
+            if color.opacity <= 0:
+                return
+            color, blendMode = color.solid_rgbx(), color.blendMode
+        if isinstance(color, Paint):
+            p, keep = color._native()
+            _check(lib().pxh_stroke_path_paint(self._h, _fp(c), len(c), _fp(m), ctypes.byref(p),
+                                               strokeWidth, lineCap, lineJoin, miterLimit, _fp(d),
+                                               nd))
+            del keep
+            return
+        if _paint_skipped(color, blendMode):
+            return
         _check(lib().pxh_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color), strokeWidth,
-                                     lineCap, lineJoin, miterLimit, _fp(d),
-                                     0 if d is None else len(d), blendMode))
+                                     lineCap, lineJoin, miterLimit, _fp(d), nd, blendMode))
 
     def fillShapes(self, shapes, color, windingRule=NonZero, blendMode=0):
         pts = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float32).reshape(-1, 2)
@@ -340,8 +390,36 @@ class Image:
         _check(lib().pxh_fill_shapes(self._h, _fp(pts), ofs.ctypes.data_as(
             ctypes.POINTER(ctypes.c_int32)), len(shapes), pack(color), windingRule, blendMode))
 
-    def draw(self, other, dx=0, dy=0, blendMode=0):
-        _check(lib().pxh_draw(self._h, other._h, int(dx), int(dy), blendMode))
+    def draw(self, other, dx=0, dy=0, blendMode=0, transform=None):
+        """Integer-offset composite (blendRect), or with `transform` pixie's
+        draw(a, b, transform, blendMode) (images.nim:636-678)."""
+        if transform is not None:
+            m = _mat(transform)
+            _check(lib().pxh_draw_transform(self._h, other._h, _fp(m), blendMode))
+        else:
+            _check(lib().pxh_draw(self._h, other._h, int(dx), int(dy), blendMode))
+
+    def drawTiled(self, other, mat, blendMode=0):
+        """drawTiled (images.nim:680-683)."""
+        m = _mat(mat)
+        _check(lib().pxh_draw_tiled(self._h, other._h, _fp(m), blendMode))
+
+    def _wrap(self, h):
+        if not h:
+            raise PixieError(6, lib().pxh_last_error().decode())
+        L = lib()
+        img = Image.__new__(Image)
+        img._h = ctypes.c_void_p(h)
+        img.width, img.height, img.a8 = L.pxh_image_width(h), L.pxh_image_height(h), False
+        return img
+
+    def minifyBy2(self, power=1):
+        """minifyBy2 (images.nim:168-228): a new image."""
+        return self._wrap(lib().pxh_minify_by2(self._h, int(power)))
+
+    def magnifyBy2(self, power=1):
+        """magnifyBy2 (images.nim:230-259): a new image."""
+        return self._wrap(lib().pxh_magnify_by2(self._h, int(power)))
 
     def blur(self, radius, outOfBounds=(0, 0, 0, 0), lane=0):
         _check(lib().pxh_blur_on(self._h, float(radius), pack(outOfBounds), lane))

@@ -14,6 +14,7 @@
 
 #include "../../include/pixie_hip.h"
 #include "pxh_common.h"
+#include "pxh_draw.h"
 #include "pxh_host_geom.h"
 #include "pxh_kernels.h"
 #include "pxh_pixel.h"
@@ -144,6 +145,9 @@ struct Ctx {
   DevBuf tmp2;
   DevBuf plane;      // alpha plane
   DevBuf blendBuf;   // pxh_blend_pixels
+  DevBuf minA, minB; // minifyBy2 steps of a transformed draw (ping-pong)
+  DevBuf magBuf;     // a magnified source for the integer composite
+  DevBuf paintFill;  // an image paint drawn onto a transparent canvas
   int32_t* errFlag = nullptr;
   // Per lane: error bits of scene renders that stopped before compose (an
   // overflowed pool); read and cleared when the lane is synchronized.  A
@@ -1120,74 +1124,354 @@ int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float*
   return fill_shapes_impl(img, sh, rgbx, winding_rule, blend_mode);
 }
 
+// ---- vmath Mat3 in float32, m[3 * col + row] (a[i, j] = m[3 * i + j]) ----
+static void m3_apply(const float* m, float x, float y, float* ox, float* oy) {
+  *ox = m[0] * x + m[3] * y + m[6];
+  *oy = m[1] * x + m[4] * y + m[7];
+}
+static void m3_mul(const float* a, const float* b, float* o) {
+  float r[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      r[3 * i + j] = b[3 * i + 0] * a[0 * 3 + j] + b[3 * i + 1] * a[1 * 3 + j] + b[3 * i + 2] * a[2 * 3 + j];
+  memcpy(o, r, sizeof(r));
+}
+// cofactor inverse expanded along the first column index
+static void m3_inverse(const float* m, float* o) {
+  auto A = [m](int i, int j) { return m[3 * i + j]; };
+  const float det = A(0, 0) * (A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)) -
+                    A(0, 1) * (A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0)) +
+                    A(0, 2) * (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0));
+  const float inv = 1.0f / det;
+  float r[9];
+  r[0] = (A(1, 1) * A(2, 2) - A(2, 1) * A(1, 2)) * inv;
+  r[1] = -(A(0, 1) * A(2, 2) - A(0, 2) * A(2, 1)) * inv;
+  r[2] = (A(0, 1) * A(1, 2) - A(0, 2) * A(1, 1)) * inv;
+  r[3] = -(A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0)) * inv;
+  r[4] = (A(0, 0) * A(2, 2) - A(0, 2) * A(2, 0)) * inv;
+  r[5] = -(A(0, 0) * A(1, 2) - A(1, 0) * A(0, 2)) * inv;
+  r[6] = (A(1, 0) * A(2, 1) - A(2, 0) * A(1, 1)) * inv;
+  r[7] = -(A(0, 0) * A(2, 1) - A(2, 0) * A(0, 1)) * inv;
+  r[8] = (A(0, 0) * A(1, 1) - A(1, 0) * A(0, 1)) * inv;
+  memcpy(o, r, sizeof(r));
+}
+static float fix_angle(float a) {  // vmath fixAngle
+  const float PI = 3.14159265358979323846f;
+  while (a > PI) a -= PI * 2.0f;
+  while (a < -PI) a += PI * 2.0f;
+  return a;
+}
+
 // The inverse gradient matrix of a radial paint, (translate(center) *
-// rotate(angle) * scale(distanceX, distanceY)).inverse() (paints.nim:185-195),
-// in float32 with vmath's Mat3 layout m[3 * col + row].
+// rotate(angle) * scale(distanceX, distanceY)).inverse() (paints.nim:192-203).
 static void radial_matrix(const float* hp, float* out) {
   auto len2 = [](float x, float y) { return sqrtf(x * x + y * y); };
-  auto fix = [](float a) {
-    const float PI = 3.14159265358979323846f;
-    while (a > PI) a -= PI * 2.0f;
-    while (a < -PI) a += PI * 2.0f;
-    return a;
-  };
   const float cx = hp[0], cy = hp[1];
   const float dX = len2(cx - hp[2], cy - hp[3]), dY = len2(cx - hp[4], cy - hp[5]);
   const float ln = len2(cx - hp[2], cy - hp[3]);
-  const float ga = fix(atan2f((cy - hp[3]) / ln, (cx - hp[2]) / ln));
+  const float ga = fix_angle(atan2f((cy - hp[3]) / ln, (cx - hp[2]) / ln));
   const float c = cosf(ga), s = sinf(ga);
   const float T[9] = {1, 0, 0, 0, 1, 0, cx, cy, 1};
   const float R[9] = {c, s, 0, -s, c, 0, 0, 0, 1};
   const float S[9] = {dX, 0, 0, 0, dY, 0, 0, 0, 1};
-  auto mul = [](const float* a, const float* b, float* o) {
-    float r[9];
-    for (int col = 0; col < 3; col++)
-      for (int row = 0; row < 3; row++)
-        r[3 * col + row] = a[row] * b[3 * col] + a[3 + row] * b[3 * col + 1] + a[6 + row] * b[3 * col + 2];
-    memcpy(o, r, sizeof(r));
-  };
   float TR[9], M[9];
-  mul(T, R, TR);
-  mul(TR, S, M);
-  const float a = M[0], b = M[3], cc = M[6], d = M[1], e = M[4], f = M[7], g2 = M[2], h = M[5], i = M[8];
-  const float A = e * i - f * h, B = -(d * i - f * g2), C = d * h - e * g2;
-  const float inv = 1.0f / (a * A + b * B + cc * C);
-  out[0] = A * inv;
-  out[1] = B * inv;
-  out[2] = C * inv;
-  out[3] = -(b * i - cc * h) * inv;
-  out[4] = (a * i - cc * g2) * inv;
-  out[5] = -(a * h - b * g2) * inv;
-  out[6] = (b * f - cc * e) * inv;
-  out[7] = -(a * f - cc * d) * inv;
-  out[8] = (a * e - b * d) * inv;
+  m3_mul(T, R, TR);
+  m3_mul(TR, S, M);
+  m3_inverse(M, out);
 }
 
-int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
-                        const pxh_paint* paint, int32_t winding_rule) {
+// ---- transformed draws (images.nim:405-445, 531-683) ----
+static const float kH = 0.5f;  // images.nim:5
+
+// p = inv * (h, h), dx / dy = inv * (1 + h, h) / (h, 1 + h) - p
+static void movement(const float* inv, float* p, float* dx, float* dy) {
+  float ax, ay, bx, by;
+  m3_apply(inv, 0 + kH, 0 + kH, &p[0], &p[1]);
+  m3_apply(inv, 1 + kH, 0 + kH, &ax, &ay);
+  m3_apply(inv, 0 + kH, 1 + kH, &bx, &by);
+  dx[0] = ax - p[0];
+  dx[1] = ay - p[1];
+  dy[0] = bx - p[0];
+  dy[1] = by - p[1];
+}
+
+// The minifyBy2 (> 0) or magnifyBy2 (< 0) steps of a draw, with the vectors
+// adjusted as the loops of images.nim:415-429 / 647-661 adjust them.  pixie
+// loops forever on a transform whose filterBy2 is 0 or infinite; that raises.
+static int filter_steps(float* p, float* dx, float* dy, int* steps) {
+  const float lx = sqrtf(dx[0] * dx[0] + dx[1] * dx[1]), ly = sqrtf(dy[0] * dy[0] + dy[1] * dy[1]);
+  float f = nim_maxf(lx, ly);
+  if (!(f > 0.0f) || isinf(f)) return fail(kErrArg, "degenerate draw transform");
+  int n = 0;
+  while (f >= 2.0f) {
+    for (int k = 0; k < 2; k++) {
+      p[k] /= 2;
+      dx[k] /= 2;
+      dy[k] /= 2;
+    }
+    f /= 2;
+    n++;
+  }
+  while (f <= 0.5f) {
+    for (int k = 0; k < 2; k++) {
+      p[k] *= 2;
+      dx[k] *= 2;
+      dy[k] *= 2;
+    }
+    f *= 2;
+    n--;
+  }
+  *steps = n;
+  return 0;
+}
+
+// The draw source after the filter steps: minified on the device into
+// ping-pong temporaries, or magnified in place (DrawSrc::mag).
+static int filter_source(const uint32_t* b, int32_t bw, int32_t bh, int steps, DrawSrc* out) {
+  *out = DrawSrc{b, bw, bh, 0};
+  if (steps < 0) {
+    const int s = -steps;
+    if (s > 20 || ((int64_t)bw << s) > (1 << 30) || ((int64_t)bh << s) > (1 << 30))
+      return fail(kErrCapacity, "magnified draw source too large");
+    out->mag = s;
+    return 0;
+  }
+  const uint32_t* cur = b;
+  int32_t w = bw, h = bh;
+  for (int i = 0; i < steps; i++) {
+    const int32_t nw = (w + 1) / 2, nh = (h + 1) / 2;
+    DevBuf& dst = (i & 1) ? g.minB : g.minA;
+    if (int e = dst.ensure((size_t)nw * nh * 4)) return fail(e, "minify allocation failed");
+    MinifyArgs m{cur, w, h, (uint32_t*)dst.p};
+    if (launch_minify(m, g.stream) != 0) return fail(kErrDevice, "minify launch failed");
+    cur = (const uint32_t*)dst.p;
+    w = nw;
+    h = nh;
+  }
+  *out = DrawSrc{cur, w, h, 0};
+  return 0;
+}
+
+// blendRect (images.nim:468-529) of raw RGBA8 / A8 buffers
+static int composite_impl(uint32_t* dst, int32_t aw, int32_t ah, const void* src, bool srcA8,
+                          int32_t bw, int32_t bh, int64_t dx, int64_t dy, int32_t mode) {
+  if (dx >= aw || dx + bw <= 0 || dy >= ah || dy + bh <= 0) {
+    if (mode == kMask) PXH_HIP(hipMemsetAsync(dst, 0, (size_t)aw * ah * 4, g.stream));
+    return 0;
+  }
+  CompositeArgs c;
+  c.dst = dst;
+  c.src = src;
+  c.aw = aw;
+  c.ah = ah;
+  c.bw = bw;
+  c.bh = bh;
+  c.px = (int32_t)dx;
+  c.py = (int32_t)dy;
+  c.xStart = -dx > 0 ? (int32_t)-dx : 0;
+  c.yStart = -dy > 0 ? (int32_t)-dy : 0;
+  c.xEnd = (int64_t)bw < aw - dx ? bw : (int32_t)(aw - dx);
+  c.yEnd = (int64_t)bh < ah - dy ? bh : (int32_t)(ah - dy);
+  c.mode = mode;
+  if (launch_composite(c, srcA8, g.stream) != 0) return fail(kErrDevice, "composite launch failed");
+  return 0;
+}
+
+// draw(a, b, transform, blendMode) (images.nim:636-678), or drawTiled
+// (drawCorrect with wrapped samples, images.nim:405-445, 680-683)
+static int draw_transform_impl(uint32_t* dst, int32_t aw, int32_t ah, const uint32_t* b, int32_t bw,
+                               int32_t bh, const float* mat9, int32_t mode, bool tiled) {
+  float t[9], inv[9], p[2], dx[2], dy[2];
+  memcpy(t, mat9 ? mat9 : kIdentity, sizeof(t));
+  m3_inverse(t, inv);
+  movement(inv, p, dx, dy);
+  int steps;
+  if (int r = filter_steps(p, dx, dy, &steps)) return r;
+  DrawSrc src;
+  if (int r = filter_source(b, bw, bh, steps, &src)) return r;
+  if (tiled) {
+    // inverseTransform = scale(1/2) * inverse per minify step, scale(2) *
+    // inverse per magnify step: the first two rows scale by 2^-steps
+    const float k = ldexpf(1.0f, -steps);
+    for (int c = 0; c < 3; c++) {
+      inv[3 * c + 0] *= k;
+      inv[3 * c + 1] *= k;
+    }
+    CorrectArgs ca;
+    ca.dst = dst;
+    ca.aw = aw;
+    ca.ah = ah;
+    ca.src = src;
+    memcpy(ca.inv, inv, sizeof(inv));
+    ca.mode = mode;
+    ca.wrapped = 1;
+    if (launch_draw_correct(ca, g.stream) != 0) return fail(kErrDevice, "draw launch failed");
+    return 0;
+  }
+  // transform * scale(2) per minify step, * scale(1/2) per magnify step:
+  // the first two columns scale by 2^steps
+  const float k = ldexpf(1.0f, steps);
+  for (int r = 0; r < 3; r++) {
+    t[r] *= k;
+    t[3 + r] *= k;
+  }
+  const bool rotScale = !(dx[0] == 1.0f && dx[1] == 0.0f && dy[0] == 0.0f && dy[1] == 1.0f);
+  const float tx = t[6], ty = t[7];
+  // vmath fractional(v) = |v| - floor(|v|); only its test against 0 is used
+  const bool smooth = !(sqrtf(dx[0] * dx[0] + dx[1] * dx[1]) == 1.0f &&
+                        sqrtf(dy[0] * dy[0] + dy[1] * dy[1]) == 1.0f &&
+                        (fabsf(tx) - floorf(fabsf(tx))) == 0.0f &&
+                        (fabsf(ty) - floorf(fabsf(ty))) == 0.0f);
+  if (!rotScale && !smooth) {
+    // blendRect at ivec2(transform[2, 0].int32, transform[2, 1].int32);
+    // offsets past the int32 range put the image off the canvas
+    const float lim = 2147483520.0f;
+    int64_t ox = fabsf(tx) > lim ? (int64_t)1 << 40 : (int64_t)tx;
+    int64_t oy = fabsf(ty) > lim ? (int64_t)1 << 40 : (int64_t)ty;
+    const void* sp = src.px;
+    int32_t sw = src.w, sh = src.h;
+    if (src.mag > 0) {  // the integer composite reads the magnified image itself
+      const int64_t mw = (int64_t)src.w << src.mag, mh = (int64_t)src.h << src.mag;
+      if (mw * mh > ((int64_t)1 << 30)) return fail(kErrCapacity, "magnified draw source too large");
+      if (int e = g.magBuf.ensure((size_t)(mw * mh * 4))) return fail(e, "magnify allocation failed");
+      MagnifyArgs ma{src.px, src.w, src.h, src.mag, (uint32_t*)g.magBuf.p};
+      if (launch_magnify(ma, g.stream) != 0) return fail(kErrDevice, "magnify launch failed");
+      sp = g.magBuf.p;
+      sw = (int32_t)mw;
+      sh = (int32_t)mh;
+    }
+    return composite_impl(dst, aw, ah, sp, false, sw, sh, ox, oy, mode);
+  }
+  // drawSmooth (images.nim:531-634) with the adjusted transform
+  SmoothArgs sa;
+  float c[4][2];
+  const float sw = (float)(src.w << src.mag), sh = (float)(src.h << src.mag);
+  m3_apply(t, 0, 0, &c[0][0], &c[0][1]);
+  m3_apply(t, sw, 0, &c[1][0], &c[1][1]);
+  m3_apply(t, sw, sh, &c[2][0], &c[2][1]);
+  m3_apply(t, 0, sh, &c[3][0], &c[3][1]);
+  for (int i = 0; i < 4; i++) {
+    sa.per[i][0] = c[i][0];
+    sa.per[i][1] = c[i][1];
+    sa.per[i][2] = c[(i + 1) & 3][0];
+    sa.per[i][3] = c[(i + 1) & 3][1];
+  }
+  float inv2[9];
+  m3_inverse(t, inv2);
+  movement(inv2, sa.p, sa.dx, sa.dy);
+  int64_t y0 = ah, y1 = 0;
+  for (int i = 0; i < 4; i++) {
+    const int64_t f = nim_f2i(floorf(sa.per[i][1])), ce = nim_f2i(ceilf(sa.per[i][1]));
+    y0 = f < y0 ? f : y0;
+    y1 = ce > y1 ? ce : y1;
+  }
+  sa.yStart = (int32_t)(y0 < 0 ? 0 : (y0 > ah ? ah : y0));
+  sa.yEnd = (int32_t)(y1 < 0 ? 0 : (y1 > ah ? ah : y1));
+  sa.dst = dst;
+  sa.aw = aw;
+  sa.ah = ah;
+  sa.src = src;
+  sa.mode = mode;
+  if (launch_draw_smooth(sa, g.stream) != 0) return fail(kErrDevice, "draw launch failed");
+  return 0;
+}
+
+static int check_draw_pair(pxh_image* a, const pxh_image* b, int32_t blend_mode) {
+  if (int r = check_image(a)) return r;
+  if (int r = check_image(b)) return r;
+  if (a->fmt != PXH_FORMAT_RGBA8 || b->fmt != PXH_FORMAT_RGBA8)
+    return fail(kErrArg, "transformed draws need RGBA8 images");
+  if (blend_mode < 0 || blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (a->dev == b->dev) return fail(kErrArg, "draw source and target must differ");
+  return 0;
+}
+
+int pxh_draw_transform(pxh_image* a, const pxh_image* b, const float* mat9, int32_t blend_mode) {
   PXH_INIT();
-  if (int r = check_image(img)) return r;
-  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "paints need an RGBA8 image");
-  if (!paint) return fail(kErrArg, "invalid paint");
-  if (paint->kind < PXH_LINEAR_GRADIENT || paint->kind > PXH_ANGULAR_GRADIENT)
-    return fail(kErrArg, "Paint must be a gradient");
-  if (paint->blend_mode < 0 || paint->blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
-  if (paint->nstops <= 0 || !paint->stops) return fail(kErrArg, "Gradient must have at least 1 color stop");
-  if (paint->nstops > PXH_MAX_STOPS) return fail(kErrCapacity, "too many gradient stops");
-  float opacity = paint->opacity;  // paths.nim:2101: clamp(opacity, 0, 1)
-  if (!(opacity > 0.0f)) opacity = 0.0f;
-  if (opacity > 1.0f) opacity = 1.0f;
-  if (opacity == 0.0f) return 0;
-  // the path's coverage: a white NonZero / EvenOdd fill onto a transparent
-  // 8-bit plane, which holds the alpha pixie's RGBA mask would hold
-  const size_t npx = (size_t)img->w * img->h;
-  if (int e = g.plane.ensure(npx)) return fail(e, "mask allocation failed");
-  PXH_HIP(hipMemsetAsync(g.plane.p, 0, npx, g.stream));
-  pxh_image* mask = pxh_image_view(g.plane.p, img->w, img->h, PXH_FORMAT_A8);
-  if (!mask) return kErrArg;
-  int r = pxh_fill_path(mask, cmds, ncmds, mat9, 0xFFFFFFFFu, winding_rule, kNormal);
-  pxh_image_free(mask);
-  if (r) return r;
+  if (int r = check_draw_pair(a, b, blend_mode)) return r;
+  return draw_transform_impl((uint32_t*)a->dev, a->w, a->h, (const uint32_t*)b->dev, b->w, b->h, mat9,
+                             blend_mode, false);
+}
+
+int pxh_draw_tiled(pxh_image* a, const pxh_image* b, const float* mat9, int32_t blend_mode) {
+  PXH_INIT();
+  if (int r = check_draw_pair(a, b, blend_mode)) return r;
+  return draw_transform_impl((uint32_t*)a->dev, a->w, a->h, (const uint32_t*)b->dev, b->w, b->h, mat9,
+                             blend_mode, true);
+}
+
+pxh_image* pxh_minify_by2(const pxh_image* img, int32_t power) {
+  if (pxh_init() != 0) return nullptr;
+  if (check_image(img) != 0) return nullptr;
+  if (img->fmt != PXH_FORMAT_RGBA8) {
+    fail(kErrArg, "minifyBy2 needs an RGBA8 image");
+    return nullptr;
+  }
+  if (power < 0) {
+    fail(kErrArg, "Cannot minifyBy2 with negative power");
+    return nullptr;
+  }
+  int32_t w = img->w, h = img->h;
+  for (int i = 0; i < power; i++) {
+    w = (w + 1) / 2;
+    h = (h + 1) / 2;
+  }
+  pxh_image* out = pxh_image_new(w, h);
+  if (!out) return nullptr;
+  if (power == 0) {
+    if (pxh_image_copy(out, img) != 0) {
+      pxh_image_free(out);
+      return nullptr;
+    }
+    return out;
+  }
+  DrawSrc src;
+  if (filter_source((const uint32_t*)img->dev, img->w, img->h, power - 1, &src) != 0) {
+    pxh_image_free(out);
+    return nullptr;
+  }
+  MinifyArgs m{src.px, src.w, src.h, (uint32_t*)out->dev};
+  if (launch_minify(m, g.stream) != 0) {
+    fail(kErrDevice, "minify launch failed");
+    pxh_image_free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+pxh_image* pxh_magnify_by2(const pxh_image* img, int32_t power) {
+  if (pxh_init() != 0) return nullptr;
+  if (check_image(img) != 0) return nullptr;
+  if (img->fmt != PXH_FORMAT_RGBA8) {
+    fail(kErrArg, "magnifyBy2 needs an RGBA8 image");
+    return nullptr;
+  }
+  if (power < 0) {
+    fail(kErrArg, "Cannot magnifyBy2 with negative power");
+    return nullptr;
+  }
+  if (power > 20 || ((int64_t)img->w << power) > (1 << 30) || ((int64_t)img->h << power) > (1 << 30)) {
+    fail(kErrCapacity, "magnified image too large");
+    return nullptr;
+  }
+  pxh_image* out = pxh_image_new(img->w << power, img->h << power);
+  if (!out) return nullptr;
+  MagnifyArgs m{(const uint32_t*)img->dev, img->w, img->h, power, (uint32_t*)out->dev};
+  if (launch_magnify(m, g.stream) != 0) {
+    fail(kErrDevice, "magnify launch failed");
+    pxh_image_free(out);
+    return nullptr;
+  }
+  return out;
+}
+
+// ---- non-solid paints (paths.nim:2115-2142, 2176-2214) ----
+
+// After the path's coverage is in g.plane: build the paint's colour
+// (gradients per pixel in the paint kernel, images through a draw onto a
+// transparent temporary), mask it with the coverage at the paint's opacity
+// and blend it onto img, in one pass over the canvas.
+static int paint_route(pxh_image* img, const pxh_paint* paint, float opacity) {
   PaintArgs a = {};
   a.dst = (uint32_t*)img->dev;
   a.mask = (const uint8_t*)g.plane.p;
@@ -1197,21 +1481,97 @@ int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const 
   a.mode = paint->blend_mode;
   a.useOpacity = opacity != 1.0f;
   a.o255 = (uint16_t)nim_f2u32(roundf(255.0f * opacity));  // images.nim:263
-  memcpy(a.hx, paint->handles, sizeof(a.hx));
-  if (paint->kind == PXH_RADIAL_GRADIENT) radial_matrix(paint->handles, a.m);
-  if (paint->kind == PXH_ANGULAR_GRADIENT) {
-    const float PI = 3.14159265358979323846f;
-    const float vx = paint->handles[2] - paint->handles[0], vy = paint->handles[3] - paint->handles[1];
-    const float l = sqrtf(vx * vx + vy * vy);
-    float ga = atan2f(vy / l, vx / l);
-    while (ga > PI) ga -= PI * 2.0f;
-    while (ga < -PI) ga += PI * 2.0f;
-    a.ga = ga;
+  if (paint->kind == PXH_IMAGE_PAINT || paint->kind == PXH_TILED_IMAGE_PAINT) {
+    const pxh_image* im = paint->image;
+    if (int r = check_image(im)) return r;
+    if (im->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "paint image must be RGBA8");
+    const size_t bytes = (size_t)img->w * img->h * 4;
+    if (int e = g.paintFill.ensure(bytes)) return fail(e, "paint allocation failed");
+    PXH_HIP(hipMemsetAsync(g.paintFill.p, 0, bytes, g.stream));
+    if (int r = draw_transform_impl((uint32_t*)g.paintFill.p, img->w, img->h, (const uint32_t*)im->dev,
+                                    im->w, im->h, paint->image_mat, kNormal,
+                                    paint->kind == PXH_TILED_IMAGE_PAINT))
+      return r;
+    a.fill = (const uint32_t*)g.paintFill.p;
+  } else {
+    // fillGradient's checks (paints.nim:98-102, 182-186, 214-218, 247-248)
+    const int k = paint->kind;
+    if (k == PXH_LINEAR_GRADIENT && paint->nhandles != 2)
+      return fail(kErrArg, "Linear gradient requires 2 handles");
+    if (k == PXH_RADIAL_GRADIENT && paint->nhandles != 3)
+      return fail(kErrArg, "Radial gradient requires 3 handles");
+    if (k == PXH_ANGULAR_GRADIENT && paint->nhandles != 3)
+      return fail(kErrArg, "Angular gradient requires 2 handles");  // sic: 3 are read
+    if (k < PXH_LINEAR_GRADIENT || k > PXH_ANGULAR_GRADIENT) return fail(kErrArg, "Paint must be a gradient");
+    if (paint->nstops <= 0 || !paint->stops) return fail(kErrArg, "Gradient must have at least 1 color stop");
+    if (paint->nstops > PXH_MAX_STOPS) return fail(kErrCapacity, "too many gradient stops");
+    memcpy(a.hx, paint->handles, sizeof(a.hx));
+    if (k == PXH_RADIAL_GRADIENT) radial_matrix(paint->handles, a.m);
+    if (k == PXH_ANGULAR_GRADIENT) {
+      const float vx = paint->handles[2] - paint->handles[0], vy = paint->handles[3] - paint->handles[1];
+      const float l = sqrtf(vx * vx + vy * vy);
+      a.ga = fix_angle(atan2f(vy / l, vx / l));
+    }
+    a.nstops = paint->nstops;
+    memcpy(a.stops, paint->stops, sizeof(float) * 5 * (size_t)paint->nstops);
   }
-  a.nstops = paint->nstops;
-  memcpy(a.stops, paint->stops, sizeof(float) * 5 * (size_t)paint->nstops);
   if (launch_paint(a, g.stream) != 0) return fail(kErrDevice, "paint launch failed");
   return 0;
+}
+
+// The common head of fillPath / strokePath with a non-solid paint: clamps
+// the opacity (0 draws nothing), checks the paint, and readies the 8-bit
+// coverage plane (cleared) as an image view for the white mask fill.
+static int paint_begin(pxh_image* img, const pxh_paint* paint, float* opacity, pxh_image** mask) {
+  *mask = nullptr;
+  if (int r = check_image(img)) return r;
+  if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "paints need an RGBA8 image");
+  if (!paint) return fail(kErrArg, "invalid paint");
+  if (paint->kind <= 0 || paint->kind > PXH_ANGULAR_GRADIENT)
+    return fail(kErrArg, "solid paints go through pxh_fill_path / pxh_stroke_path");
+  if (paint->blend_mode < 0 || paint->blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  float o = paint->opacity;  // paths.nim:2101 / 2156: clamp(opacity, 0, 1)
+  if (!(o > 0.0f)) o = 0.0f;
+  if (o > 1.0f) o = 1.0f;
+  *opacity = o;
+  if (o == 0.0f) return 0;
+  const size_t npx = (size_t)img->w * img->h;
+  if (int e = g.plane.ensure(npx)) return fail(e, "mask allocation failed");
+  PXH_HIP(hipMemsetAsync(g.plane.p, 0, npx, g.stream));
+  *mask = pxh_image_view(g.plane.p, img->w, img->h, PXH_FORMAT_A8);
+  if (!*mask) return kErrArg;
+  return 0;
+}
+
+int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                        const pxh_paint* paint, int32_t winding_rule) {
+  PXH_INIT();
+  float opacity;
+  pxh_image* mask;
+  if (int r = paint_begin(img, paint, &opacity, &mask)) return r;
+  if (!mask) return 0;  // opacity 0
+  // the path's coverage: a white fill onto a transparent 8-bit plane, which
+  // holds the alpha pixie's RGBA mask holds (its rgb equals its alpha)
+  int r = pxh_fill_path(mask, cmds, ncmds, mat9, 0xFFFFFFFFu, winding_rule, kNormal);
+  pxh_image_free(mask);
+  if (r) return r;
+  return paint_route(img, paint, opacity);
+}
+
+int pxh_stroke_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
+                          const pxh_paint* paint, float stroke_width, int32_t line_cap,
+                          int32_t line_join, float miter_limit, const float* dashes,
+                          int32_t ndashes) {
+  PXH_INIT();
+  float opacity;
+  pxh_image* mask;
+  if (int r = paint_begin(img, paint, &opacity, &mask)) return r;
+  if (!mask) return 0;
+  int r = pxh_stroke_path(mask, cmds, ncmds, mat9, 0xFFFFFFFFu, stroke_width, line_cap, line_join,
+                          miter_limit, dashes, ndashes, kNormal);
+  pxh_image_free(mask);
+  if (r) return r;
+  return paint_route(img, paint, opacity);
 }
 
 int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,

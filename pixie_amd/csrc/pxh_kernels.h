@@ -59,15 +59,17 @@ struct SpreadArgs {
   int32_t dx, dy;        // read src translated by (dx, dy)
 };
 
-// Gradient paint over the whole canvas (fillPath's route for non-solid
-// paints, paths.nim:2115-2142): dst = blend(mode, dst, blendMask(gradient,
-// applyOpacity(mask))), the mask being the coverage of a white fill.
+// Non-solid paint over the whole canvas (fillPath's route, paths.nim:
+// 2115-2142): dst = blend(mode, dst, blendMask(fill, applyOpacity(mask))),
+// the mask being the coverage of a white fill and fill the gradient (made
+// per pixel) or the paint's image drawn onto a transparent canvas.
 constexpr int kMaxStops = 32;
 struct PaintArgs {
   uint32_t* dst;
   const uint8_t* mask;      // A8 coverage of the path
   int32_t w, h;
-  int32_t kind, mode;       // PaintKind ordinal (3..5), blend mode
+  const uint32_t* fill;     // image paints (kinds 1, 2): the image drawn on transparent
+  int32_t kind, mode;       // PaintKind ordinal (1..5), blend mode
   int32_t useOpacity;       // opacity != 1: mask *= o255 / 255
   uint32_t o255;            // round(255 * opacity)
   float hx[6];              // gradient handles

@@ -124,6 +124,27 @@ __device__ __forceinline__ float fix_angle_p(float x) {
   return x;
 }
 
+// the gradient parameter t of pixel (x, y) (paints.nim:108-113, 199-209,
+// 224-236)
+__device__ float gradient_t(const PaintArgs& a, int32_t x, int32_t y) {
+  const float fx = (float)x, fy = (float)y;
+  if (a.kind == 3) {  // linear: toLineSpace
+    const float dx = a.hx[2] - a.hx[0], dy = a.hx[3] - a.hx[1];
+    const float det = dx * dx + dy * dy;
+    return (dy * (fy - a.hx[1]) + dx * (fx - a.hx[0])) / det;
+  }
+  if (a.kind == 4) {  // radial: |inverse gradient matrix * xy|
+    const float px = a.m[0] * fx + a.m[3] * fy + a.m[6];
+    const float py = a.m[1] * fx + a.m[4] * fy + a.m[7];
+    return sqrtf(px * px + py * py);
+  }
+  const float PI = 3.14159265358979323846f;  // angular
+  const float vx = fx - a.hx[0], vy = fy - a.hx[1];
+  const float l = sqrtf(vx * vx + vy * vy);
+  const float angle = atan2f(vy / l, vx / l);
+  return fix_angle_p(angle + a.ga + PI / 2.0f) / 2.0f / PI + 0.5f;
+}
+
 __global__ void __launch_bounds__(256) paint_kernel(PaintArgs a) {
   const int32_t y = blockIdx.y;
   const int32_t x = blockIdx.x * 256 + threadIdx.x;
@@ -131,24 +152,8 @@ __global__ void __launch_bounds__(256) paint_kernel(PaintArgs a) {
   const size_t i = (size_t)y * a.w + x;
   uint32_t m = a.mask[i];
   if (a.useOpacity) m = m * a.o255 / 255u;  // applyOpacity on the mask (images.nim:261-277)
-  float t;
-  const float fx = (float)x, fy = (float)y;
-  if (a.kind == 3) {  // linear: toLineSpace (paints.nim:113-118)
-    const float dx = a.hx[2] - a.hx[0], dy = a.hx[3] - a.hx[1];
-    const float det = dx * dx + dy * dy;
-    t = (dy * (fy - a.hx[1]) + dx * (fx - a.hx[0])) / det;
-  } else if (a.kind == 4) {  // radial: |inverse gradient matrix * xy|
-    const float px = a.m[0] * fx + a.m[3] * fy + a.m[6];
-    const float py = a.m[1] * fx + a.m[4] * fy + a.m[7];
-    t = sqrtf(px * px + py * py);
-  } else {  // angular (paints.nim:232-236)
-    const float PI = 3.14159265358979323846f;
-    const float vx = fx - a.hx[0], vy = fy - a.hx[1];
-    const float l = sqrtf(vx * vx + vy * vy);
-    const float angle = atan2f(vy / l, vx / l);
-    t = fix_angle_p(angle + a.ga + PI / 2.0f) / 2.0f / PI + 0.5f;
-  }
-  const uint32_t g = gradient_color(a, t);
+  // image paints read the image drawn onto a transparent canvas
+  const uint32_t g = a.kind <= 2 ? a.fill[i] : gradient_color(a, gradient_t(a, x, y));
   // fill.draw(mask, MaskBlend): every channel times the mask's alpha
   const uint32_t src = pack(ch_r(g) * m / 255u, ch_g(g) * m / 255u, ch_b(g) * m / 255u, ch_a(g) * m / 255u);
   // a transparent source leaves the backdrop of these modes bit for bit

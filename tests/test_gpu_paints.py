@@ -133,8 +133,8 @@ def test_errors_and_no_ops(gpu):
     assert np.array_equal(img.download(), before)          # opacity 0 draws nothing
     with pytest.raises(px.PixieError, match="at least 1 color stop"):
         img.fillPath("M 0 0 L 64 0 L 64 64 Z", _paint(3, [(0, 0), (64, 0)], []))
-    with pytest.raises(px.PixieError, match="must be a gradient"):
-        img.fillPath("M 0 0 L 64 0 L 64 64 Z", _paint(0, [], STOPS))
+    with pytest.raises(px.PixieError, match="requires 2 handles"):
+        img.fillPath("M 0 0 L 64 0 L 64 64 Z", _paint(3, [(0, 0), (64, 0), (1, 1)], STOPS))
     with pytest.raises(px.PixieError):
         img.fillPath("M 0 0 L 64 0 L 64 64 Z", _paint(3, [(0, 0), (64, 0)], STOPS * 17))
     assert np.array_equal(img.download(), before)
