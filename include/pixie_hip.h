@@ -282,7 +282,9 @@ int pxh_blur(pxh_image* img, float radius, uint32_t oob_rgbx);
 int pxh_blur_noalloc(pxh_image* img, float radius, uint32_t oob_rgbx);
 /* spread (images.nim:700-758). */
 int pxh_spread(pxh_image* img, float spread);
-/* shadow (images.nim:760-776) into `out` (same size as src); integer offsets. */
+/* shadow (images.nim:760-776) into `out` (same size as src).  Whole-pixel
+ * offsets copy through the integer composite, fractional ones through
+ * drawSmooth, as draw(image, translate(offset), OverwriteBlend) does. */
 int pxh_shadow(pxh_image* out, const pxh_image* src, float offset_x, float offset_y,
                float spread, float blur, uint32_t rgbx);
 /* applyOpacity (images.nim:261-277). */

@@ -577,9 +577,9 @@ void orc_draw_rect(uint8_t* a, int aw, int ah, const uint8_t* b, int bw, int bh,
     memset(a + ((int64_t)(yEnd + py) * aw) * 4, 0, (size_t)(ah - (yEnd + py)) * aw * 4);
 }
 
-/* images.nim:760-776 shadow.  Integer offsets take the blendRect route
- * (images.nim:675-678); fractional offsets need drawSmooth, which this
- * oracle does not cover. */
+/* images.nim:760-776 shadow.  The offset copy is draw(image,
+ * translate(offset), OverwriteBlend): the integer composite for whole-pixel
+ * offsets, drawSmooth for fractional ones (images.nim:636-678). */
 int orc_shadow(uint8_t* out, const uint8_t* img, int w, int h, float offx,
                float offy, float spread, float blur, orc_rgbx color) {
   size_t bytes = (size_t)w * h * 4;
@@ -587,13 +587,13 @@ int orc_shadow(uint8_t* out, const uint8_t* img, int w, int h, float offx,
   if (offx == 0.0f && offy == 0.0f) {
     memcpy(mask, img, bytes);
   } else {
-    if (offx != floorf(offx) || offy != floorf(offy)) {
-      free(mask);
-      orc_set_error("fractional shadow offsets need drawSmooth");
-      return ORC_ERR_ARG;
-    }
     memset(mask, 0, bytes);
-    orc_draw_rect(mask, w, h, img, w, h, (int)offx, (int)offy, ORC_OVERWRITE);
+    const float t[9] = {1, 0, 0, 0, 1, 0, offx, offy, 1};
+    const int rc0 = orc_draw(mask, w, h, img, w, h, t, ORC_OVERWRITE);
+    if (rc0 != ORC_OK) {
+      free(mask);
+      return rc0;
+    }
   }
   int rc = orc_spread(mask, w, h, spread);
   if (rc == ORC_OK) rc = orc_blur(mask, w, h, blur, (orc_rgbx){0, 0, 0, 0});

@@ -148,6 +148,7 @@ struct Ctx {
   DevBuf minA, minB; // minifyBy2 steps of a transformed draw (ping-pong)
   DevBuf magBuf;     // a magnified source for the integer composite
   DevBuf paintFill;  // an image paint drawn onto a transparent canvas
+  DevBuf drawRows;   // drawSmooth: per-row spans and run start positions
   int32_t* errFlag = nullptr;
   // Per lane: error bits of scene renders that stopped before compose (an
   // overflowed pool); read and cleared when the lane is synchronized.  A
@@ -1372,6 +1373,12 @@ static int draw_transform_impl(uint32_t* dst, int32_t aw, int32_t ah, const uint
   sa.ah = ah;
   sa.src = src;
   sa.mode = mode;
+  sa.nRuns = (aw + kSmoothRun - 1) / kSmoothRun;
+  const size_t spanBytes = ((size_t)ah * sizeof(int2) + 255) & ~(size_t)255;
+  if (int e = g.drawRows.ensure(spanBytes + (size_t)ah * sa.nRuns * sizeof(float2)))
+    return fail(e, "draw scratch allocation failed");
+  sa.span = (int2*)g.drawRows.p;
+  sa.starts = (float2*)(g.drawRows.p + spanBytes);
   if (launch_draw_smooth(sa, g.stream) != 0) return fail(kErrDevice, "draw launch failed");
   return 0;
 }
@@ -1759,18 +1766,33 @@ int pxh_shadow(pxh_image* out, const pxh_image* src, float offset_x, float offse
   if (out->w != src->w || out->h != src->h || out->fmt != PXH_FORMAT_RGBA8 ||
       src->fmt != PXH_FORMAT_RGBA8)
     return fail(kErrArg, "shadow needs RGBA8 images of the same size");
-  if (offset_x != floorf(offset_x) || offset_y != floorf(offset_y))
-    return fail(kErrArg, "fractional shadow offsets need drawSmooth");
   int64_t br = nim_f2i(roundf(blur));
   if (br < 0) return fail(kErrNegBlur, "Cannot apply negative blur");
   const int32_t w = src->w, h = src->h;
-  const int32_t dx = (int32_t)offset_x, dy = (int32_t)offset_y;
   if (int e = g.tmp2.ensure((size_t)w * h * 4)) return fail(e, "shadow temporary allocation failed");
   uint32_t* mask = (uint32_t*)g.tmp2.p;
   int64_t s = nim_f2i(roundf(spread));
-  SpreadArgs sa{(const uint32_t*)src->dev, nullptr, mask, w, h, (int32_t)(s > 0 ? s : -s),
-                s > 0 ? 1 : 0, dx, dy};
-  if (s != 0) {
+  // mask = draw(image, translate(offset), OverwriteBlend) onto transparent
+  // (images.nim:765-768): whole-pixel offsets fold into the spread pass's
+  // reads; fractional ones go through drawSmooth first
+  const bool whole = offset_x == floorf(offset_x) && offset_y == floorf(offset_y) &&
+                     fabsf(offset_x) < 1073741824.0f && fabsf(offset_y) < 1073741824.0f;
+  const uint32_t* from = (const uint32_t*)src->dev;
+  int32_t dx = 0, dy = 0;
+  if (whole) {
+    dx = (int32_t)offset_x;
+    dy = (int32_t)offset_y;
+  } else {
+    PXH_HIP(hipMemsetAsync(mask, 0, (size_t)w * h * 4, g.stream));
+    const float t[9] = {1, 0, 0, 0, 1, 0, offset_x, offset_y, 1};
+    if (int r = draw_transform_impl(mask, w, h, (const uint32_t*)src->dev, w, h, t, kOverwrite, false))
+      return r;
+    from = mask;  // spread in place: its passes read and write different buffers
+  }
+  SpreadArgs sa{from, nullptr, mask, w, h, (int32_t)(s > 0 ? s : -s), s > 0 ? 1 : 0, dx, dy};
+  if (!whole && s == 0) {
+    // the drawn mask is ready
+  } else if (s != 0) {
     // mask = draw(image, translate(offset), OverwriteBlend); spread(mask)
     if (int e = g.plane.ensure((size_t)w * h)) return fail(e, "plane allocation failed");
     sa.plane = (uint8_t*)g.plane.p;

@@ -28,7 +28,10 @@ struct MagnifyArgs {
   uint32_t* dst;
 };
 
-// drawSmooth (images.nim:531-634) of src onto dst.
+// drawSmooth (images.nim:531-634) of src onto dst.  Rows are cut into runs
+// of kSmoothRun pixels; the running sample position at each run's start is
+// kept in `starts` (ah x nRuns).
+constexpr int kSmoothRun = 16;
 struct SmoothArgs {
   uint32_t* dst;
   int32_t aw, ah;
@@ -37,6 +40,9 @@ struct SmoothArgs {
   float p[2], dx[2], dy[2];   // movement vectors of the inverse transform
   int32_t yStart, yEnd;       // clamped row range of the perimeter
   int32_t mode;
+  int2* span;                 // per row: [xStart, xEnd) (scratch, ah entries)
+  float2* starts;             // per row and run: the start position (scratch)
+  int32_t nRuns;              // ceil(aw / kSmoothRun)
 };
 
 // drawCorrect (images.nim:405-445): every pixel sampled at inv * (x+h, y+h).

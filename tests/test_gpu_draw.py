@@ -218,3 +218,12 @@ def test_draw_errors(gpu):
                    gradientStops=[px.ColorStop((1, 0, 0, 1), 0)])
     with pytest.raises(px.PixieError, match="requires 2 handles"):
         a.fillPath("M 0 0 L 50 0 L 50 50 Z", bad)
+
+
+@pytest.mark.parametrize("offset,spread,blur", [((3.25, -1.5), 2, 4), ((0.5, 0.5), 0, 6),
+                                                 ((-7.75, 12.1), -3, 0), ((12, 12), 4, 16)])
+def test_shadow_offsets(gpu, orc, offset, spread, blur):
+    src = _rand(80, 400, 300, transparent=0.4)
+    color = (20, 40, 60, 200)
+    got = px.Image.from_array(src).shadow(offset, spread, blur, color).download()
+    _exact(got, orc.shadow(src, offset, spread, blur, color), "shadow %s" % (offset,))

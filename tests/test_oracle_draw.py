@@ -303,3 +303,21 @@ def test_sub_pixel_tall_draw_on_a_scanline_draws_nothing(orc):
     # is drawn
     orc.draw_transform(a, b, mul(T(3, 5.5), S(1, 0.75)))
     assert (a[5, 3:13, 3] == 255).all() and (a[5, 3:13, 0] < 255).all()
+
+
+def test_shadow_fractional_offset_is_smooth_draw(orc):
+    # shadow (images.nim:760-776) with a fractional offset: the mask is the
+    # image drawn with OverwriteBlend through drawSmooth, then spread, blur
+    # and the colour masked by it
+    src = _rand_img(7, 90, 70)
+    color = (20, 40, 60, 200)
+    got = orc.shadow(src, (3.25, -1.5), 2, 4, color)
+    mask = orc.new_image(90, 70)
+    orc.draw_transform(mask, src, T(3.25, -1.5), 17)
+    orc.spread(mask, 2)
+    orc.blur(mask, 4)
+    want = _canvas(orc, 90, 70, color)
+    orc.draw(want, mask, 0, 0, 16)
+    assert np.array_equal(got, want)
+    # the sub-pixel shift changes the shadow's edge
+    assert not np.array_equal(got, orc.shadow(src, (3, -1), 2, 4, color))
