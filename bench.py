@@ -271,6 +271,71 @@ def cpu_baseline(cmds, canvas, seconds):
             "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
 
 
+def draw_extra(px, WL, reps=5):
+    """Transformed draws and paints on the 4096x4096 canvas (images.nim:
+    531-683, paths.nim:2115-2142): a 3000x3000 image rotated and scaled,
+    translated by a fractional offset, drawn tiled, and a heart-shaped path
+    filled with a radial gradient and with an image paint, each timed with
+    HIP events on the library stream (median of `reps`)."""
+    import numpy as np
+    rng = np.random.default_rng(8)
+    W = H = 4096
+    base = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
+    base[..., :3] = np.minimum(base[..., :3], base[..., 3:4])
+    pic = rng.integers(0, 256, (3000, 3000, 4), dtype=np.uint8)
+    pic[..., :3] = np.minimum(pic[..., :3], pic[..., 3:4])
+    src = px.Image.from_array(base)
+    work = px.Image(W, H)
+    b = px.Image.from_array(pic)
+    c30, s30 = np.float32(np.cos(np.pi / 6)), np.float32(np.sin(np.pi / 6))
+    rot = np.array([1.2 * c30, 1.2 * s30, 0, -1.2 * s30, 1.2 * c30, 0, 2048, -500, 1], np.float32)
+    frac = np.array([1, 0, 0, 0, 1, 0, 500.5, 300.25, 1], np.float32)
+    tile = np.array([0.8 * 0.985, 0.8 * 0.174, 0, -0.8 * 0.174, 0.8 * 0.985, 0, 0, 0, 1], np.float32)
+    heart = ("M 10,30 A 20,20 0,0,1 50,30 A 20,20 0,0,1 90,30 Q 90,60 50,90 Q 10,60 10,30 z")
+    hm = np.array([40, 0, 0, 0, 40, 0, 48, 248, 1], np.float32)
+    grad = px.Paint(px.PaintKind.RadialGradientPaint, blendMode=px.BlendMode.Multiply, opacity=0.8,
+                    gradientHandlePositions=[(2048, 2048), (3500, 2048), (2048, 3000)],
+                    gradientStops=[px.ColorStop((1, 0, 0, 1), 0), px.ColorStop((0, 0, 1, 0.5), 0.7),
+                                   px.ColorStop((0, 1, 0, 1), 1)])
+    ipaint = px.Paint(px.PaintKind.ImagePaint, image=b, imageMat=rot)
+
+    def timed_op(fn):
+        ms = []
+        for _ in range(reps):
+            px.lib().pxh_image_copy(work.handle, src.handle)
+            px.event_record(0)
+            fn()
+            px.event_record(1)
+            ms.append(px.event_elapsed_ms(0, 1))
+        return sorted(ms)[len(ms) // 2]
+
+    out = {"workload": "draws and paints on a 4096x4096 RGBA8 canvas: a 3000x3000 source; "
+                       "algorithmic bytes 8 per canvas pixel written plus the source once"}
+    for name, fn, npx_written in (
+            ("draw_rot30_scale1.2", lambda: work.draw(b, transform=rot), None),
+            ("draw_translate_frac", lambda: work.draw(b, transform=frac), 3000 * 3000),
+            ("draw_tiled_rot10_scale0.8", lambda: work.drawTiled(b, tile), W * H),
+            ("fill_radial_gradient", lambda: work.fillPath(heart, grad, transform=hm), W * H),
+            ("fill_image_paint", lambda: work.fillPath(heart, ipaint, transform=hm), W * H)):
+        ms = timed_op(fn)
+        rec = {"ms": round(ms, 4)}
+        if npx_written:
+            rec["algorithmic_gbs"] = round((8.0 * npx_written + 4.0 * 3000 * 3000) / (ms / 1e3) / 1e9, 1)
+        out[name] = rec
+    return out
+
+
+def calibrate(px, reps=3):
+    """A plain device copy of a 64 MiB image, for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    a = px.Image(4096, 4096)
+    b = px.Image(4096, 4096)
+    for _ in range(reps):
+        px.lib().pxh_image_copy(b.handle, a.handle)
+    px.synchronize()
+    print(json.dumps({"calibration": "pxh_image_copy of 67108864 bytes", "copies": reps}), flush=True)
+
+
 def star_extra(px, WL, reps=50):
     import oracle
     cmd, w, h = WL.star512()
@@ -304,6 +369,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--no-glyphs", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only copy a 64 MiB image (counter calibration under rocprofv3)")
+    ap.add_argument("--only", default=None, choices=["image_ops", "draw", "glyphs", "serving"],
+                    help="run one extra section and print it (profiling)")
     ap.add_argument("--shape", type=int, default=None,
                     help="fixed phase-A launch shape instead of the autotuner's choice (profiling)")
     args = ap.parse_args()
@@ -321,6 +390,14 @@ def main():
     from pixie_amd import workloads as WL
 
     px.init()
+    if args.calibrate:
+        calibrate(px)
+        return
+    if args.only:
+        fn = {"image_ops": image_ops_extra, "draw": draw_extra, "glyphs": glyph_extra,
+              "serving": serve_extra}[args.only]
+        print(json.dumps({args.only: fn(px, WL)}), flush=True)
+        return
     t = time.perf_counter()
     cmds, canvas = WL.mixed_scene(n_paths=args.paths)
     log("scene: %d calls generated in %.1fs" % (len(cmds), time.perf_counter() - t))
@@ -387,7 +464,7 @@ def main():
     if os.path.exists(tpath):
         with open(tpath) as f:
             tr = json.load(f)
-        if tr.get("kernel") == kname and args.form == "batched":
+        if tr.get("kernel") == kname and args.form == "batched" and tr.get("shape") == shape:
             result["roofline"]["traffic"] = tr["traffic_bytes_per_launch"]
             result["roofline"]["traffic_source"] = "profiles/traffic.json: " + tr["source"]
     if not args.no_cpu:
@@ -397,6 +474,7 @@ def main():
         if not args.no_glyphs:
             extra["glyph_atlas"] = glyph_extra(px, WL)
         extra["image_ops"] = image_ops_extra(px, WL)
+        extra["draw"] = draw_extra(px, WL)
         extra["serving"] = serve_extra(px, WL)
         if args.form == "batched":
             pimg = px.Image.from_array(canvas)

@@ -190,7 +190,8 @@ int pxh_scene_render(pxh_scene* scene, pxh_image* img);
  * that key takes the cached shape without timing. */
 int pxh_scene_autotune(pxh_scene* scene, pxh_image* img, int32_t reps, int32_t* shape,
                        float* ms);
-/* Sets the launch shape directly (0..3; 1 is the default). */
+/* Sets the launch shape directly (0..5; 0, one launch of every task, is the
+ * default). */
 int pxh_scene_set_shape(pxh_scene* scene, int32_t shape);
 /* 1 when renders of this scene replay a captured HIP graph (after the first
  * render at a given image, lane and launch shape), 0 when they run eagerly. */
