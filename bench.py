@@ -174,10 +174,15 @@ def image_ops_extra(px, WL):
     # the mask: 500 random circles filled NonZero into an 8-bit plane
     circ = WL.circles_mask_cmds()
     mask = px.Image(W, H, a8=True)
-    px.event_record(0)
-    mask.fillPath(circ.cmds, circ.color)
-    px.event_record(1)
-    out["mask_fill_500_circles_ms"] = round(px.event_elapsed_ms(0, 1), 4)
+    fills = []
+    for k in range(4):  # the first call also sizes the device pools
+        px.lib().pxh_image_fill(mask.handle, 0)
+        px.event_record(0)
+        mask.fillPath(circ.cmds, circ.color)
+        px.event_record(1)
+        fills.append(px.event_elapsed_ms(0, 1))
+    out["mask_fill_500_circles_ms"] = round(sorted(fills[1:])[1], 4)
+    out["mask_fill_500_circles_first_call_ms"] = round(fills[0], 4)
     for name, mode in (("subtract", px.BlendMode.SubtractMask), ("intersect", px.BlendMode.Mask),
                        ("exclude", px.BlendMode.ExcludeMask)):
         best, med = timed_op(lambda: work.draw(mask, 0, 0, mode))

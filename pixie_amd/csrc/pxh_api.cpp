@@ -58,6 +58,8 @@ struct pxh_scene {
   pxh::SceneCmd* dCmds = nullptr;
   pxh::ScenePoly* dPolys = nullptr;
   int32_t* dOrder = nullptr;  // geometry workgroup -> call, longest call first
+  int32_t nBig = 0;           // calls of more than 256 commands (first in order)
+  int32_t bigCmds = 0;        // their commands
   float* dDashes = nullptr;
   size_t dItemsCap = 0, dCmdsCap = 0, dPolysCap = 0, dOrderCap = 0, dDashesCap = 0;
   // per-render buffers
@@ -149,6 +151,7 @@ struct Ctx {
   DevBuf magBuf;     // a magnified source for the integer composite
   DevBuf paintFill;  // an image paint drawn onto a transparent canvas
   DevBuf drawRows;   // drawSmooth: per-row spans and run start positions
+  pxh_scene* bigScene = nullptr;  // per-call fills / strokes of many commands
   int32_t* errFlag = nullptr;
   // Per lane: error bits of scene renders that stopped before compose (an
   // overflowed pool); read and cleared when the lane is synchronized.  A
@@ -772,6 +775,9 @@ int scene_alloc(pxh_scene* sc) {
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
   p.itemOrder = sc->dOrder;
+  p.nBig = sc->nBig;
+  p.bigCmdStart = sc->dOrder + sc->items.size();
+  p.bigCmds = sc->bigCmds;
   p.dashes = sc->dDashes;
   p.nItems = (int32_t)nI;
   p.geo = (ItemGeom*)(b + oGeo);
@@ -858,12 +864,27 @@ int scene_upload(pxh_scene* sc, int lane) {
       if (it.stroke && !it.strokeNone) c *= (it.join == 1 || it.cap == 1) ? 16 : 4;
       cost[i] = c;
     }
+    // calls of more than 256 commands first: their geometry runs in
+    // 1024-thread workgroups of a launch of its own
+    auto big = [&](int32_t i) { return sc->items[i].cmdCount > 256; };
     sc->order.resize(n);
     for (size_t i = 0; i < n; i++) sc->order[i] = (int32_t)i;
-    std::stable_sort(sc->order.begin(), sc->order.end(),
-                     [&](int32_t a, int32_t b) { return cost[a] > cost[b]; });
-    if (int r = scene_buf(&sc->dOrder, &sc->dOrderCap, n)) return r;
-    PXH_HIP(hipMemcpyAsync(sc->dOrder, sc->order.data(), n * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    std::stable_sort(sc->order.begin(), sc->order.end(), [&](int32_t a, int32_t b) {
+      if (big(a) != big(b)) return big(a);
+      return cost[a] > cost[b];
+    });
+    sc->nBig = 0;
+    for (size_t i = 0; i < n; i++) sc->nBig += big((int32_t)i) ? 1 : 0;
+    // after the order: the big calls' command prefix
+    sc->bigCmds = 0;
+    for (int32_t k = 0; k < sc->nBig; k++) {
+      sc->order.push_back(sc->bigCmds);
+      sc->bigCmds += sc->items[sc->order[k]].cmdCount;
+    }
+    sc->order.push_back(sc->bigCmds);
+    const size_t m = sc->order.size();
+    if (int r = scene_buf(&sc->dOrder, &sc->dOrderCap, m)) return r;
+    PXH_HIP(hipMemcpyAsync(sc->dOrder, sc->order.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, st));
   }
   sc->lane = lane;
   // first estimates of the pools; a render that overflows grows them.  The
@@ -898,6 +919,9 @@ int scene_launch(pxh_scene* sc, pxh_image* img, int lane, bool checked = false) 
   p.cmds = sc->dCmds;
   p.polys = sc->dPolys;
   p.itemOrder = sc->dOrder;
+  p.nBig = sc->nBig;
+  p.bigCmdStart = sc->dOrder + sc->items.size();
+  p.bigCmds = sc->bigCmds;
   p.dashes = sc->dDashes;
   const RowShape shape = kRowShapes[sc->rowShape];
   p.ldsCap = shape.ldsCap;
@@ -1112,10 +1136,43 @@ int pxh_parse_path(const char* path, float** cmds, int32_t* ncmds) {
 
 void pxh_free(void* p) { free(p); }
 
+// A fillPath / strokePath of many commands is flattened on the device, where
+// its commands spread over the CUs, instead of on one host thread: it is
+// recorded as the only call of an internal scene and rendered (the batched
+// form gives the per-call image bit for bit).  The scene keeps no graph (it
+// is re-recorded per call).  A call the device reports an error for draws
+// nothing there and goes through the host path, which raises the error with
+// pixie's message.
+constexpr int32_t kDeviceGeomFloats = 4096;  // about 500 cubics
+static pxh_scene* big_call_scene() {
+  if (!g.bigScene) {
+    g.bigScene = pxh_scene_new();
+    if (g.bigScene) g.bigScene->graphsOff = true;
+  }
+  return g.bigScene;
+}
+// 0 when drawn; -1 when the host path must run; else a status to return
+static int big_call_render(pxh_image* img) {
+  pxh_scene* sc = g.bigScene;
+  if (pxh_scene_count(sc) == 0) return 0;  // nothing to draw (a stroke of width <= 0)
+  if (int r = pxh_scene_render_on(sc, img, 0)) return r == kErrCapacity ? -1 : r;
+  int32_t st = 0;
+  if (int r = pxh_scene_status(sc, &st, 1)) return r;
+  return st == kOk ? 0 : -1;
+}
+
 int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9, uint32_t rgbx,
                   int32_t winding_rule, int32_t blend_mode) {
   PXH_INIT();
   if (int r = check_image(img)) return r;
+  if (ncmds >= kDeviceGeomFloats && big_call_scene()) {
+    pxh_scene* sc = g.bigScene;
+    pxh_scene_clear(sc);
+    int r = pxh_scene_fill_path(sc, cmds, ncmds, mat9, rgbx, winding_rule, blend_mode);
+    if (r) return r;
+    r = big_call_render(img);
+    if (r >= 0) return r;
+  }
   if (!mat9) mat9 = kIdentity;
   Shapes sh;
   std::string err;
@@ -1586,6 +1643,15 @@ int pxh_stroke_path(pxh_image* img, const float* cmds, int32_t ncmds, const floa
                     float miter_limit, const float* dashes, int32_t ndashes, int32_t blend_mode) {
   PXH_INIT();
   if (int r = check_image(img)) return r;
+  if (ncmds >= kDeviceGeomFloats && big_call_scene()) {
+    pxh_scene* sc = g.bigScene;
+    pxh_scene_clear(sc);
+    int r = pxh_scene_stroke_path(sc, cmds, ncmds, mat9, rgbx, stroke_width, line_cap, line_join,
+                                  miter_limit, dashes, ndashes, blend_mode);
+    if (r) return r;
+    r = big_call_render(img);
+    if (r >= 0) return r;
+  }
   if (!mat9) mat9 = kIdentity;
   float ps = pixel_scale(mat9);
   Shapes sh, st;

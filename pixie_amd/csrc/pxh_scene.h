@@ -141,6 +141,9 @@ struct ScenePools {
   const int32_t* itemOrder;  // geometry workgroup -> call
   const float* dashes;       // dash patterns of the recorded strokes
   int32_t nItems;
+  int32_t nBig;              // calls of more than 256 commands, first in itemOrder
+  const int32_t* bigCmdStart;  // their command counts' exclusive prefix (nBig + 1)
+  int32_t bigCmds;           // bigCmdStart[nBig]
   int32_t W, H;              // target size
   // per-call outputs
   ItemGeom* geo;

@@ -12,6 +12,7 @@
 namespace pxh {
 
 constexpr int kBlock = 256;
+constexpr int kBigGeomBlock = 1024;  // geometry workgroup of a call of more than kBlock commands
 constexpr int kStagePts = 4096;  // LDS points of the geometry stage (32 KB)
 
 // A value that is the same in every lane, moved to a scalar register.
@@ -24,7 +25,7 @@ constexpr int kChunksB = kWinB / 256;
 // block helpers (256 threads = 4 waves)
 // ---------------------------------------------------------------------------
 template <int NB>
-__device__ int block_excl_scan_n(int v, int* total, int* sh) {
+__device__ __forceinline__ int block_excl_scan_n(int v, int* total, int* sh) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   int wt;
   int x = wave_excl_scan(v, &wt);
@@ -106,7 +107,7 @@ struct StageEmit {
 // evaluate the same points).  Every lane of the group calls emit alike.
 constexpr int kFlatGroup = 8;
 template <class Emit>
-__device__ int32_t flatten_cubic_group(V2 at, V2 c1, V2 c2, V2 to, float errSq, Emit& emit, int u,
+__device__ __forceinline__ int32_t flatten_cubic_group(V2 at, V2 c1, V2 c2, V2 to, float errSq, Emit& emit, int u,
                                        int gbase) {
   float t = 0.0f, step = 1.0f;
   V2 prev = at;
@@ -166,7 +167,7 @@ struct StoreEmit {
 
 // Points command i emits (its non-degenerate pieces' end points).
 template <class Emit>
-__device__ int32_t cmd_emit(const SceneCmd* cmds, int i, float errSq, Emit& e) {
+__device__ __forceinline__ int32_t cmd_emit(const SceneCmd* cmds, int i, float errSq, Emit& e) {
   const SceneCmd& c = cmds[i];
   int k = (int)c.kind;
   if (k == 1) return kOk;  // Move
@@ -405,29 +406,162 @@ __device__ __forceinline__ void stroke_point(const float2* P, int k, int n, cons
 // ---------------------------------------------------------------------------
 // geometry: one workgroup per call
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
-  __shared__ int sh[8];
+// Stages 6-7 of the geometry kernel: the call's bounds (merged from every
+// thread's `bd`) give its fill locals and band layout (fill_locals,
+// band_layout), then every segment gets its band range and AA bit, and every
+// 64-segment chunk the band range of its segments.
+template <int NB>
+__device__ __forceinline__ void geom_finish(const ScenePools& P, const SceneItem& I, int it, Bounds bd, int nseg, int sBase,
+                            Bounds* sBd, FillInfo& sF) {
+  const int tid = threadIdx.x;
+  // 6. bounds -> fill locals, band layout
+  for (int o = 32; o > 0; o >>= 1) {
+    Bounds ob;
+    ob.xMin = __shfl_xor(bd.xMin, o, 64);
+    ob.xMax = __shfl_xor(bd.xMax, o, 64);
+    ob.yMin = __shfl_xor(bd.yMin, o, 64);
+    ob.yMax = __shfl_xor(bd.yMax, o, 64);
+    bd = bounds_merge(bd, ob);
+  }
+  if ((tid & 63) == 0) sBd[tid >> 6] = bd;
+  __syncthreads();
+  if (tid == 0) {
+    Bounds b = sBd[0];
+    for (int w = 1; w < NB / 64; w++) b = bounds_merge(b, sBd[w]);
+    FillInfo f = {};
+    f.rgbx = I.rgbx;
+    f.mode = I.mode;
+    f.rule = I.stroke ? kNonZero : I.rule;
+    int32_t r = fill_locals(b, false, P.W, P.H, &f);
+    f.status = r;
+    if (r == kOk && f.pathWidth > 0) band_layout(&f, nseg);
+    P.fills[it] = f;
+    sF = f;
+  }
+  __syncthreads();
+  // 7. band range of every segment (partitionSegments 1201-1213) and its
+  // requiresAntiAliasing bit, and the band range of each 64-segment chunk
+  const FillInfo fl = sF;
+  if (fl.status == kOk && fl.pathWidth > 0) {
+    // four rounds at a time, their loads first
+    for (int j0 = 0; j0 < nseg; j0 += 4 * NB) {
+      float4 sg[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + u * NB + tid;
+        sg[u] = j < nseg ? P.segs[sBase + j] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + u * NB + tid;
+        int lo = 0x7fffffff, hi = -1;
+        if (j < nseg) {
+          uint32_t l = 0, h = 0;
+          if (fl.nParts > 1)
+            partition_range(sg[u].y, sg[u].w, (uint32_t)fl.startY, (uint32_t)fl.bandH, (uint32_t)fl.nParts,
+                            &l, &h);
+          const bool aa = seg_requires_aa(v2(sg[u].x, sg[u].y), v2(sg[u].z, sg[u].w));
+          P.segBand[sBase + j] = make_int2((int)l, (int)h | (aa ? kSegAA : 0));
+          lo = (int)l;
+          hi = (int)h;
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+          lo = min(lo, __shfl_xor(lo, o, 64));
+          hi = max(hi, __shfl_xor(hi, o, 64));
+        }
+        if ((tid & 63) == 0 && j < nseg) P.segChunk[(sBase + j) >> 6] = make_int2(lo, hi);
+      }
+    }
+  }
+}
+
+// Thread per command of the calls of more than kBlock commands (the first
+// nBig calls of itemOrder; bigCmdStart holds their command prefix): counts
+// its points (emit = 0; a negative count is an error code), or writes them
+// at the offsets the front half computed (emit = 1).
+__global__ void __launch_bounds__(256) scene_big_cmd_kernel(ScenePools P, int emit) {
+  const int gid = blockIdx.x * 256 + threadIdx.x;
+  if (gid >= P.bigCmdStart[P.nBig]) return;
+  int lo = 0, hi = P.nBig;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (P.bigCmdStart[mid] <= gid) lo = mid;
+    else hi = mid;
+  }
+  const int it = P.itemOrder[lo];
+  const SceneItem& I = P.items[it];
+  const int i = I.cmdBase + (gid - P.bigCmdStart[lo]);
+  const bool xf = !I.stroke && !I.identity;
+  if (!emit) {
+    CountEmit ce;
+    const int32_t r = cmd_emit(P.cmds, i, I.errSq, ce);
+    P.cmdCount[i] = r != kOk ? -r : ce.n;
+    return;
+  }
+  if (P.geo[it].status != kOk || P.cmdCount[i] <= 0) return;
+  const int o = P.geo[it].ptBase + P.cmdOff[i];
+  StoreEmit se{P.pts + o, P.ptPoly + o, P.cmds[i].pad[0], I.mat, xf};
+  cmd_emit(P.cmds, i, I.errSq, se);
+}
+
+// Geometry kernel stages by MODE: kGeomAll runs every stage for a call;
+// calls of more than kBlock commands run in four launches so that their
+// commands spread over the CUs: scene_big_cmd_kernel counts every command's
+// points, kGeomFront takes those counts through stage 3 (offsets, the
+// call's point range, start and closing points), scene_big_cmd_kernel emits
+// the points, and kGeomBack runs stages 5-7.
+constexpr int kGeomAll = 0, kGeomFront = 1, kGeomBack = 2;
+
+template <int NB, int MODE>
+__global__ void __launch_bounds__(NB) scene_geom_kernel(ScenePools P, int orderBase) {
+  __shared__ int sh[NB / 64];
   __shared__ int sBase;
-  __shared__ Bounds sBd[kBlock / 64];
+  __shared__ Bounds sBd[NB / 64];
   __shared__ FillInfo sF;
   __shared__ int sErr;
   __shared__ float2 sPts[kStagePts];
-  __shared__ int sPre[kBlock + 1];
-  const int it = P.itemOrder[blockIdx.x];
+  __shared__ int sPre[NB + 1];
+  const int it = P.itemOrder[orderBase + blockIdx.x];
   const int tid = threadIdx.x;
-  const SceneItem I = P.items[it];
+  const SceneItem& I = P.items[it];
   const bool xf = !I.stroke && !I.identity;  // fills transform points here
   if (tid == 0) sErr = kOk;
   __syncthreads();
 
-  // 1. points per command.  A call of at most kBlock commands keeps each
+  int total = 0, ptBase = 0;
+  if (MODE == kGeomBack) {
+    const ItemGeom gf = P.geo[it];
+    if (gf.status != kOk) return;  // the front wrote the error
+    total = gf.ptCount;
+    ptBase = gf.ptBase;
+    goto segments;
+  }
+  if (MODE == kGeomFront) {
+    // the counts of scene_big_cmd_kernel: a negative count is an error code
+    for (int c0 = 0; c0 < I.cmdCount; c0 += NB) {
+      const int i = I.cmdBase + c0 + tid;
+      if (c0 + tid < I.cmdCount && P.cmdCount[i] < 0) atomicMax(&sErr, -P.cmdCount[i]);
+    }
+    __syncthreads();
+    if (sErr != kOk) {
+      if (tid == 0) {
+        P.geo[it].status = sErr;
+        P.geo[it].segCount = 0;
+        P.fills[it].status = sErr;
+        P.fills[it].pathWidth = 0;
+      }
+      return;
+    }
+  }
+  {
+  // 1. points per command.  A call of at most NB commands keeps each
   // command's points in an LDS slice of its own (thread tid <-> command tid
   // in stages 1 and 4), so stage 4 copies them unless a slice overflowed.
-  const bool stage = I.cmdCount <= kBlock;
+  const bool stage = MODE == kGeomAll && I.cmdCount <= NB;
   const int slice = stage && I.cmdCount > 0 ? kStagePts / I.cmdCount : 0;
   bool staged = false;
   int myN = 0;
-  if (stage && I.cmdCount <= kBlock / kFlatGroup) {
+  if (stage && I.cmdCount <= NB / kFlatGroup) {
     // few commands: kFlatGroup lanes per command, so a cubic's adaptive
     // steps are tried kFlatGroup halvings at a time
     const int c = tid / kFlatGroup, u = tid % kFlatGroup;
@@ -453,8 +587,8 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       myN = sPre[tid];
       staged = myN <= slice;
     }
-  } else {
-    for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
+  } else if (MODE == kGeomAll) {
+    for (int c0 = 0; c0 < I.cmdCount; c0 += NB) {
       int i = I.cmdBase + c0 + tid;
       if (c0 + tid < I.cmdCount) {
         StageEmit se{sPts + tid * slice, slice, I.mat, xf};
@@ -472,14 +606,13 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   // command offsets take two LDS reads instead of a loop over commands.
   if (stage) {
     int tot;
-    const int pre = block_excl_scan(tid < I.cmdCount ? myN : 0, &tot, sh);
+    const int pre = block_excl_scan_n<NB>(tid < I.cmdCount ? myN : 0, &tot, sh);
     if (tid < I.cmdCount) sPre[tid] = pre;
     if (tid == 0) sPre[I.cmdCount] = tot;
     __syncthreads();
   }
   // 2. points per subpath: start + pieces (+ the implicit closing point of a fill)
-  int total = 0;
-  for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
+  for (int q0 = 0; q0 < I.polyCount; q0 += NB) {
     int q = I.polyBase + q0 + tid;
     int np = 0;
     if (q0 + tid < I.polyCount) {
@@ -501,7 +634,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
       }
     }
     int t;
-    int off = block_excl_scan(np, &t, sh);
+    int off = block_excl_scan_n<NB>(np, &t, sh);
     if (q0 + tid < I.polyCount) {
       P.polyPt[q] = total + off;
       P.polyN[q] = np;
@@ -520,7 +653,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
     sBase = base;
   }
   __syncthreads();
-  const int ptBase = sBase;
+  ptBase = sBase;
   if (sErr != kOk) {
     if (tid == 0) {
       P.geo[it].status = sErr;
@@ -535,7 +668,7 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
   int32_t* ptSeg = P.ptSeg + ptBase;  // segments of each point (stage 5)
 
   // 3. per-command point offsets, start and closing points (thread per subpath)
-  for (int q0 = 0; q0 < I.polyCount; q0 += kBlock) {
+  for (int q0 = 0; q0 < I.polyCount; q0 += NB) {
     int ql = q0 + tid;
     if (ql < I.polyCount) {
       int q = I.polyBase + ql;
@@ -565,8 +698,19 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
     }
   }
   __syncthreads();
+  if (MODE == kGeomFront) {
+    // stage 4 runs in scene_big_cmd_kernel; the back half reads these
+    if (tid == 0) {
+      ItemGeom gf = {};
+      gf.ptBase = ptBase;
+      gf.ptCount = total;
+      gf.status = kOk;
+      P.geo[it] = gf;
+    }
+    return;
+  }
   // 4. emit points (thread per command)
-  for (int c0 = 0; c0 < I.cmdCount; c0 += kBlock) {
+  for (int c0 = 0; c0 < I.cmdCount; c0 += NB) {
     int i = I.cmdBase + c0 + tid;
     if (c0 + tid < I.cmdCount && P.cmdCount[i] > 0) {
       // subpath of command i (the host stores it in pad[0] of every command)
@@ -592,135 +736,116 @@ __global__ void __launch_bounds__(kBlock) scene_geom_kernel(ScenePools P) {
     }
   }
   __syncthreads();
+  }
+segments:
+  float2* pts = P.pts + ptBase;
+  int32_t* ptPoly = P.ptPoly + ptBase;
+  int32_t* ptSeg = P.ptSeg + ptBase;  // segments of each point (stage 5)
 
-  // 5. segments: count, allocate, emit (thread per point)
+  // 5. segments: count, allocate, emit (thread per point).  A fill makes at
+  // most one segment per point, so it allocates that bound and emits in one
+  // pass; a stroke counts first.
   int nseg = 0;
-  for (int pass = 0; pass < 2; pass++) {
-    int segOff = 0;
-    Bounds bd = bounds_init();
-    for (int j0 = 0; j0 < total; j0 += kBlock) {
-      int j = j0 + tid;
-      SegSink sink;
-      int cnt = 0;
-      if (pass == 1) {
-        // the counts of pass 0 (this thread wrote them)
-        if (j < total) cnt = ptSeg[j];
-      } else if (j < total) {
-        int ql = ptPoly[j];
-        int q = I.polyBase + ql;
-        int pb = P.polyPt[q], np = P.polyN[q];
-        int k = j - pb;
-        if (!I.stroke) {
-          // shapesToSegments: candidate (prev point -> this point); the first
-          // point pairs with the polygon's last point
-          V2 a = k == 0 ? v2(pts[pb + np - 1].x, pts[pb + np - 1].y) : v2(pts[j - 1].x, pts[j - 1].y);
-          V2 b = v2(pts[j].x, pts[j].y);
-          V2 v1 = v2(a.x, quantize_y(a.y)), v2_ = v2(b.x, quantize_y(b.y));
-          if (!(k == 0 && veq(v1, v2_))) {
-            Seg s;
-            if (make_segment(v1, v2_, &s)) cnt = 1;
+  const bool onePass = !I.stroke;
+  if (onePass) {
+    if (tid == 0) {
+      // segments start on a 64-segment chunk of their own
+      int base = atomicAdd(&P.counters[kCntSegs], (total + 63) & ~63);
+      if (base + total > P.segCap) {
+        atomicOr(&P.counters[kCntError], 1);
+        sErr = kErrCapacity;
+      }
+      sBase = base;
+    }
+    __syncthreads();
+    if (sErr == kOk) {
+      // four consecutive points per thread per round, their loads first
+      int segOff = 0;
+      Bounds bd = bounds_init();
+      for (int j0 = 0; j0 < total; j0 += 4 * NB) {
+        const int jb = j0 + 4 * tid;
+        Seg fs[4];
+        int c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int j = jb + u;
+          c[u] = 0;
+          if (j < total) {
+            const int q = I.polyBase + ptPoly[j];
+            const int pb = P.polyPt[q], np = P.polyN[q];
+            // shapesToSegments: candidate (prev point -> this point); the
+            // first point pairs with the polygon's last point
+            const float2 pa = j == pb ? pts[pb + np - 1] : pts[j - 1];
+            const float2 pc = pts[j];
+            const V2 v1 = v2(pa.x, quantize_y(pa.y)), v2_ = v2(pc.x, quantize_y(pc.y));
+            if (!(j == pb && veq(v1, v2_)) && make_segment(v1, v2_, &fs[u])) c[u] = 1;
           }
-        } else if (!I.strokeNone) {
-          stroke_point(pts + pb, k, np, I, P.dashes + I.dashBase, sink);
+        }
+        int t;
+        int w = segOff + block_excl_scan_n<NB>(c[0] + c[1] + c[2] + c[3], &t, sh);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (c[u]) {
+            P.segs[sBase + w] = make_float4(fs[u].at.x, fs[u].at.y, fs[u].to.x, fs[u].to.y);
+            P.segW[sBase + w] = (int8_t)fs[u].w;
+            bounds_add(&bd, fs[u]);
+            w++;
+          }
+        }
+        segOff += t;
+      }
+      nseg = segOff;
+      geom_finish<NB>(P, I, it, bd, nseg, sBase, sBd, sF);
+    }
+  } else {
+    // a stroke: count the segments of every point, allocate, then emit
+    for (int pass = 0; pass < 2 && sErr == kOk; pass++) {
+      int segOff = 0;
+      Bounds bd = bounds_init();
+      for (int j0 = 0; j0 < total; j0 += NB) {
+        int j = j0 + tid;
+        int cnt = 0;
+        if (pass == 1) {
+          // the counts of pass 0 (this thread wrote them)
+          if (j < total) cnt = ptSeg[j];
+        } else if (j < total && !I.strokeNone) {
+          int q = I.polyBase + ptPoly[j];
+          int pb = P.polyPt[q], np = P.polyN[q];
+          SegSink sink;
+          stroke_point(pts + pb, j - pb, np, I, P.dashes + I.dashBase, sink);
           cnt = sink.n;
           if (sink.err) atomicMax(&sErr, kErrCapacity);
         }
-        ptSeg[j] = cnt;
-      }
-      int t;
-      int off = block_excl_scan(cnt, &t, sh);
-      if (pass == 1 && j < total && cnt > 0) {
-        float4* so = P.segs + sBase + segOff + off;
-        int8_t* sw = P.segW + sBase + segOff + off;
-        if (!I.stroke) {
-          int ql = ptPoly[j];
-          int q = I.polyBase + ql;
-          int pb = P.polyPt[q], np = P.polyN[q];
-          int k = j - pb;
-          V2 a = k == 0 ? v2(pts[pb + np - 1].x, pts[pb + np - 1].y) : v2(pts[j - 1].x, pts[j - 1].y);
-          V2 b = v2(pts[j].x, pts[j].y);
-          Seg s;
-          make_segment(v2(a.x, quantize_y(a.y)), v2(b.x, quantize_y(b.y)), &s);
-          so[0] = make_float4(s.at.x, s.at.y, s.to.x, s.to.y);
-          sw[0] = (int8_t)s.w;
-          bounds_add(&bd, s);
-        } else {
+        if (pass == 0 && j < total) ptSeg[j] = cnt;
+        int t;
+        int off = block_excl_scan_n<NB>(cnt, &t, sh);
+        if (pass == 1 && j < total && cnt > 0) {
           SegSink out;
-          out.out = so;
-          out.outW = sw;
+          out.out = P.segs + sBase + segOff + off;
+          out.outW = P.segW + sBase + segOff + off;
           out.bd = bd;
-          int ql = ptPoly[j];
-          int q = I.polyBase + ql;
+          int q = I.polyBase + ptPoly[j];
           int pb = P.polyPt[q], np = P.polyN[q];
           stroke_point(pts + pb, j - pb, np, I, P.dashes + I.dashBase, out);
           bd = out.bd;
         }
+        segOff += t;
       }
-      segOff += t;
-    }
-    if (pass == 0) {
-      nseg = segOff;
-      __syncthreads();
-      if (tid == 0) {
-        // segments start on a 64-segment chunk of their own
-        int base = atomicAdd(&P.counters[kCntSegs], (nseg + 63) & ~63);
-        if (base + nseg > P.segCap) {
-          atomicOr(&P.counters[kCntError], 1);
-          sErr = kErrCapacity;
-        }
-        sBase = base;
-      }
-      __syncthreads();
-      if (sErr != kOk) break;
-    } else {
-      // 6. bounds -> fill locals, band layout
-      for (int o = 32; o > 0; o >>= 1) {
-        Bounds ob;
-        ob.xMin = __shfl_xor(bd.xMin, o, 64);
-        ob.xMax = __shfl_xor(bd.xMax, o, 64);
-        ob.yMin = __shfl_xor(bd.yMin, o, 64);
-        ob.yMax = __shfl_xor(bd.yMax, o, 64);
-        bd = bounds_merge(bd, ob);
-      }
-      if ((tid & 63) == 0) sBd[tid >> 6] = bd;
-      __syncthreads();
-      if (tid == 0) {
-        Bounds b = sBd[0];
-        for (int w = 1; w < kBlock / 64; w++) b = bounds_merge(b, sBd[w]);
-        FillInfo f = {};
-        f.rgbx = I.rgbx;
-        f.mode = I.mode;
-        f.rule = I.stroke ? kNonZero : I.rule;
-        int32_t r = fill_locals(b, false, P.W, P.H, &f);
-        f.status = r;
-        if (r == kOk && f.pathWidth > 0) band_layout(&f, nseg);
-        P.fills[it] = f;
-        sF = f;
-      }
-      __syncthreads();
-      // 7. band range of every segment (partitionSegments 1201-1213) and its
-      // requiresAntiAliasing bit, and the band range of each 64-segment chunk
-      const FillInfo fl = sF;
-      if (fl.status == kOk && fl.pathWidth > 0) {
-        for (int j0 = 0; j0 < nseg; j0 += kBlock) {
-          const int j = j0 + tid;
-          int lo = 0x7fffffff, hi = -1;
-          if (j < nseg) {
-            const float4 sg = P.segs[sBase + j];
-            uint32_t l = 0, h = 0;
-            if (fl.nParts > 1)
-              partition_range(sg.y, sg.w, (uint32_t)fl.startY, (uint32_t)fl.bandH, (uint32_t)fl.nParts, &l, &h);
-            const bool aa = seg_requires_aa(v2(sg.x, sg.y), v2(sg.z, sg.w));
-            P.segBand[sBase + j] = make_int2((int)l, (int)h | (aa ? kSegAA : 0));
-            lo = (int)l;
-            hi = (int)h;
+      if (pass == 0) {
+        nseg = segOff;
+        __syncthreads();
+        if (tid == 0) {
+          // segments start on a 64-segment chunk of their own
+          int base = atomicAdd(&P.counters[kCntSegs], (nseg + 63) & ~63);
+          if (base + nseg > P.segCap) {
+            atomicOr(&P.counters[kCntError], 1);
+            sErr = kErrCapacity;
           }
-          for (int o = 32; o > 0; o >>= 1) {
-            lo = min(lo, __shfl_xor(lo, o, 64));
-            hi = max(hi, __shfl_xor(hi, o, 64));
-          }
-          if ((tid & 63) == 0 && j < nseg) P.segChunk[(sBase + j) >> 6] = make_int2(lo, hi);
+          sBase = base;
         }
+        __syncthreads();
+      } else {
+        geom_finish<NB>(P, I, it, bd, nseg, sBase, sBd, sF);
       }
     }
   }
@@ -1710,7 +1835,21 @@ int scene_launch_clear(const ScenePools& p, hipStream_t s) {
 
 int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
   if (p.nItems <= 0) return 0;
-  hipLaunchKernelGGL(scene_geom_kernel, dim3(p.nItems), dim3(kBlock), 0, s, p);
+  // calls of more than kBlock commands (they come first in itemOrder):
+  // commands flattened by a thread each over the whole device, the rest in
+  // 1024-thread workgroups
+  if (p.nBig > 0) {
+    const int grid = (p.bigCmds + 255) / 256;
+    hipLaunchKernelGGL(scene_big_cmd_kernel, dim3(grid), dim3(256), 0, s, p, 0);
+    hipLaunchKernelGGL((scene_geom_kernel<kBigGeomBlock, kGeomFront>), dim3(p.nBig), dim3(kBigGeomBlock), 0, s,
+                       p, 0);
+    hipLaunchKernelGGL(scene_big_cmd_kernel, dim3(grid), dim3(256), 0, s, p, 1);
+    hipLaunchKernelGGL((scene_geom_kernel<kBigGeomBlock, kGeomBack>), dim3(p.nBig), dim3(kBigGeomBlock), 0, s,
+                       p, 0);
+  }
+  if (p.nItems > p.nBig)
+    hipLaunchKernelGGL((scene_geom_kernel<kBlock, kGeomAll>), dim3(p.nItems - p.nBig), dim3(kBlock), 0, s, p,
+                       p.nBig);
   hipLaunchKernelGGL(scene_plan_kernel, dim3(1), dim3(kScanBlock), 0, s, p);
   return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
 }

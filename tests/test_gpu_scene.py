@@ -329,6 +329,47 @@ def test_scene_band_over_capacity_renders(gpu):
     _compare(gpu, calls, canvas, "band of 9,000 entries")
 
 
+def test_scene_big_calls(gpu, orc):
+    # calls of more than 256 commands take the 1024-thread geometry
+    # workgroups: 300 circles in one fill (1,800 commands), a 400-segment
+    # polyline stroked with round joins and dashes, a 600-command EvenOdd
+    # fill, each between small calls
+    rng = np.random.default_rng(31)
+    W, H = 700, 500
+    circles = Path()
+    for _ in range(300):
+        cx, cy = rng.uniform(0, W), rng.uniform(0, H)
+        circles.circle(cx, cy, rng.uniform(3, 60))
+    poly = Path()
+    poly.move_to(20, 20)
+    for i in range(400):
+        poly.line_to(20 + i * 1.6, 250 + 200 * np.sin(i * 0.11) + rng.uniform(-5, 5))
+    star = Path()
+    star.move_to(350, 250)
+    for i in range(600):
+        a = i * 2.39996
+        r = 40 + 200 * ((i * 7) % 13) / 13.0
+        star.line_to(350 + r * np.cos(a), 250 + r * np.sin(a))
+    star.close_path()
+    small = (False, "M 10 10 L 120 30 L 60 110 Z", None,
+             dict(color=(200, 40, 40, 200), windingRule=0, blendMode=0))
+    calls = [small,
+             (False, circles.array(), None, dict(color=(255, 255, 255, 255), windingRule=0,
+                                                 blendMode=0)),
+             (True, poly.array(), None, dict(color=(20, 100, 200, 220), strokeWidth=6.5,
+                                             lineCap=1, lineJoin=1, miterLimit=4.0,
+                                             dashes=(9, 4, 2, 4), blendMode=2)),
+             small,
+             (False, star.array(), None, dict(color=(40, 160, 60, 180), windingRule=1,
+                                              blendMode=11))]
+    for c in calls:
+        assert c[0] or len(px._cmds(c[1])) > 0
+    assert sum(1 for c in calls if len(px._cmds(c[1])) > 3 * 257) >= 3
+    base = np.random.default_rng(32).integers(0, 256, (H, W, 4), dtype=np.uint8)
+    base[..., :3] = np.minimum(base[..., :3], base[..., 3:4])
+    _compare(gpu, calls, base, "big calls", orc)
+
+
 # Elliptical arcs (flattened when recorded) in every form pixie's tests use:
 # small / large arc, sweep, rotation, relative arcs, arcs inside longer paths.
 ARC_PATHS = [
@@ -605,3 +646,35 @@ def test_scene_full_glyph_atlas(gpu, orc):
         dropped += orc.fill_path(want, cmds, (255, 255, 255, 255))["dropped"]
     assert dropped == 0
     _cov_check(got.reshape(ah, aw), want[..., 3], "10k-glyph atlas vs oracle")
+
+
+def test_per_call_big_paths_through_device_geometry(gpu, orc):
+    # per-call fills and strokes of 4096 floats or more are flattened on the
+    # device (a one-call scene): same image as the oracle, and the host
+    # path's error when the device reports one
+    rng = np.random.default_rng(41)
+    W, H = 600, 400
+    circles = Path()
+    for _ in range(200):
+        circles.circle(rng.uniform(0, W), rng.uniform(0, H), rng.uniform(2, 80))
+    cmds = circles.array()
+    assert len(cmds) >= 4096
+    base = np.random.default_rng(42).integers(0, 256, (H, W, 4), dtype=np.uint8)
+    base[..., :3] = np.minimum(base[..., :3], base[..., 3:4])
+    img = px.Image.from_array(base)
+    img.fillPath(cmds, (30, 60, 90, 200), windingRule=px.EvenOdd, blendMode=3)
+    img.strokePath(cmds, (200, 100, 0, 255), strokeWidth=3.5, lineJoin=px.RoundJoin,
+                   dashes=(6, 3))
+    want = base.copy()
+    st = orc.fill_path(want, cmds, (30, 60, 90, 200), winding=1, mode=3)
+    st2 = orc.stroke_path(want, cmds, (200, 100, 0, 255), width=3.5, join=1, dashes=(6, 3))
+    assert st["dropped"] == 0 and st2["dropped"] == 0
+    _cov_check(img.download(), want, "big per-call fill + dashed stroke")
+    # a huge coordinate: pixie raises "Path int overflow detected"
+    bad = cmds.copy()
+    bad[1] = 1e30
+    before = img.download()
+    with pytest.raises(px.PixieError, match="overflow") as e:
+        img.fillPath(bad, (255, 0, 0, 255))
+    assert e.value.code == 2
+    _exact(img.download(), before, "image untouched by a failing big call")
