@@ -104,13 +104,13 @@ int pxh_draw_tiled(pxh_image* a, const pxh_image* b, const float* mat9, int32_t 
 pxh_image* pxh_minify_by2(const pxh_image* img, int32_t power);
 pxh_image* pxh_magnify_by2(const pxh_image* img, int32_t power);
 
-/* Non-solid paints (paints.nim:4-29): the PaintKind ordinal, blend mode and
- * opacity; image paints (1, 2) carry the image and imageMat, gradients
- * (3-5) the handle positions (image space: 2 points for linear, 3 for
- * radial and angular) and colour stops (straight float r, g, b, a and the
- * stop position, in order).  Solid paints go through pxh_fill_path /
- * pxh_stroke_path with their colour premultiplied. */
+/* Paints (paints.nim:4-29): the PaintKind ordinal, blend mode and opacity;
+ * a solid paint (0) carries its colour (straight float r, g, b, a), image
+ * paints (1, 2) the image and imageMat, gradients (3-5) the handle positions
+ * (image space: 2 points for linear, 3 for radial and angular) and colour
+ * stops (straight float r, g, b, a and the stop position, in order). */
 enum {
+  PXH_SOLID_PAINT = 0,
   PXH_IMAGE_PAINT = 1,
   PXH_TILED_IMAGE_PAINT = 2,
   PXH_LINEAR_GRADIENT = 3,
@@ -127,11 +127,16 @@ typedef struct {
   const float* stops;      /* nstops x (r, g, b, a, position) */
   const pxh_image* image;  /* ImagePaint / TiledImagePaint, RGBA8 */
   float image_mat[9];      /* imageMat, m[3 * col + row] */
+  float color[4];          /* SolidPaint: color (chroma Color, straight) */
 } pxh_paint;
 #define PXH_MAX_STOPS 32
 
-/* fillPath / strokePath with a non-solid paint (paths.nim:2115-2142,
- * 2176-2214).  Pixie builds a full-size coverage mask, a full-size fill
+/* fillPath / strokePath with any paint (paths.nim:2093-2214).  A solid
+ * paint takes fillPath's own route: opacity clamped to [0, 1], nothing drawn
+ * at opacity 0 or for a transparent colour unless it overwrites, else the
+ * colour with the opacity multiplied into its alpha, converted as chroma's
+ * asRgbx, filled as pxh_fill_path / pxh_stroke_path fill.  For the non-solid
+ * kinds pixie builds a full-size coverage mask, a full-size fill
  * (the image drawn with imageMat, tiled or not, or the gradient), masks one
  * with the other and draws the result; here the mask is an 8-bit plane and
  * one pass masks the fill (a gradient evaluated per pixel, or the drawn
@@ -262,6 +267,97 @@ int pxh_server_times(pxh_server* server, int64_t ticket, double* enqueued, doubl
 double pxh_server_now(pxh_server* server);
 /* Returns a finished request's canvas to the arena and forgets the ticket. */
 int pxh_server_release(pxh_server* server, int64_t ticket);
+
+/* Text (fonts.nim, fontformats/opentype.nim) ---------------------------- */
+/* A typeface: an OpenType font with TrueType (glyf) outlines parsed from
+ * bytes (copied), as parseTtf / parseOtf (fonts.nim:500-505,
+ * opentype.nim:2531-2588): cmap formats 4 and 12, head, hhea, maxp, hmtx,
+ * name, OS/2, loca + glyf, kern, GPOS pair adjustments, post.  Errors are
+ * pixie's PixieError texts ("Unexpected error reading font data, EOF",
+ * "Missing required font table: ...", "Unsupported font ...").  Glyph paths
+ * are cached per rune on first use (opentype.nim:2456-2463). */
+typedef struct pxh_typeface pxh_typeface;
+pxh_typeface* pxh_typeface_parse(const uint8_t* data, int64_t len);
+/* parseOpenTypeCollection (opentype.nim:2591-2616): face `index` of a .ttc;
+ * *count (optional) receives the number of faces. */
+pxh_typeface* pxh_typeface_parse_collection(const uint8_t* data, int64_t len, int32_t index,
+                                            int32_t* count);
+void pxh_typeface_free(pxh_typeface* tf);
+/* Typeface.fallbacks (fonts.nim:12-16): replaces the list.  The typefaces
+ * are borrowed and must outlive tf's use. */
+int pxh_typeface_set_fallbacks(pxh_typeface* tf, pxh_typeface* const* fallbacks, int32_t n);
+/* In font units (fonts.nim:59-115): out[0..8] = scale (unitsPerEm), ascent,
+ * descent, lineGap, capHeight, underlinePosition, underlineThickness,
+ * strikeoutPosition, strikeoutThickness. */
+int pxh_typeface_metrics(const pxh_typeface* tf, float* out9);
+/* fullName (opentype.nim:2523-2529), NUL-terminated into buf (cap bytes);
+ * returns the name's length. */
+int32_t pxh_typeface_name(const pxh_typeface* tf, char* buf, int32_t cap);
+/* hasGlyph (fonts.nim:117-122) of tf itself; fallbacks are not searched. */
+int32_t pxh_typeface_has_glyph(const pxh_typeface* tf, uint32_t rune);
+/* getAdvance / getKerningAdjustment (fonts.nim:163-195), in font units,
+ * through the fallbacks. */
+float pxh_typeface_advance(const pxh_typeface* tf, uint32_t rune);
+float pxh_typeface_kerning(const pxh_typeface* tf, uint32_t left, uint32_t right);
+/* getGlyphPath (fonts.nim:145-161): the rune's outline in font units, y up
+ * flipped to y down, through the fallbacks (empty when none has it).
+ * *cmds is malloc'd; release with pxh_free. */
+int pxh_typeface_glyph_path(pxh_typeface* tf, uint32_t rune, float** cmds, int32_t* ncmds);
+
+/* A Font (fonts.nim:17-26).  line_height < 0 is autoLineHeight; text_case
+ * is the TextCase ordinal (upper / lower / title case convert ASCII and
+ * Latin-1 letters); paints are font.paints, drawn in order by
+ * pxh_fill_text / pxh_stroke_text. */
+typedef struct {
+  pxh_typeface* typeface;
+  float size;
+  float line_height;
+  int32_t text_case;
+  int32_t underline;
+  int32_t strikethrough;
+  int32_t no_kerning_adjustments;
+  int32_t npaints;
+  const pxh_paint* paints;
+} pxh_font;
+/* A Span (fonts.nim:28-30): UTF-8 text (len bytes) and its font. */
+typedef struct {
+  const char* text;
+  int64_t len;
+  const pxh_font* font;
+} pxh_span;
+
+/* typeset (fonts.nim:256-466): the arrangement of the spans' runes (control
+ * runes other than LF dropped) with per-rune positions and selection rects,
+ * lines and spans.  The arrangement keeps the pxh_font pointers, as pixie's
+ * keeps its Font refs: the fonts, their typefaces and paints must outlive
+ * it, and their current values are read when it is filled or stroked.
+ * h_align / v_align are the HorizontalAlignment / VerticalAlignment
+ * ordinals.  NULL on error. */
+typedef struct pxh_arrangement pxh_arrangement;
+pxh_arrangement* pxh_typeset(const pxh_span* spans, int32_t nspans, float bounds_x, float bounds_y,
+                             int32_t h_align, int32_t v_align, int32_t wrap);
+void pxh_arrangement_free(pxh_arrangement* a);
+/* Sizes: what = 0 runes, 1 lines, 2 spans. */
+int32_t pxh_arrangement_count(const pxh_arrangement* a, int32_t what);
+/* Copies out runes (n), positions (2n), selectionRects (4n: x, y, w, h),
+ * lines and spans ((start, stop) pairs); any pointer may be NULL. */
+int pxh_arrangement_read(const pxh_arrangement* a, uint32_t* runes, float* positions,
+                         float* selection_rects, int32_t* lines, int32_t* spans);
+/* layoutBounds (fonts.nim:478-492): out2 = width, height. */
+int pxh_arrangement_layout_bounds(const pxh_arrangement* a, float* out2);
+/* computePaths (fonts.nim:517-564): span `span`'s path (glyphs placed and
+ * scaled, underline / strikethrough rects).  *cmds malloc'd (pxh_free). */
+int pxh_arrangement_span_path(const pxh_arrangement* a, int32_t span, float** cmds,
+                              int32_t* ncmds);
+/* computeBounds (fonts.nim:590-597): out4 = x, y, w, h. */
+int pxh_arrangement_compute_bounds(const pxh_arrangement* a, const float* mat9, float* out4);
+/* fillText / strokeText of an arrangement (fonts.nim:566-588, 599-690):
+ * every span's path filled (NonZero) or stroked with each of its font's
+ * paints in order, through pxh_fill_path_paint / pxh_stroke_path_paint. */
+int pxh_fill_text(pxh_image* img, const pxh_arrangement* a, const float* mat9);
+int pxh_stroke_text(pxh_image* img, const pxh_arrangement* a, const float* mat9, float stroke_width,
+                    int32_t line_cap, int32_t line_join, float miter_limit, const float* dashes,
+                    int32_t ndashes);
 
 /* Image operations ------------------------------------------------------ */
 /* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.

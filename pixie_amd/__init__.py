@@ -18,7 +18,8 @@ from .paths import Path, mat3, translate, scale, f32  # noqa: F401
 
 __all__ = ["Image", "Path", "PixieError", "parse_path", "lib", "blend_pixels",
            "BlendMode", "NonZero", "EvenOdd", "mat3", "translate", "scale", "Paint",
-           "PaintKind", "ColorStop"]
+           "PaintKind", "ColorStop", "Typeface", "Font", "Span", "Arrangement", "typeset",
+           "readFont", "readTypeface", "readTypefaces", "parseOtf", "parseTtf"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpixie_hip.so")
@@ -93,7 +94,8 @@ class Paint:
         m = (ctypes.c_float * 9)(*[float(v) for v in np.asarray(self.imageMat).ravel()])
         p = _PxhPaint(self.kind, self.blendMode, self.opacity, len(self.gradientHandlePositions),
                       (ctypes.c_float * 6)(*hp), len(stops), _fp(stops) if len(stops) else None,
-                      self.image._h if self.image is not None else None, m)
+                      self.image._h if self.image is not None else None, m,
+                      (ctypes.c_float * 4)(*self.color))
         return p, stops
 
 
@@ -102,7 +104,7 @@ class _PxhPaint(ctypes.Structure):
                 ("opacity", ctypes.c_float), ("nhandles", ctypes.c_int32),
                 ("handles", ctypes.c_float * 6), ("nstops", ctypes.c_int32),
                 ("stops", ctypes.POINTER(ctypes.c_float)), ("image", ctypes.c_void_p),
-                ("image_mat", ctypes.c_float * 9)]
+                ("image_mat", ctypes.c_float * 9), ("color", ctypes.c_float * 4)]
 
 
 class PixieError(RuntimeError):
@@ -209,6 +211,8 @@ def lib():
         L.pxh_server_now.restype = ctypes.c_double
         L.pxh_server_now.argtypes = [vp]
         L.pxh_server_release.argtypes = [vp, ctypes.c_int64]
+        from . import fonts as _fonts
+        _fonts._declare(L)
         _lib = L
     return _lib
 
@@ -379,6 +383,20 @@ class Image:
             return
         _check(lib().pxh_stroke_path(self._h, _fp(c), len(c), _fp(m), pack(color), strokeWidth,
                                      lineCap, lineJoin, miterLimit, _fp(d), nd, blendMode))
+
+    def fillText(self, font, text=None, transform=None, bounds=(0.0, 0.0), hAlign=0, vAlign=0):
+        """fillText (fonts.nim:599-620): a Font and text (typeset here) or an
+        Arrangement."""
+        from . import fonts
+        fonts.fill_text(self, fonts._arrangement(font, text, bounds, hAlign, vAlign), transform)
+
+    def strokeText(self, font, text=None, transform=None, strokeWidth=1.0, bounds=(0.0, 0.0),
+                   hAlign=0, vAlign=0, lineCap=ButtCap, lineJoin=MiterJoin, miterLimit=4.0,
+                   dashes=()):
+        """strokeText (fonts.nim:622-666)."""
+        from . import fonts
+        fonts.stroke_text(self, fonts._arrangement(font, text, bounds, hAlign, vAlign), transform,
+                          strokeWidth, lineCap, lineJoin, miterLimit, dashes)
 
     def fillShapes(self, shapes, color, windingRule=NonZero, blendMode=0):
         pts = np.ascontiguousarray(np.concatenate([np.asarray(s, np.float32).reshape(-1, 2)
@@ -602,3 +620,7 @@ def debug_fill_geometry(path, w, h, transform=None, stroke=False, width=1.0, cap
         lib().pxh_free(p)
     _check(rc)
     return np.array(info[:], np.int32), pa.reshape(-1, 4), en.reshape(-1, 4), wd
+
+
+from .fonts import (Typeface, Font, Span, Arrangement, typeset, readFont, readTypeface,  # noqa: E402
+                    readTypefaces, parseOtf, parseTtf)

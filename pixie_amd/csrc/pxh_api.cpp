@@ -1015,6 +1015,9 @@ int lane_scene_errors(int lane) {
 }
 
 }  // namespace
+
+int set_last_error(int code, const std::string& msg) { return fail(code, msg); }
+
 }  // namespace pxh
 
 using namespace pxh;
@@ -1182,18 +1185,7 @@ int pxh_fill_path(pxh_image* img, const float* cmds, int32_t ncmds, const float*
   return fill_shapes_impl(img, sh, rgbx, winding_rule, blend_mode);
 }
 
-// ---- vmath Mat3 in float32, m[3 * col + row] (a[i, j] = m[3 * i + j]) ----
-static void m3_apply(const float* m, float x, float y, float* ox, float* oy) {
-  *ox = m[0] * x + m[3] * y + m[6];
-  *oy = m[1] * x + m[4] * y + m[7];
-}
-static void m3_mul(const float* a, const float* b, float* o) {
-  float r[9];
-  for (int i = 0; i < 3; i++)
-    for (int j = 0; j < 3; j++)
-      r[3 * i + j] = b[3 * i + 0] * a[0 * 3 + j] + b[3 * i + 1] * a[1 * 3 + j] + b[3 * i + 2] * a[2 * 3 + j];
-  memcpy(o, r, sizeof(r));
-}
+// ---- vmath Mat3 in float32 (m3_apply / m3_mul in pxh_common.h) ----
 // cofactor inverse expanded along the first column index
 static void m3_inverse(const float* m, float* o) {
   auto A = [m](int i, int j) { return m[3 * i + j]; };
@@ -1591,8 +1583,7 @@ static int paint_begin(pxh_image* img, const pxh_paint* paint, float* opacity, p
   if (int r = check_image(img)) return r;
   if (img->fmt != PXH_FORMAT_RGBA8) return fail(kErrArg, "paints need an RGBA8 image");
   if (!paint) return fail(kErrArg, "invalid paint");
-  if (paint->kind <= 0 || paint->kind > PXH_ANGULAR_GRADIENT)
-    return fail(kErrArg, "solid paints go through pxh_fill_path / pxh_stroke_path");
+  if (paint->kind <= 0 || paint->kind > PXH_ANGULAR_GRADIENT) return fail(kErrArg, "invalid paint kind");
   if (paint->blend_mode < 0 || paint->blend_mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
   float o = paint->opacity;  // paths.nim:2101 / 2156: clamp(opacity, 0, 1)
   if (!(o > 0.0f)) o = 0.0f;
@@ -1607,9 +1598,36 @@ static int paint_begin(pxh_image* img, const pxh_paint* paint, float* opacity, p
   return 0;
 }
 
+// fillPath / strokePath's solid route (paths.nim:2101-2112, 2156-2174):
+// false when nothing is drawn, else the colour with the clamped opacity in
+// its alpha as asRgbx converts it (each channel clamped, times 255 rounded
+// half away from zero, then premultiplied as (c * a + 127) div 255).
+static bool solid_rgbx(const pxh_paint* p, uint32_t* rgbx) {
+  float o = p->opacity;
+  if (!(o > 0.0f)) o = 0.0f;
+  if (o > 1.0f) o = 1.0f;
+  if (o == 0.0f) return false;
+  if (!(p->color[3] > 0.0f || p->blend_mode == kOverwrite)) return false;
+  const float c[4] = {p->color[0], p->color[1], p->color[2], p->color[3] * o};
+  uint32_t b[4];
+  for (int k = 0; k < 4; k++) {
+    const float v = c[k] < 0.0f ? 0.0f : (c[k] > 1.0f ? 1.0f : c[k]);
+    b[k] = (uint32_t)floor((double)(v * 255.0f) + 0.5);
+  }
+  if (b[3] != 255)
+    for (int k = 0; k < 3; k++) b[k] = (b[k] * b[3] + 127) / 255;
+  *rgbx = pack(b[0], b[1], b[2], b[3]);
+  return true;
+}
+
 int pxh_fill_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, const float* mat9,
                         const pxh_paint* paint, int32_t winding_rule) {
   PXH_INIT();
+  if (paint && paint->kind == PXH_SOLID_PAINT) {
+    uint32_t rgbx;
+    if (!solid_rgbx(paint, &rgbx)) return check_image(img);
+    return pxh_fill_path(img, cmds, ncmds, mat9, rgbx, winding_rule, paint->blend_mode);
+  }
   float opacity;
   pxh_image* mask;
   if (int r = paint_begin(img, paint, &opacity, &mask)) return r;
@@ -1627,6 +1645,12 @@ int pxh_stroke_path_paint(pxh_image* img, const float* cmds, int32_t ncmds, cons
                           int32_t line_join, float miter_limit, const float* dashes,
                           int32_t ndashes) {
   PXH_INIT();
+  if (paint && paint->kind == PXH_SOLID_PAINT) {
+    uint32_t rgbx;
+    if (!solid_rgbx(paint, &rgbx)) return check_image(img);
+    return pxh_stroke_path(img, cmds, ncmds, mat9, rgbx, stroke_width, line_cap, line_join,
+                           miter_limit, dashes, ndashes, paint->blend_mode);
+  }
   float opacity;
   pxh_image* mask;
   if (int r = paint_begin(img, paint, &opacity, &mask)) return r;

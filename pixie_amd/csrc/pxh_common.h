@@ -60,6 +60,21 @@ PXH_HD int32_t nim_f2i32(float x) {
 PXH_HD float nim_minf(float x, float y) { return (x <= y || y != y) ? x : y; }
 PXH_HD float nim_maxf(float x, float y) { return (y <= x || y != y) ? x : y; }
 
+// vmath Mat3 in float32, m[3 * col + row] (a[i, j] = m[3 * i + j]): a point
+// transformed as a[0, j] * x + a[1, j] * y + a[2, j], and the product a * b
+// with each entry summed over k in order.
+PXH_HD void m3_apply(const float* m, float x, float y, float* ox, float* oy) {
+  *ox = m[0] * x + m[3] * y + m[6];
+  *oy = m[1] * x + m[4] * y + m[7];
+}
+PXH_HD void m3_mul(const float* a, const float* b, float* o) {
+  float r[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      r[3 * i + j] = b[3 * i + 0] * a[0 * 3 + j] + b[3 * i + 1] * a[1 * 3 + j] + b[3 * i + 2] * a[2 * 3 + j];
+  for (int i = 0; i < 9; i++) o[i] = r[i];
+}
+
 // 24.8 fixed point, paths.nim:1268-1275.
 PXH_HD int32_t fixed32(float f) { return nim_f2i32(f * 256.0f); }
 PXH_HD int32_t fx_int(int32_t p) { return p / 256; }

@@ -47,6 +47,9 @@ struct FillGeom {
   int32_t maxCount = 0;  // largest band
 };
 
+// Sets the message pxh_last_error() returns and returns `code` (pxh_api.cpp).
+int set_last_error(int code, const std::string& msg);
+
 int32_t parse_path(const char* s, std::vector<float>* out, std::string* err);
 float pixel_scale(const float* mat9);
 bool mat_is_identity(const float* mat9);
