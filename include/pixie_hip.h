@@ -359,6 +359,46 @@ int pxh_stroke_text(pxh_image* img, const pxh_arrangement* a, const float* mat9,
                     int32_t line_cap, int32_t line_join, float miter_limit, const float* dashes,
                     int32_t ndashes);
 
+/* SVG (fileformats/svg.nim) -------------------------------------------- */
+/* parseSvg (svg.nim:502-555): the document's elements (path, line,
+ * polyline, polygon, rect, circle, ellipse inside nested g) as pixie path
+ * commands with their resolved properties, and its linearGradient
+ * resources.  width = height = 0 keeps the view box size, else the content
+ * is scaled to width x height.  Errors are pixie's texts ("Invalid SVG
+ * data", "Unsupported SVG tag: ...", ...).  NULL on error. */
+typedef struct pxh_svg pxh_svg;
+pxh_svg* pxh_svg_parse(const char* data, int64_t len, int32_t width, int32_t height);
+void pxh_svg_free(pxh_svg* svg);
+int pxh_svg_size(const pxh_svg* svg, int32_t* width, int32_t* height);
+int32_t pxh_svg_count(const pxh_svg* svg); /* elements */
+/* An element's properties (SvgProperties, svg.nim:19-33) as resolved. */
+typedef struct {
+  int32_t display;
+  int32_t fill_rule;
+  char fill[64];          /* "none", a colour, or "url(#id)" (truncated) */
+  uint32_t stroke_rgbx;   /* premultiplied; 0 = no stroke */
+  float stroke_width;
+  int32_t line_cap;
+  int32_t line_join;
+  float miter_limit;
+  int32_t ndashes;
+  const float* dashes;    /* valid while the svg lives */
+  float transform[9];
+  float opacity;
+  float fill_opacity;
+  float stroke_opacity;
+} pxh_svg_props;
+/* Element i: its commands (*cmds malloc'd, pxh_free; either may be NULL)
+ * and properties (may be NULL). */
+int pxh_svg_element(const pxh_svg* svg, int32_t i, float** cmds, int32_t* ncmds,
+                    pxh_svg_props* props);
+/* newImage(svg) (svg.nim:557-602) into img (RGBA8, the svg's size): cleared
+ * to transparent, then every displayed element filled and stroked in
+ * order, the first fill overwriting.  Solid fills and strokes are drawn in
+ * one scene pass; gradient fills take the paint route in order between
+ * scene passes. */
+int pxh_svg_render(pxh_svg* svg, pxh_image* img);
+
 /* Image operations ------------------------------------------------------ */
 /* draw(a, b, translate(dx, dy), blend) for integer translations, i.e.
  * blendRect (images.nim:468-529, reached from draw images.nim:636-678). */

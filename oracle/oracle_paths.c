@@ -661,6 +661,13 @@ static rect_t compute_bounds(const segbuf* sg) { /* 1098-1117 */
   return r;
 }
 
+/* No segment (or only NaN ones): computeBounds keeps its infinite initial
+ * extremes, whose int conversions in fillShapes are undefined in C.  The
+ * reference's tests fill such paths and expect nothing drawn and no error
+ * (test_svg.nim: dashes.svg's horizontal lines with the default fill, the
+ * tiger's lone moveto; test_fonts.nim: spans6's one-space span). */
+static int no_bounds(rect_t r) { return r.x == INFINITY && r.w == -INFINITY; }
+
 static rect_t snap_to_pixels(rect_t r) { /* common.nim:92-101 */
   float xMin = r.x, xMax = r.x + r.w, yMin = r.y, yMax = r.y + r.h;
   rect_t o;
@@ -1111,7 +1118,12 @@ static int fill_shapes_impl(uint8_t* img, int W, int H, const shapes_t* sh, orc_
   segbuf sg = {0};
   shapes_to_segments(sh, 1, &sg);
   if (st) st->segments += sg.n;
-  rect_t bounds = snap_to_pixels(compute_bounds(&sg));
+  rect_t raw = compute_bounds(&sg);
+  if (no_bounds(raw)) {
+    free(sg.s);
+    return ORC_OK;
+  }
+  rect_t bounds = snap_to_pixels(raw);
   int64_t startX = nim_f2i(bounds.x);
   if (startX < 0) startX = 0;
   int64_t startY = nim_f2i(bounds.y);
@@ -1751,7 +1763,9 @@ int orc_partitions(const float* xy, const int* ofs, int np, int W, int H, int** 
   import_shapes(&s, xy, ofs, np);
   segbuf sg = {0};
   shapes_to_segments(&s, 1, &sg);
-  rect_t b = snap_to_pixels(compute_bounds(&sg));
+  rect_t raw = compute_bounds(&sg);
+  if (no_bounds(raw)) raw.x = raw.y = raw.w = raw.h = 0.0f;
+  rect_t b = snap_to_pixels(raw);
   int64_t startX = nim_f2i(b.x);
   if (startX < 0) startX = 0;
   int64_t startY = nim_f2i(b.y);

@@ -19,7 +19,8 @@ from .paths import Path, mat3, translate, scale, f32  # noqa: F401
 __all__ = ["Image", "Path", "PixieError", "parse_path", "lib", "blend_pixels",
            "BlendMode", "NonZero", "EvenOdd", "mat3", "translate", "scale", "Paint",
            "PaintKind", "ColorStop", "Typeface", "Font", "Span", "Arrangement", "typeset",
-           "readFont", "readTypeface", "readTypefaces", "parseOtf", "parseTtf"]
+           "readFont", "readTypeface", "readTypefaces", "parseOtf", "parseTtf", "Svg", "parseSvg",
+           "readSvg"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpixie_hip.so")
@@ -211,8 +212,9 @@ def lib():
         L.pxh_server_now.restype = ctypes.c_double
         L.pxh_server_now.argtypes = [vp]
         L.pxh_server_release.argtypes = [vp, ctypes.c_int64]
-        from . import fonts as _fonts
+        from . import fonts as _fonts, svg as _svg
         _fonts._declare(L)
+        _svg._declare(L)
         _lib = L
     return _lib
 
@@ -624,3 +626,4 @@ def debug_fill_geometry(path, w, h, transform=None, stroke=False, width=1.0, cap
 
 from .fonts import (Typeface, Font, Span, Arrangement, typeset, readFont, readTypeface,  # noqa: E402
                     readTypefaces, parseOtf, parseTtf)
+from .svg import Svg, parseSvg, readSvg  # noqa: E402

@@ -222,7 +222,8 @@ PXH_HD void partition_range(float aty, float toy, uint32_t startY, uint32_t band
 }
 
 // Per-fill locals of fillShapes (paths.nim:1602-1619) from the segment
-// bounds.  Returns kErrIntOverflow when pathWidth < 0 (as pixie raises).
+// bounds.  Returns kErrIntOverflow when pathWidth < 0 (as pixie raises for
+// finite bounds beyond the int range).
 struct Bounds {
   float xMin, xMax, yMin, yMax;
 };
@@ -251,6 +252,19 @@ PXH_HD Bounds bounds_merge(Bounds a, const Bounds& b) {
 }
 
 PXH_HD int32_t fill_locals(Bounds bd, bool anyNaN, int32_t W, int32_t H, FillInfo* f) {
+  f->canvasW = W;
+  f->canvasH = H;
+  if (bd.xMin == INFINITY && bd.xMax == -INFINITY) {
+    // No segment (an empty or all-horizontal path): computeBounds keeps its
+    // infinite initial extremes, whose int conversions in fillShapes are
+    // undefined.  The reference's own tests fill such paths and expect
+    // nothing drawn and no error (a horizontal <line> with the default
+    // fill in test_svg's dashes.svg, a lone moveto in its tiger, a span of
+    // one space in test_fonts' spans6): nothing is drawn here either.
+    f->startX = f->startY = f->pathWidth = f->pathHeight = 0;
+    f->xlo = f->xhi = 0;
+    return kOk;
+  }
   // computeBounds 1098-1117 then snapToPixels common.nim:92-101
   float rx = 0, ry = 0, rw = 0, rh = 0;
   if (!(anyNaN || bd.xMin != bd.xMin || bd.xMax != bd.xMax || bd.yMin != bd.yMin ||
@@ -277,8 +291,6 @@ PXH_HD int32_t fill_locals(Bounds bd, bool anyNaN, int32_t W, int32_t H, FillInf
   f->startY = (int32_t)(startY > (int64_t)H ? (int64_t)H : startY);
   f->pathWidth = pathWidth < 0 ? -1 : (int32_t)pathWidth;
   f->pathHeight = (int32_t)(pathHeight < -1 ? -1 : pathHeight);
-  f->canvasW = W;
-  f->canvasH = H;
   int64_t xlo = startX - kWriteMargin, xhi = startX + (pathWidth > 0 ? pathWidth : 0) + kWriteMargin;
   f->xlo = (int32_t)(xlo < 0 ? 0 : xlo);
   f->xhi = (int32_t)(xhi > W ? W : xhi);

@@ -108,10 +108,21 @@ def test_glyph_geometry(orc):
         _compare(orc, p.array(), 256, 256)
 
 
-def test_empty_path_raises_like_pixie(orc):
-    # all-horizontal path: no segments -> bounds are infinite -> pixie raises
-    # "Path int overflow detected" (paths.nim:1618-1619)
-    with pytest.raises(px.PixieError):
-        px.debug_fill_geometry("M 0 0 L 10 0 Z", 100, 100)
-    with pytest.raises(orc.OracleError):
-        orc.fill_path(orc.new_image(100, 100), "M 0 0 L 10 0 Z", (255, 0, 0, 255))
+def test_empty_path_draws_nothing(orc):
+    # an all-horizontal or empty path has no segment; computeBounds keeps its
+    # infinite initial extremes.  The reference's tests fill such paths and
+    # expect no error and nothing drawn (test_svg.nim's dashes.svg and tiger,
+    # test_fonts.nim's spans6), so both the product and the oracle draw
+    # nothing; finite bounds beyond the int range still raise.
+    for path in ("M 0 0 L 10 0 Z", "M 5 5", ""):
+        info, parts, entries, _ = px.debug_fill_geometry(path, 100, 100)
+        assert info[2] == 0 and len(parts) == 0 and len(entries) == 0, (path, info)
+        img = orc.new_image(100, 100)
+        orc.fill(img, (1, 2, 3, 4))
+        for mode in (0, 16, 17):
+            orc.fill_path(img, path, (255, 0, 0, 255), mode=mode)
+        assert (img.reshape(-1, 4) == (1, 2, 3, 4)).all()
+    with pytest.raises(px.PixieError, match="overflow"):
+        px.debug_fill_geometry("M 1e30 0 L -1e30 0 L 0 10 Z", 100, 100)
+    with pytest.raises(orc.OracleError, match="overflow"):
+        orc.fill_path(orc.new_image(100, 100), "M 1e30 0 L -1e30 0 L 0 10 Z", (255, 0, 0, 255))

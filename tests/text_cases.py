@@ -122,6 +122,15 @@ def _cases():
         ("\n1. free (something) from restrictive restrictions \"the regulations are intended to "
          "strallow changes in public policy\" ", U, 20, {"paints": [hexc("#4F4F4F")]})],
         (360, 360), mat=[1, 0, 0, 0, 1, 0, 20, 20, 1])])
+    c["spans6"] = (400, 400, [S([
+        ("verb [with object] ", U, 15, {"paints": [hexc("#CACACA")]}), ("strallow\n", U, 84, {}),
+        ("\nstral·low\n", U, 18, {"paints": [hexc("#007FF4")]}),
+        ("\n1. free (something) from ", U, 20, {"paints": [hexc("#4F4F4F")]}),
+        ("restrictive restrictions", U, 20, {"paints": [hexc("#4F4F4F")], "underline": True}),
+        (" ", U, 20, {"paints": [hexc("#4F4F4F")]}),
+        ("\"the regulations are intended to strallow changes in public policy\" ", U, 20,
+         {"paints": [hexc("#4F4F4F")], "strikethrough": True})],
+        (360, 360), mat=[1, 0, 0, 0, 1, 0, 20, 20, 1])])
     c["spans7"] = (400, 400, [("rect", 40, 170, 320, 60, hexc("#FFD6D6")),
                               ("bounded", [("Fancy text", P, 82, {"lineHeight": 60})],
                                {"bounds": (320, 60)}, 40, 170)])
@@ -155,13 +164,11 @@ CASES = _cases()
 # Masters made by an older pixie whose layout differs from v6's typeset (text
 # 11 px lower; a Roboto / Pacifico kerning that drifts by ~0.04 px a glyph;
 # IBM Plex baselines 1 px lower at 16 px; selection rects as tall as the
-# whole line rather than each span's font) or that v6 cannot draw (a span of
-# one space has an empty path, which fillShapes rejects as "Path int
-# overflow detected"): not pins.  Their text is still rendered by the GPU
-# tests against the oracle.
+# whole line rather than each span's font): not pins.  Their text is still
+# rendered by the GPU tests against the oracle.
 NOT_PINS = {"pairs1", "paragraph1_2", "paragraph1_3", "paragraph1_nokern_2",
             "paragraph1_nokern_3", "huge3", "paragraph3", "paragraph3_2", "paragraph3_3",
-            "paragraph5", "paragraph5_2", "paragraph5_3", "spans6", "selection_rects1",
+            "paragraph5", "paragraph5_2", "paragraph5_3", "selection_rects1",
             "selection_rects2"}
 
 
