@@ -330,6 +330,95 @@ def draw_extra(px, WL, reps=5):
     return out
 
 
+def ingest_extra(px, reps=5):
+    """Text and SVG ingestion (fonts.nim typeset + fillText, svg.nim parseSvg
+    + newImage): a 36,540-glyph page (NotoSans 12 px, 52 lines) and a
+    608-glyph paragraph (Roboto 16 px on 1000x150) filled on the device; the
+    reference's Ghostscript tiger (240 elements, 900x900) and dragon (1,377
+    elements, 1568x940) drawn on the device.  Host times are wall clock;
+    device times are HIP events around the call on the library stream
+    (median of `reps`).  cpu_baseline: the oracle (a C restatement of pixie's
+    scalar path, one core) drawing the same span paths / documents."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    golden = os.path.join(ROOT, "tests", "golden")
+    paras = ("She had come to the conclusion that you could tell a lot about a person by their "
+             "ears. The way they stuck out and the size of the earlobes could give you wonderful "
+             "insights into the person. Of course, she couldn't scientifically prove any of this, "
+             "but that didn't matter to her. ")
+
+    def wall(fn):
+        ts = []
+        for _ in range(reps):
+            px.synchronize()
+            t = time.perf_counter()
+            fn()
+            px.synchronize()
+            ts.append(time.perf_counter() - t)
+        return sorted(ts)[len(ts) // 2] * 1e3
+
+    def dev(fn):
+        ms = []
+        for _ in range(reps):
+            px.event_record(0)
+            fn()
+            px.event_record(1)
+            ms.append(px.event_elapsed_ms(0, 1))
+        return sorted(ms)[len(ms) // 2]
+
+    out = {}
+    for key, font, size, text, (w, h) in (
+            ("text_page", "NotoSans-Regular_4.ttf", 12, paras * 120, (4096, 4096)),
+            ("text_paragraph", "Roboto-Regular_1.ttf", 16, paras * 3, (1000, 150))):
+        f = px.readFont(os.path.join(golden, "fonts", font))
+        f.size = size
+        t = time.perf_counter()
+        a = px.typeset(f, text, bounds=(w - 96, h))
+        ts_ms = (time.perf_counter() - t) * 1e3
+        img = px.Image(w, h)
+        img.fill((255, 255, 255, 255))
+        img.fillText(a)  # glyph paths cached, pools sized
+        fw = wall(lambda: img.fillText(a))
+        rec = {"glyphs": len(a.runes), "canvas": [w, h], "typeset_ms": round(ts_ms, 3),
+               "fill_text_ms": round(fw, 3), "device_ms": round(dev(lambda: img.fillText(a)), 3),
+               "glyphs_per_s": round(len(a.runes) / (fw / 1e3))}
+        try:
+            import oracle as orc
+            (path,) = a.computePaths()
+            o = orc.new_image(w, h)
+            t = time.perf_counter()
+            orc.fill_path(o, path, (0, 0, 0, 255))
+            cms = (time.perf_counter() - t) * 1e3
+            rec["cpu_baseline"] = {"ms": round(cms, 2), "cores": 1, "kind": "port",
+                                   "sample": "the oracle's fillPath of the same span path"}
+        except Exception as e:  # the checker is optional here
+            rec["cpu_baseline"] = {"error": str(e)}
+        out[key] = rec
+    for name in ("Ghostscript_Tiger", "dragon2"):
+        with open(os.path.join(golden, "svg", name + ".svg"), "rb") as fh:
+            doc = fh.read()
+        t = time.perf_counter()
+        s = px.parseSvg(doc)
+        parse_ms = (time.perf_counter() - t) * 1e3
+        img = px.svg.newImage(s)
+        render = lambda: px._check(px.lib().pxh_svg_render(s._h, img._h))  # noqa: E731
+        rw = wall(render)
+        rec = {"elements": len(s), "canvas": [s.width, s.height], "parse_ms": round(parse_ms, 3),
+               "render_ms": round(rw, 3), "device_ms": round(dev(render), 3)}
+        try:
+            import oracle as orc
+            from oracle import svg as osvg
+            od = osvg.parse_svg(orc, doc)
+            t = time.perf_counter()
+            osvg.render(orc, od)
+            rec["cpu_baseline"] = {"ms": round((time.perf_counter() - t) * 1e3, 2), "cores": 1,
+                                   "kind": "port", "sample": "the oracle's newImage(svg)"}
+        except Exception as e:
+            rec["cpu_baseline"] = {"error": str(e)}
+        out["svg_" + name] = rec
+    return out
+
+
 def calibrate(px, reps=3):
     """A plain device copy of a 64 MiB image, for scaling rocprofv3's
     FETCH_SIZE / WRITE_SIZE counters to bytes."""
@@ -376,7 +465,8 @@ def main():
     ap.add_argument("--no-glyphs", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="only copy a 64 MiB image (counter calibration under rocprofv3)")
-    ap.add_argument("--only", default=None, choices=["image_ops", "draw", "glyphs", "serving"],
+    ap.add_argument("--only", default=None,
+                    choices=["image_ops", "draw", "glyphs", "serving", "ingest"],
                     help="run one extra section and print it (profiling)")
     ap.add_argument("--shape", type=int, default=None,
                     help="fixed phase-A launch shape instead of the autotuner's choice (profiling)")
@@ -400,7 +490,7 @@ def main():
         return
     if args.only:
         fn = {"image_ops": image_ops_extra, "draw": draw_extra, "glyphs": glyph_extra,
-              "serving": serve_extra}[args.only]
+              "serving": serve_extra, "ingest": lambda px, WL: ingest_extra(px)}[args.only]
         print(json.dumps({args.only: fn(px, WL)}), flush=True)
         return
     t = time.perf_counter()
@@ -480,6 +570,7 @@ def main():
             extra["glyph_atlas"] = glyph_extra(px, WL)
         extra["image_ops"] = image_ops_extra(px, WL)
         extra["draw"] = draw_extra(px, WL)
+        extra["ingest"] = ingest_extra(px)
         extra["serving"] = serve_extra(px, WL)
         if args.form == "batched":
             pimg = px.Image.from_array(canvas)

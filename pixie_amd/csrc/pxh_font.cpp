@@ -872,19 +872,6 @@ pxh_typeface* tf_fallback(pxh_typeface* t, uint32_t rune, int depth = 0) {  // f
   return nullptr;
 }
 
-std::vector<float> tf_glyph_path(pxh_typeface* t, uint32_t rune) {  // fonts.nim:133-150
-  std::vector<float> p;
-  pxh_typeface* t2 = tf_fallback(t, rune);
-  if (!t2) return p;
-  p = ot_glyph_path(t2, rune);
-  const float ratio = tf_scale(t) / tf_scale(t2);
-  if (ratio != 1.0f) {
-    const float m[9] = {ratio, 0, 0, 0, ratio, 0, 0, 0, 1};
-    path_transform(&p, m);
-  }
-  return p;
-}
-
 float tf_advance(pxh_typeface* t, uint32_t rune) {  // fonts.nim:152-165
   pxh_typeface* t2 = tf_fallback(t, rune);
   if (!t2) t2 = t;  // tofu advance
@@ -1191,6 +1178,57 @@ void path_rect(std::vector<float>* p, float x, float y, float w, float h, bool c
 
 // computePaths (fonts.nim:517-564) for one span.  `line` carries the line
 // counter across spans as the reference's loop does.
+// Appends src transformed by m (Path.transform, paths.nim:264-333, applied
+// to a copy) to out; an identity m copies the commands unchanged.
+void append_transformed(std::vector<float>* out, const std::vector<float>& src, const float* m) {
+  const size_t base = out->size();
+  out->insert(out->end(), src.begin(), src.end());
+  if (is_identity(m)) return;
+  float* c = out->data() + base;
+  const size_t n = src.size();
+  if (n && c[0] == 10.0f) c[0] = 1.0f;  // a leading RMove becomes Move
+  const float a = m[0], b = m[1], d = m[3], e = m[4], tx = m[6], ty = m[7];
+  size_t i = 0;
+  while (i < n) {
+    const int kind = (int)c[i++];
+    int pts;
+    switch (kind) {
+      case 0: pts = 0; break;
+      case 1: case 2: case 10: case 11: case 8: case 17: pts = 1; break;
+      case 6: case 15: case 7: case 16: pts = 2; break;
+      case 5: case 14: pts = 3; break;
+      default:  // glyph paths hold moves, lines, curves and closes only
+        throw FontError("glyph path holds an unexpected command");
+    }
+    // m * (x, y) as m3_apply computes it; relative commands drop the
+    // translation
+    const float ox = kind >= 10 ? 0.0f : tx, oy = kind >= 10 ? 0.0f : ty;
+    for (int k = 0; k < pts; k++, i += 2) {
+      const float x = c[i], y = c[i + 1];
+      c[i] = a * x + d * y + ox;
+      c[i + 1] = b * x + e * y + oy;
+    }
+  }
+}
+
+// The glyph's path in t's units (getGlyphPath, fonts.nim:133-150): the
+// cached outline of the typeface that has it, scaled by the units ratio when
+// that is a fallback (into *tmp).  Empty when no typeface has the rune.
+const std::vector<float>& glyph_ref(pxh_typeface* t, uint32_t rune, std::vector<float>* tmp) {
+  static const std::vector<float> none;
+  pxh_typeface* t2 = tf_fallback(t, rune);
+  if (!t2) return none;
+  const std::vector<float>& p = ot_glyph_path(t2, rune);
+  const float ratio = tf_scale(t) / tf_scale(t2);
+  if (ratio == 1.0f) return p;
+  tmp->assign(p.begin(), p.end());
+  const float m[9] = {ratio, 0, 0, 0, ratio, 0, 0, 0, 1};
+  path_transform(tmp, m);
+  return *tmp;
+}
+
+// computePaths (fonts.nim:517-564) for one span.  `line` carries the line
+// counter across spans as the reference's loop does.
 void span_path(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<float>* out) {
   const pxh_font* font = a->fonts[si];
   pxh_typeface* t = font->typeface;
@@ -1198,14 +1236,18 @@ void span_path(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<f
   const float ut = (float)t->underlineThickness * scale, up = (float)t->underlinePosition * scale;
   const float st = (float)t->strikeoutSize * scale, sp = (float)t->strikeoutPosition * scale;
   const int32_t L = (int32_t)a->lines.size();
+  const float S[9] = {scale, 0, 0, 0, scale, 0, 0, 0, 1};
+  std::vector<float> tmp;
+  size_t need = out->size();
+  for (int32_t ri = a->spans[si].first; ri <= a->spans[si].second; ri++)
+    need += glyph_ref(t, a->runes[ri], &tmp).size() + ((font->underline ? 13 : 0) + (font->strikethrough ? 13 : 0));
+  out->reserve(need);
   for (int32_t ri = a->spans[si].first; ri <= a->spans[si].second; ri++) {
     const float px = a->pos[2 * ri], py = a->pos[2 * ri + 1];
-    std::vector<float> path = tf_glyph_path(t, a->runes[ri]);
     const float T[9] = {1, 0, 0, 0, 1, 0, px, py, 1};
-    const float S[9] = {scale, 0, 0, 0, scale, 0, 0, 0, 1};
     float M[9];
     m3_mul(T, S, M);
-    path_transform(&path, M);
+    append_transformed(out, glyph_ref(t, a->runes[ri], &tmp), M);
     bool decorate = true;
     if (*line < L && ri == a->lines[*line].second) {
       (*line)++;
@@ -1213,10 +1255,9 @@ void span_path(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<f
     }
     if (decorate) {
       const float* r = a->rects.data() + 4 * ri;
-      if (font->underline) path_rect(&path, r[0], py - up + ut / 2.0f, r[2], ut, true);
-      if (font->strikethrough) path_rect(&path, r[0], py - sp, r[2], st, true);
+      if (font->underline) path_rect(out, r[0], py - up + ut / 2.0f, r[2], ut, true);
+      if (font->strikethrough) path_rect(out, r[0], py - sp, r[2], st, true);
     }
-    out->insert(out->end(), path.begin(), path.end());
   }
 }
 
@@ -1354,7 +1395,8 @@ float pxh_typeface_kerning(const pxh_typeface* tf, uint32_t left, uint32_t right
 int pxh_typeface_glyph_path(pxh_typeface* tf, uint32_t rune, float** cmds, int32_t* ncmds) {
   if (!tf) return set_last_error(kErrArg, "invalid typeface");
   try {
-    return export_cmds(tf_glyph_path(tf, rune), cmds, ncmds);
+    std::vector<float> tmp;
+    return export_cmds(glyph_ref(tf, rune, &tmp), cmds, ncmds);
   } catch (const FontError& e) {
     return set_last_error(kErrParse, e.what());
   }
