@@ -750,6 +750,11 @@ int scene_alloc(pxh_scene* sc) {
   size_t oCC = carve(nC * 4), oCO = carve(nC * 4), oPP = carve(nP * 4), oPN = carve(nP * 4);
   size_t oPts = carve((size_t)sc->ptCap * 8), oPtP = carve((size_t)sc->ptCap * 4);
   size_t oPtS = carve((size_t)sc->ptCap * 4);
+  const size_t nChunk = (size_t)sc->ptCap / kBigChunk + (size_t)sc->nBig + 1;
+  size_t oBCS = carve(((size_t)sc->nBig + 1) * 4), oBGS = carve(((size_t)sc->nBig + 1) * 4);
+  size_t oBPS = carve(((size_t)sc->nBig + 1) * 4);
+  size_t oBPC = carve((nP / kBigPolyChunk + (size_t)sc->nBig + 1) * 16);
+  size_t oBCC = carve(nChunk * 8), oBCB = carve(nChunk * 16);
   size_t oSeg = carve((size_t)sc->segCap * 16), oSegW = carve((size_t)sc->segCap);
   size_t oSegB = carve((size_t)sc->segCap * 8), oSegC = carve(((size_t)sc->segCap / 64 + 1) * 8);
   size_t oParts = carve((size_t)sc->bandCap * sizeof(PartInfo)), oBI = carve((size_t)sc->bandCap * 4);
@@ -790,6 +795,12 @@ int scene_alloc(pxh_scene* sc) {
   p.ptPoly = (int32_t*)(b + oPtP);
   p.ptSeg = (int32_t*)(b + oPtS);
   p.ptCap = sc->ptCap;
+  p.bigChunkStart = (int32_t*)(b + oBCS);
+  p.bigGroupStart = (int32_t*)(b + oBGS);
+  p.bigPolyStart = (int32_t*)(b + oBPS);
+  p.bigPolyChunk = (int4*)(b + oBPC);
+  p.bigChunkCnt = (int2*)(b + oBCC);
+  p.bigChunkBd = (float4*)(b + oBCB);
   p.segs = (float4*)(b + oSeg);
   p.segW = (int8_t*)(b + oSegW);
   p.segBand = (int2*)(b + oSegB);

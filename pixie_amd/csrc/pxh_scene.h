@@ -92,6 +92,8 @@ inline int64_t scene_run_bound(int64_t used, int64_t waves) {
 
 // segBand flag: the segment needs anti-aliasing (requiresAntiAliasing).
 constexpr int32_t kSegAA = 1 << 30;
+constexpr int32_t kBigChunk = 4096;  // points per chunk of a big fill's segment stage
+constexpr int32_t kBigPolyChunk = 1024;  // subpaths per chunk of a big call's point layout
 
 enum SceneCounter {
   kCntPoints = 0, kCntSegs, kCntBands, kCntEntries, kCntTasks,
@@ -158,6 +160,16 @@ struct ScenePools {
   int32_t* ptPoly;
   int32_t* ptSeg;            // segments each point emits (geometry scratch)
   int32_t ptCap;
+  // segments of the big fill calls, in chunks of kBigChunk points spread
+  // over the device: chunk prefix per big call (nBig + 1), per chunk
+  // (segment count, segment offset in the call) and bounds, and the prefix
+  // of their 64-segment groups (nBig + 1)
+  int32_t* bigChunkStart;
+  int32_t* bigPolyStart;     // subpath-chunk prefix of the big calls (nBig + 1)
+  int4* bigPolyChunk;        // per subpath chunk: points, error, point offset
+  int2* bigChunkCnt;
+  float4* bigChunkBd;
+  int32_t* bigGroupStart;
   float4* segs;
   int8_t* segW;
   int2* segBand;             // per segment: first band, last band | kSegAA

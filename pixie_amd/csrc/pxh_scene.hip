@@ -14,6 +14,7 @@ namespace pxh {
 constexpr int kBlock = 256;
 constexpr int kBigGeomBlock = 1024;  // geometry workgroup of a call of more than kBlock commands
 constexpr int kStagePts = 4096;  // LDS points of the geometry stage (32 KB)
+constexpr int kBigSegGrid = 1024;  // workgroups of the big-fill segment kernels (grid-stride)
 
 // A value that is the same in every lane, moved to a scalar register.
 __device__ __forceinline__ int32_t uni(int32_t v) { return __builtin_amdgcn_readfirstlane(v); }
@@ -504,13 +505,393 @@ __global__ void __launch_bounds__(256) scene_big_cmd_kernel(ScenePools P, int em
   cmd_emit(P.cmds, i, I.errSq, se);
 }
 
-// Geometry kernel stages by MODE: kGeomAll runs every stage for a call;
-// calls of more than kBlock commands run in four launches so that their
-// commands spread over the CUs: scene_big_cmd_kernel counts every command's
-// points, kGeomFront takes those counts through stage 3 (offsets, the
-// call's point range, start and closing points), scene_big_cmd_kernel emits
-// the points, and kGeomBack runs stages 5-7.
-constexpr int kGeomAll = 0, kGeomFront = 1, kGeomBack = 2;
+// ---- the segment stage of big fills, spread over the device ----
+// Stage 5 of a fill makes a segment from each point and its predecessor
+// (the first point of a subpath pairs with its last), skipping horizontal
+// ones; segments keep point order.  For a big fill the points are cut into
+// chunks of kBigChunk (256 threads x 16 consecutive points): the chunks
+// count their segments and bounds, one workgroup per call scans the counts,
+// merges the bounds and lays out the call's bands (stage 6), the chunks
+// write their segments at the scanned offsets, and one wave per 64-segment
+// group finds the band ranges (stage 7).  The result is the one-workgroup
+// back half's, segment for segment.
+
+__device__ __forceinline__ bool fill_point_seg(const ScenePools& P, const SceneItem& I, const float2* pts,
+                                               const int32_t* ptPoly, int j, Seg* s) {
+  const int q = I.polyBase + ptPoly[j];
+  const int pb = P.polyPt[q], np = P.polyN[q];
+  const float2 pa = j == pb ? pts[pb + np - 1] : pts[j - 1];
+  const float2 pc = pts[j];
+  const V2 v1 = v2(pa.x, quantize_y(pa.y)), v2_ = v2(pc.x, quantize_y(pc.y));
+  return !(j == pb && veq(v1, v2_)) && make_segment(v1, v2_, s);
+}
+
+// The big call whose range of `start` (an exclusive prefix, nBig + 1
+// entries) holds index c.
+__device__ __forceinline__ int big_call_of(const int32_t* start, int nBig, int c) {
+  int lo = 0, hi = nBig;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) / 2;
+    if (start[mid] <= c) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Block-wide bounds of 256 threads (valid in every thread after the call).
+__device__ __forceinline__ Bounds block_bounds256(Bounds bd, Bounds* sBd) {
+  for (int o = 32; o > 0; o >>= 1) {
+    Bounds ob;
+    ob.xMin = __shfl_xor(bd.xMin, o, 64);
+    ob.xMax = __shfl_xor(bd.xMax, o, 64);
+    ob.yMin = __shfl_xor(bd.yMin, o, 64);
+    ob.yMax = __shfl_xor(bd.yMax, o, 64);
+    bd = bounds_merge(bd, ob);
+  }
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sBd[threadIdx.x >> 6] = bd;
+  __syncthreads();
+  Bounds b = sBd[0];
+  for (int w = 1; w < 4; w++) b = bounds_merge(b, sBd[w]);
+  return b;
+}
+
+// Prefixes over the big calls: subpath chunks of every call (what = 2),
+// point chunks of the fills whose points were laid out (what = 0), and
+// 64-segment groups of the fills that draw (what = 1).
+__global__ void __launch_bounds__(kScanBlock) scene_big_plan_kernel(ScenePools P, int what) {
+  __shared__ int sh[kScanBlock / 64];
+  int32_t* out = what == 2 ? P.bigPolyStart : what == 1 ? P.bigGroupStart : P.bigChunkStart;
+  int carry = 0;
+  for (int i0 = 0; i0 < P.nBig; i0 += kScanBlock) {
+    const int i = i0 + threadIdx.x;
+    int n = 0;
+    if (i < P.nBig) {
+      const int it = P.itemOrder[i];
+      if (what == 2) {
+        n = (P.items[it].polyCount + kBigPolyChunk - 1) / kBigPolyChunk;
+      } else {
+        const ItemGeom g = P.geo[it];
+        if (!P.items[it].stroke && g.status == kOk) {
+          if (what == 0) n = (g.ptCount + kBigChunk - 1) / kBigChunk;
+          else if (P.fills[it].pathWidth > 0) n = (g.segCount + 63) / 64;
+        }
+      }
+    }
+    int t;
+    const int off = block_excl_scan_n<kScanBlock>(n, &t, sh);
+    if (i < P.nBig) out[i] = carry + off;
+    carry += t;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[P.nBig] = carry;
+}
+
+// ---- the point layout of big calls (geometry stages 2-3), spread over
+// the device: each chunk of kBigPolyChunk subpaths (256 threads x 4)
+// counts its subpaths' points and finds count errors, one workgroup per call
+// scans the chunk totals and allocates the call's points, and the chunks
+// place their subpaths (offsets, per-command offsets, start and closing
+// points).  The layout is the one-workgroup front half's, point for point.
+
+// Points of subpath q (stage 2): start + pieces, plus the implicit closing
+// point of a fill; *err gets a command's error code.
+__device__ __forceinline__ int big_poly_points(const ScenePools& P, const SceneItem& I, int q, int* err) {
+  const ScenePoly sp = P.polys[q];
+  int n = 0;
+  for (int c = 0; c < sp.numCmds; c++) {
+    const int k = P.cmdCount[sp.firstCmd + c];
+    if (k < 0) *err = max(*err, -k);
+    else n += k;
+  }
+  if (n <= 0) return 0;
+  int np = n + 1;
+  const SceneCmd& last = P.cmds[sp.firstCmd + sp.numCmds - 1];
+  if (!I.stroke && (int)last.kind != 0 && !sp.noClose) {
+    V2 d = cmd_end(last) - v2(last.sx, last.sy);
+    if (d.x != 0.0f || d.y != 0.0f) np++;
+  }
+  return np;
+}
+
+// emit = 0: points and errors per chunk; emit = 1: subpath placement.
+__global__ void __launch_bounds__(256) scene_big_front_kernel(ScenePools P, int emit) {
+  __shared__ int sh[4];
+  const int nChunks = P.bigPolyStart[P.nBig];
+  for (int c = blockIdx.x; c < nChunks; c += gridDim.x) {
+    const int bi = big_call_of(P.bigPolyStart, P.nBig, c);
+    const int it = P.itemOrder[bi];
+    const SceneItem& I = P.items[it];
+    const int ql0 = (c - P.bigPolyStart[bi]) * kBigPolyChunk + threadIdx.x * 4;
+    if (!emit) {
+      int sum = 0, err = kOk;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int ql = ql0 + u;
+        if (ql < I.polyCount) {
+          const int np = big_poly_points(P, I, I.polyBase + ql, &err);
+          P.polyN[I.polyBase + ql] = np;
+          sum += np;
+        }
+      }
+      int t;
+      block_excl_scan_n<256>(sum, &t, sh);
+      for (int o = 32; o > 0; o >>= 1) err = max(err, __shfl_xor(err, o, 64));
+      __syncthreads();
+      if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = err;
+      __syncthreads();
+      if (threadIdx.x == 0) P.bigPolyChunk[c] = make_int4(t, max(max(sh[0], sh[1]), max(sh[2], sh[3])), 0, 0);
+      __syncthreads();
+      continue;
+    }
+    const ItemGeom g = P.geo[it];
+    if (g.status != kOk) continue;  // the same for the whole block
+    int np[4], sum = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int ql = ql0 + u;
+      np[u] = ql < I.polyCount ? P.polyN[I.polyBase + ql] : 0;
+      sum += np[u];
+    }
+    int t;
+    int pb = P.bigPolyChunk[c].z + block_excl_scan_n<256>(sum, &t, sh);
+    const bool xf = !I.stroke && !I.identity;
+    float2* pts = P.pts + g.ptBase;
+    int32_t* ptPoly = P.ptPoly + g.ptBase;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int ql = ql0 + u;
+      if (ql < I.polyCount) {  // stage 3
+        const int q = I.polyBase + ql;
+        P.polyPt[q] = pb;
+        const ScenePoly sp = P.polys[q];
+        int off = pb + 1;
+        for (int k = 0; k < sp.numCmds; k++) {
+          P.cmdOff[sp.firstCmd + k] = off;
+          off += P.cmdCount[sp.firstCmd + k];
+        }
+        if (np[u] > 0) {
+          const SceneCmd& mv = P.cmds[sp.firstCmd];
+          V2 s = v2(mv.sx, mv.sy);
+          V2 w = xf ? mat_apply(I.mat, s) : s;
+          pts[pb] = make_float2(w.x, w.y);
+          ptPoly[pb] = ql;
+          if (off < pb + np[u]) {  // implicit closing point of a fill
+            pts[off] = make_float2(w.x, w.y);
+            ptPoly[off] = ql;
+          }
+        }
+        pb += np[u];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per big call: its point total and errors from the chunks,
+// the chunks' point offsets, and the call's point allocation.
+__global__ void __launch_bounds__(256) scene_big_front_alloc_kernel(ScenePools P) {
+  __shared__ int sh[4];
+  const int bi = blockIdx.x;
+  const int it = P.itemOrder[bi];
+  const SceneItem& I = P.items[it];
+  const int c0 = P.bigPolyStart[bi], nc = P.bigPolyStart[bi + 1] - c0;
+  int carry = 0, err = kOk;
+  for (int k0 = 0; k0 < nc; k0 += 256) {
+    const int k = k0 + threadIdx.x;
+    int n = 0;
+    if (k < nc) {
+      const int4 ch = P.bigPolyChunk[c0 + k];
+      n = ch.x;
+      err = max(err, ch.y);
+    }
+    int t;
+    const int off = block_excl_scan_n<256>(n, &t, sh);
+    if (k < nc) P.bigPolyChunk[c0 + k].z = carry + off;
+    carry += t;
+    __syncthreads();
+  }
+  for (int o = 32; o > 0; o >>= 1) err = max(err, __shfl_xor(err, o, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = err;
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  err = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
+  const int total = carry;
+  int base = 0;
+  if (err == kOk) {
+    base = atomicAdd(&P.counters[kCntPoints], total);
+    if (base + total > P.ptCap) {
+      atomicOr(&P.counters[kCntError], 1);
+      err = kErrCapacity;
+    }
+    // strokeShapes checks the dash values once it meets a shape with
+    // points (paths.nim:2029-2031)
+    if (I.stroke && !I.strokeNone && I.dashBad && total > 0 && err == kOk) err = kErrDash;
+  }
+  ItemGeom g = {};
+  if (err != kOk) {
+    g.status = err;
+    P.fills[it].status = err;
+    P.fills[it].pathWidth = 0;
+  } else {
+    g.ptBase = base;
+    g.ptCount = total;
+    g.status = kOk;
+  }
+  P.geo[it] = g;
+}
+
+// Segments and bounds of each chunk (emit = 0), or its segments written at
+// the call's segment base plus the chunk's scanned offset (emit = 1).
+__global__ void __launch_bounds__(256) scene_big_seg_kernel(ScenePools P, int emit) {
+  __shared__ int sh[4];
+  __shared__ Bounds sBd[4];
+  const int nChunks = P.bigChunkStart[P.nBig];
+  for (int c = blockIdx.x; c < nChunks; c += gridDim.x) {
+    const int bi = big_call_of(P.bigChunkStart, P.nBig, c);
+    const int it = P.itemOrder[bi];
+    const SceneItem& I = P.items[it];
+    const ItemGeom g = P.geo[it];
+    if (emit && g.status != kOk) continue;  // the same for the whole block
+    const float2* pts = P.pts + g.ptBase;
+    const int32_t* ptPoly = P.ptPoly + g.ptBase;
+    const int j0 = (c - P.bigChunkStart[bi]) * kBigChunk + threadIdx.x * 16;
+    float4 sg[16];
+    int8_t sw[16];
+    uint32_t valid = 0;
+    Bounds bd = bounds_init();
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int j = j0 + u;
+      Seg s;
+      if (j < g.ptCount && fill_point_seg(P, I, pts, ptPoly, j, &s)) {
+        sg[u] = make_float4(s.at.x, s.at.y, s.to.x, s.to.y);
+        sw[u] = (int8_t)s.w;
+        bounds_add(&bd, s);
+        valid |= 1u << u;
+      }
+    }
+    const int n = __popc(valid);
+    int t;
+    const int off = block_excl_scan_n<256>(n, &t, sh);
+    if (!emit) {
+      const Bounds b = block_bounds256(bd, sBd);
+      if (threadIdx.x == 0) {
+        P.bigChunkCnt[c] = make_int2(t, 0);
+        P.bigChunkBd[c] = make_float4(b.xMin, b.xMax, b.yMin, b.yMax);
+      }
+    } else {
+      const int w = g.segBase + P.bigChunkCnt[c].y + off;
+      int k = 0;
+#pragma unroll
+      for (int u = 0; u < 16; u++)
+        if ((valid >> u) & 1u) {
+          P.segs[w + k] = sg[u];
+          P.segW[w + k] = sw[u];
+          k++;
+        }
+    }
+    __syncthreads();
+  }
+}
+
+// One workgroup per big call: scans its chunk counts, merges their bounds,
+// allocates the segments and lays out the fill (stage 6).
+__global__ void __launch_bounds__(256) scene_big_seg_alloc_kernel(ScenePools P) {
+  __shared__ int sh[4];
+  __shared__ Bounds sBd[4];
+  const int bi = blockIdx.x;
+  const int it = P.itemOrder[bi];
+  const SceneItem& I = P.items[it];
+  if (I.stroke) return;
+  ItemGeom g = P.geo[it];
+  if (g.status != kOk) return;  // the front half wrote the error
+  const int c0 = P.bigChunkStart[bi], nc = P.bigChunkStart[bi + 1] - c0;
+  int carry = 0;
+  Bounds bd = bounds_init();
+  for (int k0 = 0; k0 < nc; k0 += 256) {
+    const int k = k0 + threadIdx.x;
+    int n = 0;
+    if (k < nc) {
+      n = P.bigChunkCnt[c0 + k].x;
+      const float4 b = P.bigChunkBd[c0 + k];
+      Bounds cb;
+      cb.xMin = b.x;
+      cb.xMax = b.y;
+      cb.yMin = b.z;
+      cb.yMax = b.w;
+      bd = bounds_merge(bd, cb);
+    }
+    int t;
+    const int off = block_excl_scan_n<256>(n, &t, sh);
+    if (k < nc) P.bigChunkCnt[c0 + k].y = carry + off;
+    carry += t;
+    __syncthreads();
+  }
+  const Bounds b = block_bounds256(bd, sBd);
+  if (threadIdx.x != 0) return;
+  const int nseg = carry;
+  // segments start on a 64-segment chunk of their own
+  const int base = atomicAdd(&P.counters[kCntSegs], (nseg + 63) & ~63);
+  FillInfo f = {};
+  f.rgbx = I.rgbx;
+  f.mode = I.mode;
+  f.rule = I.rule;
+  if (base + nseg > P.segCap) {
+    atomicOr(&P.counters[kCntError], 1);
+    f.status = kErrCapacity;
+    f.pathWidth = 0;
+    g.segBase = 0;
+    g.segCount = 0;
+    g.status = kErrCapacity;
+  } else {
+    const int32_t r = fill_locals(b, false, P.W, P.H, &f);
+    f.status = r;
+    if (r == kOk && f.pathWidth > 0) band_layout(&f, nseg);
+    g.segBase = base;
+    g.segCount = nseg;
+    g.status = r;
+  }
+  P.fills[it] = f;
+  P.geo[it] = g;
+}
+
+// Stage 7 of the big fills: one wave per 64-segment group.
+__global__ void __launch_bounds__(256) scene_big_band_kernel(ScenePools P) {
+  const int nGroups = P.bigGroupStart[P.nBig];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int gi = blockIdx.x * 4 + wave; gi < nGroups; gi += gridDim.x * 4) {
+    const int bi = big_call_of(P.bigGroupStart, P.nBig, gi);
+    const int it = P.itemOrder[bi];
+    const ItemGeom g = P.geo[it];
+    const FillInfo fl = P.fills[it];
+    const int j = (gi - P.bigGroupStart[bi]) * 64 + lane;
+    int lo = 0x7fffffff, hi = -1;
+    if (j < g.segCount) {
+      const float4 sg = P.segs[g.segBase + j];
+      uint32_t l = 0, h = 0;
+      if (fl.nParts > 1)
+        partition_range(sg.y, sg.w, (uint32_t)fl.startY, (uint32_t)fl.bandH, (uint32_t)fl.nParts, &l, &h);
+      const bool aa = seg_requires_aa(v2(sg.x, sg.y), v2(sg.z, sg.w));
+      P.segBand[g.segBase + j] = make_int2((int)l, (int)h | (aa ? kSegAA : 0));
+      lo = (int)l;
+      hi = (int)h;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if (lane == 0) P.segChunk[(g.segBase + j) >> 6] = make_int2(lo, hi);
+  }
+}
+
+// Geometry kernel stages by MODE: kGeomAll runs every stage for a call.
+// Calls of more than kBlock commands spread their stages over the CUs:
+// scene_big_cmd_kernel counts every command's points, scene_big_front_*
+// lay the points out (stages 2-3), scene_big_cmd_kernel emits them, and the
+// segment stages run in scene_big_seg_* / scene_big_band_kernel for fills
+// and in one kGeomBack workgroup per call for strokes (stages 5-7).
+constexpr int kGeomAll = 0, kGeomBack = 2;
 
 template <int NB, int MODE>
 __global__ void __launch_bounds__(NB) scene_geom_kernel(ScenePools P, int orderBase) {
@@ -532,32 +913,16 @@ __global__ void __launch_bounds__(NB) scene_geom_kernel(ScenePools P, int orderB
   if (MODE == kGeomBack) {
     const ItemGeom gf = P.geo[it];
     if (gf.status != kOk) return;  // the front wrote the error
+    if (!I.stroke) return;         // big fills: scene_big_seg_* kernels
     total = gf.ptCount;
     ptBase = gf.ptBase;
     goto segments;
-  }
-  if (MODE == kGeomFront) {
-    // the counts of scene_big_cmd_kernel: a negative count is an error code
-    for (int c0 = 0; c0 < I.cmdCount; c0 += NB) {
-      const int i = I.cmdBase + c0 + tid;
-      if (c0 + tid < I.cmdCount && P.cmdCount[i] < 0) atomicMax(&sErr, -P.cmdCount[i]);
-    }
-    __syncthreads();
-    if (sErr != kOk) {
-      if (tid == 0) {
-        P.geo[it].status = sErr;
-        P.geo[it].segCount = 0;
-        P.fills[it].status = sErr;
-        P.fills[it].pathWidth = 0;
-      }
-      return;
-    }
   }
   {
   // 1. points per command.  A call of at most NB commands keeps each
   // command's points in an LDS slice of its own (thread tid <-> command tid
   // in stages 1 and 4), so stage 4 copies them unless a slice overflowed.
-  const bool stage = MODE == kGeomAll && I.cmdCount <= NB;
+  const bool stage = I.cmdCount <= NB;
   const int slice = stage && I.cmdCount > 0 ? kStagePts / I.cmdCount : 0;
   bool staged = false;
   int myN = 0;
@@ -698,17 +1063,6 @@ __global__ void __launch_bounds__(NB) scene_geom_kernel(ScenePools P, int orderB
     }
   }
   __syncthreads();
-  if (MODE == kGeomFront) {
-    // stage 4 runs in scene_big_cmd_kernel; the back half reads these
-    if (tid == 0) {
-      ItemGeom gf = {};
-      gf.ptBase = ptBase;
-      gf.ptCount = total;
-      gf.status = kOk;
-      P.geo[it] = gf;
-    }
-    return;
-  }
   // 4. emit points (thread per command)
   for (int c0 = 0; c0 < I.cmdCount; c0 += NB) {
     int i = I.cmdBase + c0 + tid;
@@ -1841,9 +2195,19 @@ int scene_launch_geometry(const ScenePools& p, hipStream_t s) {
   if (p.nBig > 0) {
     const int grid = (p.bigCmds + 255) / 256;
     hipLaunchKernelGGL(scene_big_cmd_kernel, dim3(grid), dim3(256), 0, s, p, 0);
-    hipLaunchKernelGGL((scene_geom_kernel<kBigGeomBlock, kGeomFront>), dim3(p.nBig), dim3(kBigGeomBlock), 0, s,
-                       p, 0);
+    // point layout: subpath chunks over the device
+    hipLaunchKernelGGL(scene_big_plan_kernel, dim3(1), dim3(kScanBlock), 0, s, p, 2);
+    hipLaunchKernelGGL(scene_big_front_kernel, dim3(kBigSegGrid), dim3(256), 0, s, p, 0);
+    hipLaunchKernelGGL(scene_big_front_alloc_kernel, dim3(p.nBig), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(scene_big_front_kernel, dim3(kBigSegGrid), dim3(256), 0, s, p, 1);
     hipLaunchKernelGGL(scene_big_cmd_kernel, dim3(grid), dim3(256), 0, s, p, 1);
+    // fills: segments in chunks over the device; strokes: one workgroup
+    hipLaunchKernelGGL(scene_big_plan_kernel, dim3(1), dim3(kScanBlock), 0, s, p, 0);
+    hipLaunchKernelGGL(scene_big_seg_kernel, dim3(kBigSegGrid), dim3(256), 0, s, p, 0);
+    hipLaunchKernelGGL(scene_big_seg_alloc_kernel, dim3(p.nBig), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(scene_big_plan_kernel, dim3(1), dim3(kScanBlock), 0, s, p, 1);
+    hipLaunchKernelGGL(scene_big_seg_kernel, dim3(kBigSegGrid), dim3(256), 0, s, p, 1);
+    hipLaunchKernelGGL(scene_big_band_kernel, dim3(kBigSegGrid), dim3(256), 0, s, p);
     hipLaunchKernelGGL((scene_geom_kernel<kBigGeomBlock, kGeomBack>), dim3(p.nBig), dim3(kBigGeomBlock), 0, s,
                        p, 0);
   }
