@@ -6,6 +6,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <set>
 #include <mutex>
@@ -33,6 +34,15 @@ struct pxh_scene {
   std::vector<pxh::SceneCmd> cmds;
   std::vector<pxh::ScenePoly> polys;
   std::vector<float> dashes;   // dash patterns of the recorded strokes
+  // text calls: their commands and subpaths are written on the device by
+  // glyph instances, in a region after the host-recorded ones (item index,
+  // region-relative command and subpath bases)
+  std::vector<pxh::GlyphInst> glyphInst;  // dstCmd / dstPoly region-relative
+  std::vector<pxh::GlyphInst> glyphAbs;   // the uploaded copy, destinations absolute
+  std::vector<std::array<int32_t, 3>> textItems;
+  int32_t textCmds = 0, textPolys = 0;
+  pxh::GlyphInst* dInst = nullptr;
+  size_t dInstCap = 0;
   bool uploaded = false;
   int32_t validW = -1, validH = -1;  // target size the pools were validated for
   int32_t lane = 0;  // lane of the last render
@@ -152,6 +162,18 @@ struct Ctx {
   DevBuf paintFill;  // an image paint drawn onto a transparent canvas
   DevBuf drawRows;   // drawSmooth: per-row spans and run start positions
   pxh_scene* bigScene = nullptr;  // per-call fills / strokes of many commands
+  // text: the canonical commands of every glyph drawn so far (grow-only),
+  // keyed by (typeface, rune), and their device copy (re-uploaded as it grows)
+  struct GlyphRec {
+    int32_t cmd, nCmds, poly, nPolys;
+  };
+  std::map<std::pair<const void*, uint32_t>, int32_t> glyphKeys;
+  std::vector<GlyphRec> glyphRecs;
+  std::vector<SceneCmd> glyphCmds;
+  std::vector<ScenePoly> glyphPolys;
+  SceneCmd* dGlyphCmds = nullptr;
+  ScenePoly* dGlyphPolys = nullptr;
+  size_t dGlyphCmdsCap = 0, dGlyphPolysCap = 0, upGlyphCmds = 0, upGlyphPolys = 0;
   int32_t* errFlag = nullptr;
   // Per lane: error bits of scene renders that stopped before compose (an
   // overflowed pool); read and cleared when the lane is synchronized.  A
@@ -647,6 +669,10 @@ bool has_arc(const float* c, int32_t n) {
   return false;
 }
 
+void item_params(pxh_scene* sc, SceneItem* itp, const float* mat9, uint32_t rgbx, int32_t rule, int32_t mode,
+                 bool stroke, float width, int32_t cap, int32_t join, float miterLimit, const float* dashes,
+                 int32_t ndashes);
+
 int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9, uint32_t rgbx,
                  int32_t rule, int32_t mode, bool stroke, float width, int32_t cap, int32_t join,
                  float miterLimit, const float* dashes = nullptr, int32_t ndashes = 0) {
@@ -676,7 +702,22 @@ int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9,
     sc->polys.resize(polys0);
     return r;
   }
-  // dash pattern: an odd one is repeated to even length (paths.nim:2025-2027)
+  it.cmdCount = (int32_t)sc->cmds.size() - it.cmdBase;
+  it.polyCount = (int32_t)sc->polys.size() - it.polyBase;
+  item_params(sc, &it, mat9, rgbx, rule, mode, stroke, width, cap, join, miterLimit, dashes, ndashes);
+  sc->items.push_back(it);
+  sc->uploaded = false;
+  sc->gen++;
+  sc->validW = sc->validH = -1;
+  return 0;
+}
+
+// The paint, stroke and transform parameters of a recorded call, and its
+// dash pattern (an odd one is repeated to even length, paths.nim:2025-2027).
+void item_params(pxh_scene* sc, SceneItem* itp, const float* mat9, uint32_t rgbx, int32_t rule, int32_t mode,
+                 bool stroke, float width, int32_t cap, int32_t join, float miterLimit, const float* dashes,
+                 int32_t ndashes) {
+  SceneItem& it = *itp;
   it.dashBase = (int32_t)sc->dashes.size();
   it.dashCount = 0;
   it.dashBad = 0;
@@ -689,8 +730,6 @@ int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9,
       for (int32_t k = 0; k < ndashes; k++) sc->dashes.push_back(dashes[k]);
     it.dashCount = (int32_t)sc->dashes.size() - it.dashBase;
   }
-  it.cmdCount = (int32_t)sc->cmds.size() - it.cmdBase;
-  it.polyCount = (int32_t)sc->polys.size() - it.polyBase;
   it.stroke = stroke ? 1 : 0;
   it.rgbx = rgbx;
   it.mode = mode;
@@ -705,6 +744,52 @@ int scene_record(pxh_scene* sc, const float* cmds, int32_t n, const float* mat9,
   memcpy(it.mat, mat9, sizeof(it.mat));
   it.identity = mat_is_identity(mat9) ? 1 : 0;
   it.strokeNone = (stroke && width <= 0.0f) ? 1 : 0;
+}
+
+// A text call: its commands and subpaths are the glyph instances', placed
+// on the device at upload (scene_launch_expand) in the text region.
+int text_record_impl(pxh_scene* sc, const std::vector<TextInst>& inst, const float* mat9, uint32_t rgbx,
+                     int32_t rule, int32_t mode, bool stroke, float width, int32_t cap, int32_t join,
+                     float miterLimit, const float* dashes, int32_t ndashes) {
+  if (!sc) return fail(kErrArg, "invalid scene handle");
+  if (mode < 0 || mode >= kNumBlendModes) return fail(kErrArg, "invalid blend mode");
+  if (rule != kNonZero && rule != kEvenOdd) return fail(kErrArg, "invalid winding rule");
+  if (ndashes < 0 || (ndashes > 0 && !dashes)) return fail(kErrArg, "invalid dash array");
+  if (!mat9) mat9 = kIdentity;
+  if (stroke && width <= 0.0f) return 0;  // strokeShapes yields nothing (paths.nim:1931-1932)
+  const int32_t relC = sc->textCmds, relP = sc->textPolys;
+  int32_t nC = 0, nP = 0;
+  for (const TextInst& t : inst) {
+    GlyphInst gi = {};
+    if (t.ref < 0) {
+      gi.srcCmd = gi.srcPoly = -1;
+      gi.nCmds = 5;
+      gi.nPolys = 1;
+    } else {
+      if (t.ref >= (int32_t)g.glyphRecs.size()) return fail(kErrArg, "invalid glyph reference");
+      const Ctx::GlyphRec& r = g.glyphRecs[t.ref];
+      gi.srcCmd = r.cmd;
+      gi.nCmds = r.nCmds;
+      gi.srcPoly = r.poly;
+      gi.nPolys = r.nPolys;
+    }
+    if (gi.nCmds == 0) continue;  // an empty glyph adds nothing
+    gi.dstCmd = relC + nC;
+    gi.dstPoly = relP + nP;
+    gi.polyIdx = nP;
+    gi.identity = t.identity;
+    memcpy(gi.m, t.m, sizeof(gi.m));
+    sc->glyphInst.push_back(gi);
+    nC += gi.nCmds;
+    nP += gi.nPolys;
+  }
+  SceneItem it = {};
+  it.cmdCount = nC;
+  it.polyCount = nP;
+  item_params(sc, &it, mat9, rgbx, rule, mode, stroke, width, cap, join, miterLimit, dashes, ndashes);
+  sc->textItems.push_back({(int32_t)sc->items.size(), relC, relP});
+  sc->textCmds += nC;
+  sc->textPolys += nP;
   sc->items.push_back(it);
   sc->uploaded = false;
   sc->gen++;
@@ -726,6 +811,9 @@ void scene_free_device(pxh_scene* sc) {
   if (sc->dDashes) (void)hipFree(sc->dDashes);
   sc->dDashes = nullptr;
   sc->dDashesCap = 0;
+  if (sc->dInst) (void)hipFree(sc->dInst);
+  sc->dInst = nullptr;
+  sc->dInstCap = 0;
   if (sc->dMem) (void)hipFree(sc->dMem);
   sc->dItems = nullptr;
   sc->dCmds = nullptr;
@@ -739,7 +827,8 @@ void scene_free_device(pxh_scene* sc) {
 // (Re)allocates the per-render pools of a scene in one device block.
 int scene_alloc(pxh_scene* sc) {
   sc->gen++;
-  const size_t nI = sc->items.size(), nC = sc->cmds.size(), nP = sc->polys.size();
+  const size_t nI = sc->items.size(), nC = sc->cmds.size() + (size_t)sc->textCmds;
+  const size_t nP = sc->polys.size() + (size_t)sc->textPolys;
   size_t off = 0;
   auto carve = [&](size_t bytes) {
     size_t o = off;
@@ -845,20 +934,69 @@ int scene_buf(T** p, size_t* cap, size_t n) {
   return 0;
 }
 
+// The glyph arena's device copy, grown (whole) or extended by its new tail.
+int glyph_arena_upload(hipStream_t st) {
+  const size_t nc = g.glyphCmds.size(), np = g.glyphPolys.size();
+  if (nc == g.upGlyphCmds && np == g.upGlyphPolys) return 0;
+  if (nc > g.dGlyphCmdsCap || np > g.dGlyphPolysCap) {
+    PXH_HIP(hipDeviceSynchronize());  // other lanes' expansions may read the old copy
+    if (g.dGlyphCmds) PXH_HIP(hipFree(g.dGlyphCmds));
+    if (g.dGlyphPolys) PXH_HIP(hipFree(g.dGlyphPolys));
+    g.dGlyphCmds = nullptr;
+    g.dGlyphPolys = nullptr;
+    g.dGlyphCmdsCap = g.dGlyphPolysCap = 0;
+    g.upGlyphCmds = g.upGlyphPolys = 0;
+    if (int r = scene_buf(&g.dGlyphCmds, &g.dGlyphCmdsCap, nc * 2 + 4096)) return r;
+    if (int r = scene_buf(&g.dGlyphPolys, &g.dGlyphPolysCap, np * 2 + 1024)) return r;
+  }
+  PXH_HIP(hipMemcpyAsync(g.dGlyphCmds + g.upGlyphCmds, g.glyphCmds.data() + g.upGlyphCmds,
+                         (nc - g.upGlyphCmds) * sizeof(SceneCmd), hipMemcpyHostToDevice, st));
+  if (np > g.upGlyphPolys)
+    PXH_HIP(hipMemcpyAsync(g.dGlyphPolys + g.upGlyphPolys, g.glyphPolys.data() + g.upGlyphPolys,
+                           (np - g.upGlyphPolys) * sizeof(ScenePoly), hipMemcpyHostToDevice, st));
+  g.upGlyphCmds = nc;
+  g.upGlyphPolys = np;
+  PXH_HIP(hipStreamSynchronize(st));  // the host arena may grow (and move) after this
+  return 0;
+}
+
 int scene_upload(pxh_scene* sc, int lane) {
   if (sc->uploaded) return 0;
   hipStream_t st = g.lane[lane];
   PXH_HIP(hipStreamSynchronize(g.lane[sc->lane]));  // a previous render may still read the buffers
+  // text calls take the region after the host-recorded commands and subpaths
+  const int32_t regC = (int32_t)sc->cmds.size(), regP = (int32_t)sc->polys.size();
+  for (const auto& t : sc->textItems) {
+    sc->items[t[0]].cmdBase = regC + t[1];
+    sc->items[t[0]].polyBase = regP + t[2];
+  }
+  const size_t allC = (size_t)regC + sc->textCmds, allP = (size_t)regP + sc->textPolys;
   if (int r = scene_buf(&sc->dItems, &sc->dItemsCap, sc->items.size())) return r;
-  if (int r = scene_buf(&sc->dCmds, &sc->dCmdsCap, sc->cmds.size())) return r;
-  if (int r = scene_buf(&sc->dPolys, &sc->dPolysCap, sc->polys.size())) return r;
+  if (int r = scene_buf(&sc->dCmds, &sc->dCmdsCap, allC)) return r;
+  if (int r = scene_buf(&sc->dPolys, &sc->dPolysCap, allP)) return r;
   PXH_HIP(hipMemcpyAsync(sc->dItems, sc->items.data(), sc->items.size() * sizeof(SceneItem),
                          hipMemcpyHostToDevice, st));
-  PXH_HIP(hipMemcpyAsync(sc->dCmds, sc->cmds.data(), sc->cmds.size() * sizeof(SceneCmd),
-                         hipMemcpyHostToDevice, st));
-  if (!sc->polys.empty())
-    PXH_HIP(hipMemcpyAsync(sc->dPolys, sc->polys.data(), sc->polys.size() * sizeof(ScenePoly),
+  if (regC > 0)
+    PXH_HIP(hipMemcpyAsync(sc->dCmds, sc->cmds.data(), (size_t)regC * sizeof(SceneCmd), hipMemcpyHostToDevice,
+                           st));
+  if (regP > 0)
+    PXH_HIP(hipMemcpyAsync(sc->dPolys, sc->polys.data(), (size_t)regP * sizeof(ScenePoly),
                            hipMemcpyHostToDevice, st));
+  if (!sc->glyphInst.empty()) {
+    if (int r = glyph_arena_upload(st)) return r;
+    // region-relative destinations -> absolute, in a copy that stays
+    // untouched until the next upload (which first waits for the lane)
+    const size_t n = sc->glyphInst.size();
+    sc->glyphAbs = sc->glyphInst;
+    for (GlyphInst& gi : sc->glyphAbs) {
+      gi.dstCmd += regC;
+      gi.dstPoly += regP;
+    }
+    if (int r = scene_buf(&sc->dInst, &sc->dInstCap, n)) return r;
+    PXH_HIP(hipMemcpyAsync(sc->dInst, sc->glyphAbs.data(), n * sizeof(GlyphInst), hipMemcpyHostToDevice, st));
+    if (scene_launch_expand(sc->dInst, (int32_t)n, g.dGlyphCmds, g.dGlyphPolys, sc->dCmds, sc->dPolys, st))
+      return fail(kErrDevice, "glyph expansion launch failed");
+  }
   if (int r = scene_buf(&sc->dDashes, &sc->dDashesCap, sc->dashes.size())) return r;
   if (!sc->dashes.empty())
     PXH_HIP(hipMemcpyAsync(sc->dDashes, sc->dashes.data(), sc->dashes.size() * sizeof(float),
@@ -901,7 +1039,7 @@ int scene_upload(pxh_scene* sc, int lane) {
   // first estimates of the pools; a render that overflows grows them.  The
   // sizes a scene has grown to are kept across clear() and re-recording, so
   // a scene reused for similar requests validates once, not per request.
-  size_t nC = sc->cmds.size();
+  size_t nC = allC;
   auto keep = [](int32_t prev, int64_t est) { return (int32_t)std::max<int64_t>(prev, est); };
   sc->ptCap = keep(sc->ptCap, std::min<size_t>(64 * nC + 4096, 1u << 28));
   sc->segCap = keep(sc->segCap, (int64_t)sc->ptCap * 2);
@@ -1966,6 +2104,9 @@ int pxh_scene_clear(pxh_scene* sc) {
   sc->cmds.clear();
   sc->polys.clear();
   sc->dashes.clear();
+  sc->glyphInst.clear();
+  sc->textItems.clear();
+  sc->textCmds = sc->textPolys = 0;
   sc->uploaded = false;
   sc->validW = sc->validH = -1;
   return 0;
@@ -2081,7 +2222,7 @@ int pxh_scene_autotune(pxh_scene* sc, pxh_image* img, int32_t reps, int32_t* sha
   if (reps < 1) reps = 1;
   // a scene of the same shape of work on the same device and target reuses
   // the shape found before
-  size_t ncmd = sc->cmds.size(), bucket = 0;
+  size_t ncmd = sc->cmds.size() + (size_t)sc->textCmds, bucket = 0;
   while (ncmd >>= 1) bucket++;
   const std::string key = g.arch + "|" + std::to_string(img->w) + "x" + std::to_string(img->h) + "|" +
                           std::to_string(img->fmt) + "|" + std::to_string(sc->items.size()) + "|" +
@@ -2320,3 +2461,70 @@ int pxh_debug_fill_geometry(const float* cmds, int32_t ncmds, const float* mat9,
 }
 
 }  // extern "C"
+
+// ---- text calls through a scene (fonts.nim:566-690) ----
+namespace pxh {
+
+int text_glyph_ref(const void* typeface, uint32_t rune, const float* cmds, int32_t n, int32_t* ref) {
+  const auto key = std::make_pair(typeface, rune);
+  auto it = g.glyphKeys.find(key);
+  if (it != g.glyphKeys.end()) {
+    *ref = it->second;
+    return 0;
+  }
+  Ctx::GlyphRec r;
+  r.cmd = (int32_t)g.glyphCmds.size();
+  r.poly = (int32_t)g.glyphPolys.size();
+  if (int e = canonicalize(cmds, n, &g.glyphCmds, &g.glyphPolys)) {
+    g.glyphCmds.resize(r.cmd);
+    g.glyphPolys.resize(r.poly);
+    return e;
+  }
+  r.nCmds = (int32_t)g.glyphCmds.size() - r.cmd;
+  r.nPolys = (int32_t)g.glyphPolys.size() - r.poly;
+  *ref = (int32_t)g.glyphRecs.size();
+  g.glyphRecs.push_back(r);
+  g.glyphKeys[key] = *ref;
+  return 0;
+}
+
+// Typefaces changed (freed, or their fallbacks): later lookups re-record.
+void text_glyphs_forget() { g.glyphKeys.clear(); }
+
+pxh_scene* text_scene() {
+  if (pxh_init() != 0) return nullptr;
+  pxh_scene* sc = big_call_scene();
+  if (sc) pxh_scene_clear(sc);
+  return sc;
+}
+
+int text_record(pxh_scene* sc, const std::vector<TextInst>& inst, const float* mat9, uint32_t rgbx, int32_t rule,
+                int32_t mode, bool stroke, float width, int32_t cap, int32_t join, float miterLimit,
+                const float* dashes, int32_t ndashes) {
+  return text_record_impl(sc, inst, mat9, rgbx, rule, mode, stroke, width, cap, join, miterLimit, dashes,
+                          ndashes);
+}
+
+// 0 when drawn; -1 when the pools ran out (the canvas is untouched and the
+// caller draws the calls one by one); else the first failing call's error
+// (the other calls are drawn).
+int text_scene_render(pxh_image* img) {
+  pxh_scene* sc = g.bigScene;
+  const int32_t n = pxh_scene_count(sc);
+  if (n == 0) return 0;
+  if (int r = pxh_scene_render_on(sc, img, 0)) return r == kErrCapacity ? -1 : r;
+  std::vector<int32_t> st((size_t)n);
+  if (int r = pxh_scene_status(sc, st.data(), n)) return r;
+  for (int32_t k = 0; k < n; k++) {
+    if (st[k] == kOk) continue;
+    if (st[k] == kErrIntOverflow) return fail(st[k], "Path int overflow detected");
+    if (st[k] == kErrDash) return fail(st[k], "Invalid line dash value");
+    if (st[k] == kErrDiscretize) return fail(st[k], "Unable to discretize curve");
+    return fail(st[k], "a text call failed");
+  }
+  return 0;
+}
+
+bool paint_solid_rgbx(const pxh_paint* p, uint32_t* rgbx) { return solid_rgbx(p, rgbx); }
+
+}  // namespace pxh

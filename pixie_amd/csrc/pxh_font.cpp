@@ -1261,6 +1261,74 @@ void span_path(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<f
   }
 }
 
+// computePaths (fonts.nim:517-564) of span si as glyph instances for the
+// device: each glyph's cached outline with its placement, each decoration
+// rect as written by path.rect.
+int span_instances(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<TextInst>* out) {
+  const pxh_font* font = a->fonts[si];
+  pxh_typeface* t = font->typeface;
+  const float scale = font_scale(font);
+  const float ut = (float)t->underlineThickness * scale, up = (float)t->underlinePosition * scale;
+  const float st = (float)t->strikeoutSize * scale, sp = (float)t->strikeoutPosition * scale;
+  const int32_t L = (int32_t)a->lines.size();
+  const float S[9] = {scale, 0, 0, 0, scale, 0, 0, 0, 1};
+  std::vector<float> tmp;
+  for (int32_t ri = a->spans[si].first; ri <= a->spans[si].second; ri++) {
+    const float px = a->pos[2 * ri], py = a->pos[2 * ri + 1];
+    const uint32_t rune = a->runes[ri];
+    const std::vector<float>& gp = glyph_ref(t, rune, &tmp);
+    if (!gp.empty()) {
+      const float T[9] = {1, 0, 0, 0, 1, 0, px, py, 1};
+      float M[9];
+      m3_mul(T, S, M);
+      TextInst gi;
+      if (int r = text_glyph_ref(t, rune, gp.data(), (int32_t)gp.size(), &gi.ref)) return r;
+      gi.identity = is_identity(M) ? 1 : 0;
+      const float m6[6] = {M[0], M[1], M[3], M[4], M[6], M[7]};
+      memcpy(gi.m, m6, sizeof(m6));
+      out->push_back(gi);
+    }
+    bool decorate = true;
+    if (*line < L && ri == a->lines[*line].second) {
+      (*line)++;
+      if (rune == kSP) decorate = false;
+    }
+    if (decorate) {
+      const float* r = a->rects.data() + 4 * ri;
+      if (font->underline) out->push_back({-1, 0, {r[0], py - up + ut / 2.0f, r[2], ut, 0, 0}});
+      if (font->strikethrough) out->push_back({-1, 0, {r[0], py - sp, r[2], st, 0, 0}});
+    }
+  }
+  return 0;
+}
+
+// fillText / strokeText with solid paints only: every (span, paint) as a
+// text call of one scene, drawn in one device pass.  0 when drawn, -1 when
+// the caller must draw span by span, else an error status.
+int text_through_scene(pxh_image* img, const pxh_arrangement* a, const float* mat9, bool stroke, float width,
+                       int32_t cap, int32_t join, float miter, const float* dashes, int32_t ndashes) {
+  for (const pxh_font* f : a->fonts)
+    for (int32_t k = 0; k < f->npaints; k++)
+      if (f->paints[k].kind != PXH_SOLID_PAINT) return -1;
+  pxh_scene* sc = text_scene();
+  if (!sc) return -1;
+  int32_t line = 0;
+  std::vector<TextInst> inst;
+  for (size_t s = 0; s < a->spans.size(); s++) {
+    inst.clear();
+    if (int r = span_instances(a, s, &line, &inst)) return r;
+    const pxh_font* f = a->fonts[s];
+    for (int32_t k = 0; k < f->npaints; k++) {
+      uint32_t rgbx;
+      if (!paint_solid_rgbx(&f->paints[k], &rgbx)) continue;  // fillPath draws nothing
+      if (int r = text_record(sc, inst, mat9, rgbx, kNonZero, f->paints[k].blend_mode, stroke, width, cap, join,
+                              miter, dashes, ndashes))
+        return r;
+    }
+  }
+  return text_scene_render(img);
+}
+
 std::vector<std::vector<float>> compute_paths(const pxh_arrangement* a) {
   std::vector<std::vector<float>> paths(a->spans.size());
   int32_t line = 0;
@@ -1347,11 +1415,15 @@ pxh_typeface* pxh_typeface_parse_collection(const uint8_t* data, int64_t len, in
   }
 }
 
-void pxh_typeface_free(pxh_typeface* tf) { delete tf; }
+void pxh_typeface_free(pxh_typeface* tf) {
+  if (tf) text_glyphs_forget();  // its address may come back as another typeface
+  delete tf;
+}
 
 int pxh_typeface_set_fallbacks(pxh_typeface* tf, pxh_typeface* const* fallbacks, int32_t n) {
   if (!tf || n < 0 || (n > 0 && !fallbacks)) return set_last_error(kErrArg, "invalid fallbacks");
   tf->fallbacks.assign(fallbacks, fallbacks + n);
+  text_glyphs_forget();  // a rune may now resolve to another typeface's glyph
   return 0;
 }
 
@@ -1549,6 +1621,12 @@ int pxh_fill_text(pxh_image* img, const pxh_arrangement* a, const float* mat9) {
   if (!a) return set_last_error(kErrArg, "invalid arrangement");
   for (const pxh_font* f : a->fonts)
     if (int r = check_font(f)) return r;
+  try {
+    const int r = text_through_scene(img, a, mat9, false, 0.0f, 0, 0, 4.0f, nullptr, 0);
+    if (r >= 0) return r;
+  } catch (const FontError& e) {
+    return set_last_error(kErrParse, e.what());
+  }
   std::vector<std::vector<float>> paths;
   try {
     paths = compute_paths(a);
@@ -1571,6 +1649,13 @@ int pxh_stroke_text(pxh_image* img, const pxh_arrangement* a, const float* mat9,
   if (!a) return set_last_error(kErrArg, "invalid arrangement");
   for (const pxh_font* f : a->fonts)
     if (int r = check_font(f)) return r;
+  try {
+    const int r = text_through_scene(img, a, mat9, true, stroke_width, line_cap, line_join, miter_limit, dashes,
+                                     ndashes);
+    if (r >= 0) return r;
+  } catch (const FontError& e) {
+    return set_last_error(kErrParse, e.what());
+  }
   std::vector<std::vector<float>> paths;
   try {
     paths = compute_paths(a);

@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "../../include/pixie_hip.h"
 #include "pxh_common.h"
 #include "pxh_geom.h"
 
@@ -49,6 +50,26 @@ struct FillGeom {
 
 // Sets the message pxh_last_error() returns and returns `code` (pxh_api.cpp).
 int set_last_error(int code, const std::string& msg);
+
+// Text calls (pxh_api.cpp): span paths drawn through the library's scene,
+// their glyphs' canonical commands kept once in a device glyph arena and
+// placed per glyph on the device.  An instance is a glyph (ref from
+// text_glyph_ref) transformed by m (columns 0 and 1 and the translation of
+// a Mat3; identity = copy), or, with ref < 0, path.rect(m[0], m[1], m[2],
+// m[3]).
+struct TextInst {
+  int32_t ref;
+  int32_t identity;
+  float m[6];
+};
+int text_glyph_ref(const void* typeface, uint32_t rune, const float* cmds, int32_t n, int32_t* ref);
+void text_glyphs_forget();
+pxh_scene* text_scene();  // cleared
+int text_record(pxh_scene* sc, const std::vector<TextInst>& inst, const float* mat9, uint32_t rgbx, int32_t rule,
+                int32_t mode, bool stroke, float width, int32_t cap, int32_t join, float miterLimit,
+                const float* dashes, int32_t ndashes);
+int text_scene_render(pxh_image* img);
+bool paint_solid_rgbx(const pxh_paint* p, uint32_t* rgbx);
 
 int32_t parse_path(const char* s, std::vector<float>* out, std::string* err);
 float pixel_scale(const float* mat9);

@@ -34,6 +34,21 @@ struct ScenePoly {     // one subpath of a call
   int32_t pad;
 };
 
+// One glyph (or underline / strikethrough rect) of a text call, placed into
+// the call's command and subpath range on the device (scene_launch_expand):
+// a glyph's canonical commands are copied from the glyph arena with every
+// point transformed by m (a vmath Mat3 without its last row: m[0] m[1] are
+// column 0, m[2] m[3] column 1, m[4] m[5] the translation); a rect
+// (srcCmd < 0) is path.rect(m[0], m[1], m[2], m[3]) written out.
+struct GlyphInst {
+  int32_t srcCmd, nCmds;    // arena commands
+  int32_t srcPoly, nPolys;  // arena subpaths
+  int32_t dstCmd, dstPoly;  // absolute destination in the scene
+  int32_t polyIdx;          // subpath index of the first subpath within its call
+  int32_t identity;         // m is the identity (the points are copied)
+  float m[6];
+};
+
 struct SceneItem {
   int32_t cmdBase, cmdCount;   // SceneCmd range
   int32_t polyBase, polyCount; // ScenePoly range
@@ -250,6 +265,8 @@ inline int scene_scratch_slots(const RowShape& s, int numCUs) {
 
 int scene_set_lds_limits(size_t bytes);
 size_t scene_gscratch_bytes(int rowGrid, int gcap);  // global row scratch of scene_launch_rows
+int scene_launch_expand(const GlyphInst* inst, int32_t n, const SceneCmd* arenaCmds,
+                        const ScenePoly* arenaPolys, SceneCmd* cmds, ScenePoly* polys, hipStream_t s);
 int scene_launch_clear(const ScenePools& p, hipStream_t s);
 int scene_launch_geometry(const ScenePools& p, hipStream_t s);
 int scene_launch_bands(const ScenePools& p, int grid, hipStream_t s);

@@ -2181,6 +2181,73 @@ __global__ void __launch_bounds__(256) scene_clear_kernel(int32_t* counters, int
   for (int i = threadIdx.x; i < nq; i += blockDim.x) queues[i] = 0;
 }
 
+// Text calls: one wave per glyph instance writes its commands and subpaths
+// into the scene's recording (GlyphInst).  Points are transformed as
+// Path.transform transforms them (paths.nim:264-333): m * (x, y) in float32.
+__global__ void __launch_bounds__(256) scene_expand_kernel(const GlyphInst* inst, int n,
+                                                           const SceneCmd* aCmds, const ScenePoly* aPolys,
+                                                           SceneCmd* cmds, ScenePoly* polys) {
+  const int lane = threadIdx.x & 63;
+  for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += gridDim.x * 4) {
+    const GlyphInst g = inst[w];
+    const float* m = g.m;
+    if (g.srcCmd < 0) {  // path.rect(x, y, w, h), clockwise (paths.nim:506-515)
+      const float x = m[0], y = m[1], rw = m[2], rh = m[3];
+      if (lane < 5) {
+        SceneCmd c = {};
+        c.kind = lane == 0 ? 1.0f : lane == 4 ? 0.0f : 2.0f;
+        if (lane == 0) { c.p[0] = x; c.p[1] = y; }
+        if (lane == 1) { c.p[0] = x + rw; c.p[1] = y; }
+        if (lane == 2) { c.p[0] = x + rw; c.p[1] = y + rh; }
+        if (lane == 3) { c.p[0] = x; c.p[1] = y + rh; }
+        c.sx = x;
+        c.sy = y;
+        c.pad[0] = g.polyIdx;
+        cmds[g.dstCmd + lane] = c;
+      }
+      if (lane == 0) {
+        ScenePoly sp = {};
+        sp.firstCmd = g.dstCmd;
+        sp.numCmds = 5;
+        polys[g.dstPoly] = sp;
+      }
+      continue;
+    }
+    for (int k = lane; k < g.nCmds; k += 64) {
+      SceneCmd c = aCmds[g.srcCmd + k];
+      if (!g.identity) {
+        const int kind = (int)c.kind;
+        const int pts = kind == 5 ? 3 : kind == 7 ? 2 : (kind == 1 || kind == 2) ? 1 : 0;
+        for (int q = 0; q < pts; q++) {
+          const float x = c.p[2 * q], y = c.p[2 * q + 1];
+          c.p[2 * q] = m[0] * x + m[2] * y + m[4];
+          c.p[2 * q + 1] = m[1] * x + m[3] * y + m[5];
+        }
+        const float sx = c.sx, sy = c.sy;
+        c.sx = m[0] * sx + m[2] * sy + m[4];
+        c.sy = m[1] * sx + m[3] * sy + m[5];
+      }
+      c.pad[0] += g.polyIdx;
+      cmds[g.dstCmd + k] = c;
+    }
+    for (int k = lane; k < g.nPolys; k += 64) {
+      ScenePoly sp = aPolys[g.srcPoly + k];
+      sp.firstCmd = sp.firstCmd - g.srcCmd + g.dstCmd;
+      polys[g.dstPoly + k] = sp;
+    }
+  }
+}
+
+int scene_launch_expand(const GlyphInst* inst, int32_t n, const SceneCmd* arenaCmds, const ScenePoly* arenaPolys,
+                        SceneCmd* cmds, ScenePoly* polys, hipStream_t s) {
+  if (n <= 0) return 0;
+  int grid = (n + 3) / 4;
+  if (grid > 2048) grid = 2048;
+  hipLaunchKernelGGL(scene_expand_kernel, dim3(grid), dim3(256), 0, s, inst, n, arenaCmds, arenaPolys, cmds,
+                     polys);
+  return hipGetLastError() == hipSuccess ? 0 : kErrDevice;
+}
+
 int scene_launch_clear(const ScenePools& p, hipStream_t s) {
   hipLaunchKernelGGL(scene_clear_kernel, dim3(1), dim3(256), 0, s, p.counters, (int)kCounterInts,
                      p.queues, kQueueSets * kQueues * kQueueStride);
