@@ -43,6 +43,8 @@ struct RowLds {
   float* tx0;        // cap: trapezoid x at scanTop (by selection position)
   float* tx1;        // cap: trapezoid x at scanBottom
   int cap, ww;
+  uint64_t* wide;    // LDS staging of the wide path (wideCap keys), or null
+  int wideCap;
 };
 
 PXH_HD size_t row_lds_fixed_bytes(int ww) { return (size_t)(ww + 4) * 4 + (size_t)ww * 4; }
@@ -60,6 +62,8 @@ __device__ __forceinline__ RowLds row_lds_carve(char* fixed, char* arr, int cap,
   L.B = L.A + cap;
   L.tx0 = (float*)(L.B + cap);
   L.tx1 = L.tx0 + cap;
+  L.wide = nullptr;
+  L.wideCap = 0;
   return L;
 }
 
@@ -237,90 +241,44 @@ __device__ __forceinline__ int32_t key_at(uint64_t k) {
 // instead of L.B[low word].
 __device__ __forceinline__ int32_t key_winding(uint64_t k) { return (int32_t)(int8_t)(uint8_t)k; }
 
-template <bool kPacked = false, class SpanFn>
-__device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
-  int lane = lane_id();
-  int32_t carryCount = 0;
-  int32_t carryPrev = 0;      // prevAt so far (0 before any event)
-  bool carryConsumed = false; // first hit of this chunk consumed by a skip
-  for (int base = 0; base < n; base += kWave) {
-    int i = base + lane;
-    bool valid = i < n;
-    uint64_t k = valid ? L.keys[i] : 0ull;
-    int32_t at = valid ? key_at(k) : 0;
-    int32_t w = valid ? (kPacked ? key_winding(k) : L.B[(uint32_t)k]) : 0;
-    int32_t tot;
-    int32_t countBefore = carryCount + wave_excl_scan(w, &tot);
-    bool filling = should_fill(rule, countBefore);
-    bool hasNext = valid && i < n - 1;
-    uint64_t kn = hasNext ? L.keys[i + 1] : 0ull;
-    int32_t nextAt = hasNext ? key_at(kn) : 0;
-    int32_t nextW = hasNext ? (kPacked ? key_winding(kn) : L.B[(uint32_t)kn]) : 0;
-    bool canSkip = valid && at > 0 && filling && hasNext && nextAt == at && w + nextW == 0;
-    uint64_t cm = ballot(canSkip);
-    uint64_t skipMask = 0, consMask = carryConsumed ? 1ull : 0ull;
-    bool carryOut = false;
-    while (cm) {
-      int b = __builtin_ctzll(cm);
-      cm &= cm - 1;
-      if (!((consMask >> b) & 1ull)) {
-        skipMask |= 1ull << b;
-        if (b + 1 < kWave) consMask |= 1ull << (b + 1);
-        else carryOut = true;
-      }
-    }
-    bool consumed = (consMask >> lane) & 1ull;
-    bool skipStart = (skipMask >> lane) & 1ull;
-    // 0 = none, 1 = start (prevAt = at), 2 = end (yield, prevAt = at)
-    int ev = 0;
-    if (valid && !consumed && !skipStart && at > 0) {
-      if (filling) {
-        bool cont = (rule == kNonZero) && hasNext && (countBefore + w != 0);
-        ev = cont ? 0 : 2;
-      } else {
-        ev = 1;
-      }
-    }
-    int32_t prevHere = wave_excl_max(ev ? at : INT32_MIN);
-    int32_t prevAt = prevHere > carryPrev ? prevHere : carryPrev;
-    if (ev == 2) span(prevAt, at);
-    int32_t lastEv = wave_max(ev ? at : INT32_MIN);
-    if (lastEv > carryPrev) carryPrev = lastEv;
-    carryCount += tot;
-    carryConsumed = carryOut;
-  }
-}
+// Walk state carried from one 64-hit chunk to the next.
+struct WalkCarry {
+  int32_t count = 0;      // winding count before the chunk
+  int32_t prev = 0;       // prevAt so far (0 before any event)
+  bool consumed = false;  // the chunk's first hit was consumed by a skip
+};
 
-// wave_walk<true> for at most 64 sorted hits held in registers (hit i in
-// lane i, padding keys ~0 beyond n): the same events, with the next hit
-// taken from the next lane.
+// One chunk of the walk: lane i holds a sorted hit (at, w) when valid and,
+// when hasNext, the hit after it (nextAt, nextW), which for the last lane is
+// the next chunk's first hit.  Yields pixie's spans in parallel: winding
+// counts by prefix sum, the zero-width skip of a cancelling pair by ballots
+// (a skip consumes the next hit, so skips are taken left to right), the
+// NonZero shortcut per lane, and prevAt as an exclusive prefix max of the
+// event positions.
 template <class SpanFn>
-__device__ __forceinline__ void wave_walk_reg(uint64_t k, int n, int32_t rule, SpanFn& span) {
+__device__ __forceinline__ void walk_chunk(int32_t at, int32_t w, bool valid, bool hasNext, int32_t nextAt,
+                                           int32_t nextW, int32_t rule, WalkCarry& c, SpanFn& span) {
   const int lane = lane_id();
-  const bool valid = lane < n;
-  const int32_t at = valid ? key_at(k) : 0;
-  const int32_t w = valid ? key_winding(k) : 0;
   int32_t tot;
-  const int32_t countBefore = wave_excl_scan(w, &tot);
+  const int32_t countBefore = c.count + wave_excl_scan(w, &tot);
   const bool filling = should_fill(rule, countBefore);
-  const bool hasNext = valid && lane < n - 1;
-  const uint64_t kn = __shfl(k, lane + 1 < kWave ? lane + 1 : lane, kWave);
-  const int32_t nextAt = hasNext ? key_at(kn) : 0;
-  const int32_t nextW = hasNext ? key_winding(kn) : 0;
   const bool canSkip = valid && at > 0 && filling && hasNext && nextAt == at && w + nextW == 0;
   uint64_t cm = ballot(canSkip);
-  uint64_t skipMask = 0, consMask = 0;
+  uint64_t skipMask = 0, consMask = c.consumed ? 1ull : 0ull;
+  bool carryOut = false;
   while (cm) {
     const int b = __builtin_ctzll(cm);
     cm &= cm - 1;
     if (!((consMask >> b) & 1ull)) {
       skipMask |= 1ull << b;
       if (b + 1 < kWave) consMask |= 1ull << (b + 1);
+      else carryOut = true;
     }
   }
   const bool consumed = (consMask >> lane) & 1ull;
   const bool skipStart = (skipMask >> lane) & 1ull;
-  int ev = 0;  // 0 none, 1 start, 2 end (yield)
+  // 0 = none, 1 = start (prevAt = at), 2 = end (yield, prevAt = at)
+  int ev = 0;
   if (valid && !consumed && !skipStart && at > 0) {
     if (filling) {
       const bool cont = (rule == kNonZero) && hasNext && (countBefore + w != 0);
@@ -330,8 +288,43 @@ __device__ __forceinline__ void wave_walk_reg(uint64_t k, int n, int32_t rule, S
     }
   }
   const int32_t prevHere = wave_excl_max(ev ? at : INT32_MIN);
-  const int32_t prevAt = prevHere > 0 ? prevHere : 0;  // prevAt starts at 0 (walk 1300)
+  const int32_t prevAt = prevHere > c.prev ? prevHere : c.prev;
   if (ev == 2) span(prevAt, at);
+  const int32_t lastEv = wave_max(ev ? at : INT32_MIN);
+  if (lastEv > c.prev) c.prev = lastEv;
+  c.count += tot;
+  c.consumed = carryOut;
+}
+
+template <bool kPacked = false, class SpanFn>
+__device__ void wave_walk(RowLds& L, int n, int32_t rule, SpanFn& span) {
+  const int lane = lane_id();
+  WalkCarry c;
+  for (int base = 0; base < n; base += kWave) {
+    const int i = base + lane;
+    const bool valid = i < n;
+    const uint64_t k = valid ? L.keys[i] : 0ull;
+    const int32_t at = valid ? key_at(k) : 0;
+    const int32_t w = valid ? (kPacked ? key_winding(k) : L.B[(uint32_t)k]) : 0;
+    const bool hasNext = valid && i < n - 1;
+    const uint64_t kn = hasNext ? L.keys[i + 1] : 0ull;
+    const int32_t nextAt = hasNext ? key_at(kn) : 0;
+    const int32_t nextW = hasNext ? (kPacked ? key_winding(kn) : L.B[(uint32_t)kn]) : 0;
+    walk_chunk(at, w, valid, hasNext, nextAt, nextW, rule, c, span);
+  }
+}
+
+// wave_walk<true> for at most 64 sorted hits held in registers (hit i in
+// lane i, padding keys ~0 beyond n), the next hit taken from the next lane.
+template <class SpanFn>
+__device__ __forceinline__ void wave_walk_reg(uint64_t k, int n, int32_t rule, SpanFn& span) {
+  const int lane = lane_id();
+  const bool valid = lane < n;
+  const bool hasNext = valid && lane < n - 1;
+  const uint64_t kn = __shfl(k, lane + 1 < kWave ? lane + 1 : lane, kWave);
+  WalkCarry c;
+  walk_chunk(valid ? key_at(k) : 0, valid ? key_winding(k) : 0, valid, hasNext,
+             hasNext ? key_at(kn) : 0, hasNext ? key_winding(kn) : 0, rule, c, span);
 }
 
 // One odd-even transposition step over the wave's keys (pairs (2m, 2m+1), or
@@ -687,6 +680,143 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
   }
 }
 
+// Bitonic sort of the 64 * NS keys held in slots k[0..NS) (key s * 64 + lane
+// in slot s of that lane).  Compare-exchanges at strides below 64 move keys
+// between lanes; strides of 64 and more pair two slots of the same lane.
+template <int NS>
+__device__ __forceinline__ void slots_sort(uint64_t (&k)[4]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int size = 2; size <= kWave * NS; size <<= 1) {
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      if (stride >= kWave) {
+        const int ss = stride / kWave;
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          if (s & ss) continue;
+          const bool up = ((s * kWave) & size) == 0;
+          const uint64_t a = k[s], b = k[s | ss];
+          const uint64_t mn = a < b ? a : b, mx = a < b ? b : a;
+          k[s] = up ? mn : mx;
+          k[s | ss] = up ? mx : mn;
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          const uint64_t o = __shfl_xor(k[s], stride, kWave);
+          const bool up = (((s * kWave) | lane) & size) == 0;
+          const bool lower = (lane & stride) == 0;
+          const uint64_t mn = k[s] < o ? k[s] : o, mx = k[s] < o ? o : k[s];
+          k[s] = lower == up ? mn : mx;
+        }
+      }
+    }
+  }
+}
+
+// The 5-sample coverage of an anti-aliased row whose selection holds 65 to
+// kWideMax entries (computeCoverage 1350-1431 with walk 1298-1334), kept out
+// of memory: selection j sits in slot j / 64 of lane j % 64 for all five
+// sample lines, each line's hits are compacted in selection order through
+// the LDS stage L.wide into the same slot layout, sorted there by
+// (x, hit position) -- pixie's stable insertion order -- and walked slot by
+// slot with the walk's carries.  count is the band's entry count (C.E and
+// C.EW start at the band); the selection is read from L.A when it fits
+// there and is otherwise selected again into the stage.
+constexpr int kWideSlots = 4;
+constexpr int kWideMax = kWideSlots * kWave;
+__device__ __forceinline__ void row_coverage_wide(const RowCtx& C, RowLds& L, int count, int ne, float scanTop,
+                                                  float scanBottom, float yLine, float offset, int32_t rule,
+                                                  CovSpan& span) {
+  const int lane = lane_id();
+  const float Wf = (float)C.F->canvasW;
+  const int32_t* selIdx = L.A;
+  if (ne > L.cap) {
+    int32_t* stage = (int32_t*)L.wide;
+    int n = 0;
+    for (int base = 0; base < count; base += kWave) {
+      const int i = base + lane;
+      bool s = false;
+      if (i < count) {
+        const Entry e = C.E[i];
+        s = !(e.toy <= scanTop || e.aty >= scanBottom);
+      }
+      const uint64_t m = ballot(s);
+      if (s) stage[n + __popcll(m & ((1ull << lane) - 1ull))] = i;
+      n += __popcll(m);
+    }
+    wave_sync();
+    selIdx = stage;
+  }
+  Entry e[kWideSlots];
+  int32_t w[kWideSlots];
+  bool v[kWideSlots];
+#pragma unroll
+  for (int s = 0; s < kWideSlots; s++) {
+    const int j = s * kWave + lane;
+    v[s] = j < ne;
+    e[s] = Entry{0.0f, 0.0f, 0.0f, 0.0f};
+    w[s] = 0;
+    if (v[s]) {
+      const int k = selIdx[j];
+      e[s] = C.E[k];
+      w[s] = C.EW[k];
+    }
+  }
+  wave_sync();  // the stage now takes the hit keys
+  uint64_t* stage = L.wide;
+  for (int m = 0; m < 5; m++) {
+    yLine += offset;
+    int nh = 0;
+#pragma unroll
+    for (int s = 0; s < kWideSlots; s++) {
+      bool hit = false;
+      int32_t at = 0;
+      if (v[s] && e[s].aty <= yLine && e[s].toy >= yLine) {
+        const float x = e[s].m == 0.0f ? e[s].b : (yLine - e[s].b) / e[s].m;
+        at = fixed32(nim_minf(x, Wf));
+        hit = true;
+      }
+      const uint64_t mk = ballot(hit);
+      const int pos = nh + __popcll(mk & ((1ull << lane) - 1ull));
+      // (x, position << 8 | winding): the winding rides in the low byte
+      if (hit)
+        stage[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | ((uint32_t)pos << 8) |
+                     (uint32_t)(uint8_t)w[s];
+      nh += __popcll(mk);
+    }
+    wave_sync();
+    uint64_t k[kWideSlots];
+#pragma unroll
+    for (int s = 0; s < kWideSlots; s++) {
+      const int j = s * kWave + lane;
+      k[s] = j < nh ? stage[j] : ~0ull;  // padding sorts last
+    }
+    if (nh <= kWave)
+      slots_sort<1>(k);
+    else if (nh <= 2 * kWave)
+      slots_sort<2>(k);
+    else
+      slots_sort<4>(k);
+    WalkCarry c;
+#pragma unroll
+    for (int s = 0; s < kWideSlots; s++) {
+      if (s * kWave >= nh) break;
+      const int j = s * kWave + lane;
+      const bool valid = j < nh, hasNext = j < nh - 1;
+      uint64_t kn = __shfl(k[s], lane + 1 < kWave ? lane + 1 : lane, kWave);
+      if (s + 1 < kWideSlots) {
+        const uint64_t first = __shfl(k[s + 1], 0, kWave);
+        if (lane == kWave - 1) kn = first;
+      }
+      walk_chunk(valid ? key_at(k[s]) : 0, valid ? key_winding(k[s]) : 0, valid, hasNext,
+                 hasNext ? key_at(kn) : 0, hasNext ? key_winding(kn) : 0, rule, c, span);
+    }
+    wave_sync();
+  }
+}
+
 // Entries row C.y selects (row_program's selection), so that a caller can
 // place the row's arrays before running it: the arrays hold the selection,
 // which is often much smaller than the band.
@@ -784,10 +914,23 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
     }
   }
   wave_sync();
+  // An anti-aliased row of more than 64 selected entries that is not a
+  // trapezoid candidate takes the wide path when its selection fits it.
+  const bool wide = aa && !two && !(allSpan && (ne % 2) == 0) && ne > kWave &&
+                    ne <= (L.wideCap < kWideMax ? L.wideCap : kWideMax);
   // the row's arrays hold the selection (not the whole band); D is untouched
   // so far, so a selection too large for them leaves nothing behind
-  if (ne > L.cap) return -1;
+  if (!wide && ne > L.cap) return -1;
   if (mask && lane == 0) add_run(C, L, 0, W, kClear);  // default for mask rows
+  if (wide) {  // computeCoverage 1350-1431 at quality 5
+    const float offset = 1.0f / 5.0f;
+    const float yLine = (float)y + (offset / 2.0f + 0.00031415926535897936f) - offset;
+    const int64_t covLo = F.startX, covHi = (int64_t)F.startX + F.pathWidth;
+    if (mask && lane == 0) add_run(C, L, covLo, covHi, -kClear);  // coverage range
+    CovSpan cs{&C, &L, 255 / 5, (int32_t)covLo, (int32_t)covHi};
+    row_coverage_wide(Cp, L, P.count, ne, scanTop, scanBottom, yLine, offset, F.rule, cs);
+    return kRowCoverage;
+  }
   const bool inReg = ne <= kWave;
   float sAty = 0.0f, sToy = 0.0f, sM = 0.0f, sB = 0.0f;
   int32_t sW = 0;
