@@ -28,10 +28,10 @@ __global__ void __launch_bounds__(64) fill_rows_kernel(FillArgs a) {
   const size_t fixedBytes = row_lds_fixed_bytes(a.ww);
   char* arr = a.capInLds ? smem + fixedBytes : a.gscratch + (size_t)blockIdx.x * row_cap_bytes(a.cap);
   RowLds L = row_lds_carve(smem, arr, a.cap, a.ww);
-  if (a.capInLds) {  // the wide path stages through the LDS arrays
-    L.wide = L.keys;
-    L.wideCap = 3 * a.cap;
-  }
+  // the wide path stages its keys through AX and, when they are in LDS, the
+  // arrays that follow it
+  L.wide = (uint64_t*)L.AX;
+  L.wideCap = (int)(((size_t)a.ww * 4 + (a.capInLds ? row_cap_bytes(a.cap) : 0)) / 8);
   for (int i = threadIdx.x; i < a.ww + 4; i += kWave) L.D[i] = 0;
   wave_sync();
   const bool mask = F.mode == kMask;

@@ -43,8 +43,9 @@ struct RowLds {
   float* tx0;        // cap: trapezoid x at scanTop (by selection position)
   float* tx1;        // cap: trapezoid x at scanBottom
   int cap, ww;
-  uint64_t* wide;    // LDS staging of the wide path (wideCap keys), or null
-  int wideCap;
+  uint64_t* wide;    // LDS staging of the wide path's hit keys (wideCap of
+  int wideCap;       // them; 0 turns it off): AX, which coverage rows leave
+                     // free, and the cap arrays after it when they are in LDS
 };
 
 PXH_HD size_t row_lds_fixed_bytes(int ww) { return (size_t)(ww + 4) * 4 + (size_t)ww * 4; }
@@ -683,30 +684,36 @@ __device__ __forceinline__ void row_coverage_presorted(const RowCtx& C, RowLds& 
 // Bitonic sort of the 64 * NS keys held in slots k[0..NS) (key s * 64 + lane
 // in slot s of that lane).  Compare-exchanges at strides below 64 move keys
 // between lanes; strides of 64 and more pair two slots of the same lane.
+constexpr int kWideSlots = 4;
+template <int SS>
+__device__ __forceinline__ void slots_exchange(uint64_t (&k)[kWideSlots], int size) {
+#pragma unroll
+  for (int s = 0; s < kWideSlots; s++) {
+    if (s & SS) continue;
+    const bool up = ((s * kWave) & size) == 0;
+    const uint64_t a = k[s], b = k[s | SS];
+    const uint64_t mn = a < b ? a : b, mx = a < b ? b : a;
+    k[s] = up ? mn : mx;
+    k[s | SS] = up ? mx : mn;
+  }
+}
 template <int NS>
-__device__ __forceinline__ void slots_sort(uint64_t (&k)[4]) {
+__device__ __forceinline__ void slots_sort(uint64_t (&k)[kWideSlots]) {
   const int lane = lane_id();
 #pragma unroll
   for (int size = 2; size <= kWave * NS; size <<= 1) {
 #pragma unroll
     for (int stride = size >> 1; stride > 0; stride >>= 1) {
-      if (stride >= kWave) {
-        const int ss = stride / kWave;
-#pragma unroll
-        for (int s = 0; s < NS; s++) {
-          if (s & ss) continue;
-          const bool up = ((s * kWave) & size) == 0;
-          const uint64_t a = k[s], b = k[s | ss];
-          const uint64_t mn = a < b ? a : b, mx = a < b ? b : a;
-          k[s] = up ? mn : mx;
-          k[s | ss] = up ? mx : mn;
-        }
+      if (stride >= 2 * kWave) {
+        slots_exchange<2>(k, size);
+      } else if (stride >= kWave) {
+        slots_exchange<1>(k, size);
       } else {
+        const bool lower = (lane & stride) == 0;
 #pragma unroll
         for (int s = 0; s < NS; s++) {
           const uint64_t o = __shfl_xor(k[s], stride, kWave);
           const bool up = (((s * kWave) | lane) & size) == 0;
-          const bool lower = (lane & stride) == 0;
           const uint64_t mn = k[s] < o ? k[s] : o, mx = k[s] < o ? o : k[s];
           k[s] = lower == up ? mn : mx;
         }
@@ -715,78 +722,118 @@ __device__ __forceinline__ void slots_sort(uint64_t (&k)[4]) {
   }
 }
 
-// The 5-sample coverage of an anti-aliased row whose selection holds 65 to
-// kWideMax entries (computeCoverage 1350-1431 with walk 1298-1334), kept out
-// of memory: selection j sits in slot j / 64 of lane j % 64 for all five
-// sample lines, each line's hits are compacted in selection order through
-// the LDS stage L.wide into the same slot layout, sorted there by
-// (x, hit position) -- pixie's stable insertion order -- and walked slot by
-// slot with the walk's carries.  count is the band's entry count (C.E and
-// C.EW start at the band); the selection is read from L.A when it fits
-// there and is otherwise selected again into the stage.
-constexpr int kWideSlots = 4;
-constexpr int kWideMax = kWideSlots * kWave;
+// The 5-sample coverage of an anti-aliased row that selects 65 to
+// kWideMax entries (computeCoverage 1350-1431 with walk 1298-1334), with no
+// per-row arrays in memory.  A selection of at most 256 entries is held in
+// registers (selection j in slot j / 64 of lane j % 64); a larger one is
+// read from the band for every sample line (C.E / C.EW start at the band,
+// count entries; a hit implies the selection test, which is repeated so
+// that the two agree at any y).  Either way each line's hits are compacted
+// in selection order, their keys (x, hit position << 8 | winding) staged in
+// L.wide, and sorted, which gives pixie's stable insertion order: up to 256
+// hits in four 64-key register slots, walked slot by slot with the walk's
+// carries; more in the stage itself (LDS), whose kWideMax keys hold the
+// sort's padding.  The selection indices are in L.A when ne <= L.cap.
+constexpr int kWideMax = 8 * kWave;
+__device__ __forceinline__ int wide_limit(const RowLds& L) {
+  return L.wideCap >= kWideMax ? kWideMax : (L.wideCap < kWideSlots * kWave ? L.wideCap : kWideSlots * kWave);
+}
 __device__ __forceinline__ void row_coverage_wide(const RowCtx& C, RowLds& L, int count, int ne, float scanTop,
                                                   float scanBottom, float yLine, float offset, int32_t rule,
                                                   CovSpan& span) {
   const int lane = lane_id();
   const float Wf = (float)C.F->canvasW;
-  const int32_t* selIdx = L.A;
-  if (ne > L.cap) {
-    int32_t* stage = (int32_t*)L.wide;
-    int n = 0;
-    for (int base = 0; base < count; base += kWave) {
-      const int i = base + lane;
-      bool s = false;
-      if (i < count) {
-        const Entry e = C.E[i];
-        s = !(e.toy <= scanTop || e.aty >= scanBottom);
-      }
-      const uint64_t m = ballot(s);
-      if (s) stage[n + __popcll(m & ((1ull << lane) - 1ull))] = i;
-      n += __popcll(m);
-    }
-    wave_sync();
-    selIdx = stage;
-  }
+  uint64_t* stage = L.wide;
+  const bool inReg = ne <= kWideSlots * kWave;
   Entry e[kWideSlots];
   int32_t w[kWideSlots];
-  bool v[kWideSlots];
 #pragma unroll
   for (int s = 0; s < kWideSlots; s++) {
-    const int j = s * kWave + lane;
-    v[s] = j < ne;
     e[s] = Entry{0.0f, 0.0f, 0.0f, 0.0f};
     w[s] = 0;
-    if (v[s]) {
-      const int k = selIdx[j];
-      e[s] = C.E[k];
-      w[s] = C.EW[k];
-    }
   }
-  wave_sync();  // the stage now takes the hit keys
-  uint64_t* stage = L.wide;
+  if (inReg) {
+    const int32_t* selIdx = L.A;
+    if (ne > L.cap) {  // select again, into the stage
+      int32_t* idx = (int32_t*)stage;
+      int n = 0;
+      for (int base = 0; base < count; base += kWave) {
+        const int i = base + lane;
+        bool sel = false;
+        if (i < count) {
+          const Entry ei = C.E[i];
+          sel = !(ei.toy <= scanTop || ei.aty >= scanBottom);
+        }
+        const uint64_t m = ballot(sel);
+        if (sel) idx[n + __popcll(m & ((1ull << lane) - 1ull))] = i;
+        n += __popcll(m);
+      }
+      wave_sync();
+      selIdx = idx;
+    }
+#pragma unroll
+    for (int s = 0; s < kWideSlots; s++) {
+      const int j = s * kWave + lane;
+      if (j < ne) {
+        const int k = selIdx[j];
+        e[s] = C.E[k];
+        w[s] = C.EW[k];
+      }
+    }
+    wave_sync();  // the stage now takes the hit keys
+  }
   for (int m = 0; m < 5; m++) {
     yLine += offset;
     int nh = 0;
+    if (inReg) {
 #pragma unroll
-    for (int s = 0; s < kWideSlots; s++) {
-      bool hit = false;
-      int32_t at = 0;
-      if (v[s] && e[s].aty <= yLine && e[s].toy >= yLine) {
-        const float x = e[s].m == 0.0f ? e[s].b : (yLine - e[s].b) / e[s].m;
-        at = fixed32(nim_minf(x, Wf));
-        hit = true;
+      for (int s = 0; s < kWideSlots; s++) {
+        bool hit = false;
+        int32_t at = 0;
+        if (s * kWave + lane < ne && e[s].aty <= yLine && e[s].toy >= yLine) {
+          const float x = e[s].m == 0.0f ? e[s].b : (yLine - e[s].b) / e[s].m;
+          at = fixed32(nim_minf(x, Wf));
+          hit = true;
+        }
+        const uint64_t mk = ballot(hit);
+        const int pos = nh + __popcll(mk & ((1ull << lane) - 1ull));
+        if (hit)
+          stage[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | ((uint32_t)pos << 8) |
+                       (uint32_t)(uint8_t)w[s];
+        nh += __popcll(mk);
       }
-      const uint64_t mk = ballot(hit);
-      const int pos = nh + __popcll(mk & ((1ull << lane) - 1ull));
-      // (x, position << 8 | winding): the winding rides in the low byte
-      if (hit)
-        stage[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | ((uint32_t)pos << 8) |
-                     (uint32_t)(uint8_t)w[s];
-      nh += __popcll(mk);
+    } else {
+#pragma unroll 2
+      for (int base = 0; base < count; base += kWave) {
+        const int i = base + lane;
+        bool hit = false;
+        int32_t at = 0, wi = 0;
+        if (i < count) {
+          const Entry ei = C.E[i];
+          if (!(ei.toy <= scanTop || ei.aty >= scanBottom) && ei.aty <= yLine && ei.toy >= yLine) {
+            const float x = ei.m == 0.0f ? ei.b : (yLine - ei.b) / ei.m;
+            at = fixed32(nim_minf(x, Wf));
+            wi = C.EW[i];
+            hit = true;
+          }
+        }
+        const uint64_t mk = ballot(hit);
+        const int pos = nh + __popcll(mk & ((1ull << lane) - 1ull));
+        if (hit)
+          stage[pos] = ((uint64_t)((uint32_t)at ^ 0x80000000u) << 32) | ((uint32_t)pos << 8) |
+                       (uint32_t)(uint8_t)wi;
+        nh += __popcll(mk);
+      }
     }
     wave_sync();
+    if (nh > kWideSlots * kWave) {
+      RowLds Ls = L;
+      Ls.keys = stage;
+      wave_sort_keys(stage, nh, kWideMax);
+      wave_walk<true>(Ls, nh, rule, span);
+      wave_sync();
+      continue;
+    }
     uint64_t k[kWideSlots];
 #pragma unroll
     for (int s = 0; s < kWideSlots; s++) {
@@ -916,8 +963,7 @@ __device__ int32_t row_program(const RowCtx& C, RowLds& L) {
   wave_sync();
   // An anti-aliased row of more than 64 selected entries that is not a
   // trapezoid candidate takes the wide path when its selection fits it.
-  const bool wide = aa && !two && !(allSpan && (ne % 2) == 0) && ne > kWave &&
-                    ne <= (L.wideCap < kWideMax ? L.wideCap : kWideMax);
+  const bool wide = aa && !two && !(allSpan && (ne % 2) == 0) && ne > kWave && ne <= wide_limit(L);
   // the row's arrays hold the selection (not the whole band); D is untouched
   // so far, so a selection too large for them leaves nothing behind
   if (!wide && ne > L.cap) return -1;

@@ -1760,9 +1760,9 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   const size_t fixedBytes = row_lds_fixed_bytes(kWinA);
   RowLds Ll = row_lds_carve(smem, smem + fixedBytes, P.ldsCap, kWinA);
   RowLds Lg = row_lds_carve(smem, P.gscratch + (size_t)blockIdx.x * row_cap_bytes(P.gcap), P.gcap, kWinA);
-  // the wide path stages through the LDS arrays (row_cap_bytes(ldsCap) / 8 keys)
-  Ll.wide = Lg.wide = Ll.keys;
-  Ll.wideCap = Lg.wideCap = 3 * P.ldsCap;
+  // the wide path stages its keys through AX and the LDS arrays, which follow it
+  Ll.wide = Lg.wide = (uint64_t*)Ll.AX;
+  Ll.wideCap = Lg.wideCap = (int)((kWinA * 4 + row_cap_bytes(P.ldsCap)) / 8);
   for (int i = lane; i < kWinA + 4; i += 64) Ll.D[i] = 0;
   __syncthreads();
   if (P.counters[kCntError]) return;
