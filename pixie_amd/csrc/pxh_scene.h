@@ -99,6 +99,8 @@ constexpr int32_t kRunXMask = (1 << 28) - 1;
 // kMaxTaskRuns written entries.  Whatever order the waves take tasks in, a
 // render of U written entries takes at most U / (kRunChunk - kMaxTaskRuns)
 // left chunks plus one open chunk per wave (scene_run_bound).
+constexpr int kRowGroup = 8;       // rows of a grouped phase-A work item
+constexpr int kGroupMaxClass = 7;  // grouped rows' bands hold fewer than 128 entries
 constexpr int kRunChunk = 2048;
 constexpr int kMaxTaskRuns = 1025;
 inline int64_t scene_run_bound(int64_t used, int64_t waves) {
@@ -115,12 +117,21 @@ enum SceneCounter {
   kCntRuns,      // run-pool entries handed out in chunks
   kCntRunsUsed,  // run entries written
   kCntError, kCntMaxBand, kCntBinEntries,
-  kCntBigTasks,  // phase-A tasks of the large-band launch (the first in task order)
+  kCntBigTasks,  // phase-A work items of the large-band launch (the first in their order)
+  kCntGroups,    // phase-A work items (set by the class scan)
   kNumCounters
 };
 
-// Phase-A tasks run longest first: a task's cost class is the bit length of
-// its band's entry count, and the tasks are ordered by class, descending.
+// A phase-A task is one (call, row, window).  A wave takes a work item:
+// kRowGroup consecutive rows of one call and window when the bands of those
+// rows hold at most kGroupMaxClass-bit entry counts (the call's records are
+// then read once and the rows' band entries mostly come from the L1 cache),
+// or else a single row, so that no item is long.  Items run longest first:
+// an item's cost class is the bit length of the largest band entry count
+// among its rows, and the items are ordered by class, descending.  Work
+// items are named by the task of their first row; the task map marks
+// grouped items (taskItem bit 31) and gives the other rows of a group no
+// class (-1).
 // The class histogram and the scatter cursors follow the counters.
 constexpr int kCostClasses = 32;
 constexpr int kCntClassHist = kNumCounters;                  // kCostClasses ints
@@ -197,9 +208,9 @@ struct ScenePools {
   int8_t* entW;
   int32_t entryCap;
   RowRuns* tasks;
-  int32_t* taskItem;         // call of each task
-  int32_t* taskOrder;        // tasks in the order phase A takes them (longest first)
-  int8_t* taskClass;         // cost class of each task
+  int32_t* taskItem;         // call of each work item
+  int32_t* taskOrder;        // work items in the order phase A takes them (longest first)
+  int8_t* taskClass;         // cost class of each work item
   int32_t taskCap;
   int2* runs;
   int32_t runCap;

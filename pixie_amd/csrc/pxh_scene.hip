@@ -1647,12 +1647,11 @@ __device__ __forceinline__ int queue_take(int32_t* counter, int n) {
   return __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
 }
 
-// Call of every phase-A task (one wave per call writes its task range).
-// Cost class of task t (0 .. nt-1) of a call: the bit length of the entry
-// count of its row's band (0 for rows outside the path).
-__device__ __forceinline__ int task_class(const ScenePools& P, const ItemGeom& g, const FillInfo& F,
-                                          int t) {
-  const int y = g.rowStart + t / g.nWin;
+// Call of every phase-A work item (one workgroup per call writes its
+// range).  Cost class of row r (0 .. rowCount-1) of a call: the bit length
+// of the entry count of its band (0 for rows outside the path).
+__device__ __forceinline__ int row_class(const ScenePools& P, const ItemGeom& g, const FillInfo& F, int r) {
+  const int y = g.rowStart + r;
   if (y < F.startY || y >= F.pathHeight || F.nParts <= 0 || F.bandH <= 0) return 0;
   int pi = (y - F.startY) / F.bandH;
   if (pi > F.nParts - 1) pi = F.nParts - 1;
@@ -1660,9 +1659,9 @@ __device__ __forceinline__ int task_class(const ScenePools& P, const ItemGeom& g
   return cnt <= 0 ? 0 : 32 - __clz(cnt);
 }
 
-// Maps tasks to calls, and records and counts (per block in LDS, then one
-// global atomic per class and block) the cost class of every task.  The
-// classes of a call's bands are staged in LDS first.
+// Maps work items to calls and records their cost classes, counting them
+// per class (per block in LDS, then one global atomic per class and block).
+// The classes of a call's bands are staged in LDS first.
 constexpr int kLdsBandClasses = 4096;
 constexpr int kTaskMapBlock = 1024;
 __global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePools P) {
@@ -1687,23 +1686,34 @@ __global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePool
     for (int t0 = 0; t0 < nt; t0 += kTaskMapBlock) {
       const int t = t0 + tid;
       const bool on = t < nt;
-      int c = 0;
+      int c = -1;
+      bool grouped = false;
       if (on) {
-        const int y = g.rowStart + t / g.nWin;
-        if (banded && y >= F.startY && y < F.pathHeight) {
-          int pi = (y - F.startY) / F.bandH;
-          if (pi > nb - 1) pi = nb - 1;
-          c = pi < kLdsBandClasses ? cls[pi] : task_class(P, g, F, t);
+        // classes of the rows of this task's group
+        const int r = t / g.nWin, rg = r - r % kRowGroup;
+        int own = 0, chunk = 0;
+        for (int q = rg; q < rg + kRowGroup && q < g.rowCount; q++) {
+          const int y = g.rowStart + q;
+          int rc = 0;
+          if (banded && y >= F.startY && y < F.pathHeight) {
+            int pi = (y - F.startY) / F.bandH;
+            if (pi > nb - 1) pi = nb - 1;
+            rc = pi < kLdsBandClasses ? cls[pi] : row_class(P, g, F, q);
+          }
+          if (q == r) own = rc;
+          chunk = rc > chunk ? rc : chunk;
         }
-        P.taskItem[g.taskBase + t] = it;
+        grouped = chunk <= kGroupMaxClass;
+        c = !grouped ? own : (r == rg ? chunk : -1);
+        P.taskItem[g.taskBase + t] = grouped ? (int32_t)((uint32_t)it | 0x80000000u) : it;
         P.taskClass[g.taskBase + t] = (int8_t)c;
       }
       // one LDS atomic per class present in the wave
-      uint64_t pending = __ballot(on);
+      uint64_t pending = __ballot(c >= 0);
       while (pending) {
         const int leader = __builtin_ctzll(pending);
         const int cl = __builtin_amdgcn_readlane(c, leader);
-        const uint64_t m = __ballot(on && c == cl);
+        const uint64_t m = __ballot(c == cl);
         if (lane == leader) atomicAdd(&hist[cl], (int)__popcll(m));
         pending &= ~m;
       }
@@ -1714,15 +1724,15 @@ __global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePool
   if (tid < kCostClasses && hist[tid]) atomicAdd(&P.counters[kCntClassHist + tid], hist[tid]);
 }
 
-// Block b's contiguous share of the tasks.
+// Block b's contiguous share of the work items.
 __device__ __forceinline__ int2 task_share(int ntasks) {
   const int per = ((ntasks + (int)gridDim.x - 1) / (int)gridDim.x + 63) & ~63;
   const int lo = (int)blockIdx.x * per;
   return make_int2(lo, lo + per < ntasks ? lo + per : ntasks);
 }
 
-// Start of each cost class in the task order, largest class first, and the
-// number of tasks in classes >= bigClass (the large-band launch's share).
+// Start of each cost class in the work-item order, largest class first, and
+// the number of items in classes >= bigClass (the large-band launch's share).
 __global__ void scene_task_scan_kernel(int32_t* counters, int bigClass) {
   if (threadIdx.x != 0) return;
   int off = 0;
@@ -1732,10 +1742,11 @@ __global__ void scene_task_scan_kernel(int32_t* counters, int bigClass) {
     off += counters[kCntClassHist + c];
   }
   if (bigClass <= 0) counters[kCntBigTasks] = off;
+  counters[kCntGroups] = off;
 }
 
-// Places every task in the task order by its cost class: each block
-// reserves a range of every class for its share of the tasks, then deals it
+// Places every work item in the order by its cost class: each block
+// reserves a range of every class for its share of the items, then deals it
 // out with LDS atomics.
 __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   __shared__ int32_t cnt[kCostClasses], cur[kCostClasses];
@@ -1744,11 +1755,13 @@ __global__ void __launch_bounds__(64) scene_task_order_kernel(ScenePools P) {
   if (lane < kCostClasses) cnt[lane] = 0;
   __syncthreads();
   const int2 r = task_share(P.counters[kCntTasks]);
-  for (int t = r.x + lane; t < r.y; t += 64) atomicAdd(&cnt[P.taskClass[t]], 1);
+  for (int t = r.x + lane; t < r.y; t += 64)
+    if (P.taskClass[t] >= 0) atomicAdd(&cnt[P.taskClass[t]], 1);
   __syncthreads();
   if (lane < kCostClasses) cur[lane] = cnt[lane] ? atomicAdd(&P.counters[kCntClassCursor + lane], cnt[lane]) : 0;
   __syncthreads();
-  for (int t = r.x + lane; t < r.y; t += 64) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
+  for (int t = r.x + lane; t < r.y; t += 64)
+    if (P.taskClass[t] >= 0) P.taskOrder[atomicAdd(&cur[P.taskClass[t]], 1)] = t;
 }
 
 // kBig: the large-band launch (tasks [0, nBig) of the task order), else the
@@ -1766,10 +1779,10 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   for (int i = lane; i < kWinA + 4; i += 64) Ll.D[i] = 0;
   __syncthreads();
   if (P.counters[kCntError]) return;
-  // this launch's share of the task order: the large-band tasks come first
+  // this launch's share of the work-item order: the large-band items come first
   const int nBig = P.counters[kCntBigTasks];
   const int tBegin = kBig ? 0 : nBig;
-  const int tEnd = kBig ? nBig : P.counters[kCntTasks];
+  const int tEnd = kBig ? nBig : P.counters[kCntGroups];
   int32_t* heads = P.queues + (kBig ? 0 : kQueues * kQueueStride);
   // Each wave appends its runs to chunks of the run pool it takes one at a
   // time (one atomic per chunk, not per task).
@@ -1777,16 +1790,18 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   for (;;) {
     const int q = tBegin + queue_next(heads, tEnd - tBegin);
     if (q >= tEnd) break;
-    // Task, call and the call's records are the same in every lane: read
-    // them into scalar registers.
-    const int task = uni(P.taskOrder[q]);
-    const int it = uni(P.taskItem[task]);
+    // Work item, call and the call's records are the same in every lane:
+    // read them into scalar registers.
+    const int first = uni(P.taskOrder[q]);
+    const int itw = uni(P.taskItem[first]);
+    const int it = itw & 0x7fffffff;
     ItemGeom g = P.geo[it];
     g.rowStart = uni(g.rowStart);
     g.nWin = uni(g.nWin);
     g.winBase = uni(g.winBase);
     g.taskBase = uni(g.taskBase);
     g.bandBase = uni(g.bandBase);
+    g.rowCount = uni(g.rowCount);
     FillInfo Fu = P.fills[it];
     Fu.startX = uni(Fu.startX);
     Fu.startY = uni(Fu.startY);
@@ -1800,109 +1815,114 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
     Fu.xhi = uni(Fu.xhi);
     Fu.canvasW = uni(Fu.canvasW);
     const FillInfo& F = Fu;
-    const int local = task - g.taskBase;
-    const int y = g.rowStart + local / g.nWin;
+    const int local = first - g.taskBase;
     const int win = local % g.nWin;
+    const int r0 = local / g.nWin;
+    const int r1 = itw >= 0 ? r0 + 1 : (r0 + kRowGroup < g.rowCount ? r0 + kRowGroup : g.rowCount);
     const bool mask = F.mode == kMask;
-    RowCtx C;
-    C.F = &Fu;
-    C.parts = P.parts + g.bandBase;
-    C.E = P.entries;
-    C.EW = P.entW;
-    C.y = y;
-    C.cx0 = mask ? 0 : F.xlo;
-    C.cx1 = mask ? P.W : F.xhi;
-    C.wx0 = g.winBase + win * kWinA;
-    C.wx1 = C.wx0 + kWinA < C.cx1 ? C.wx0 + kWinA : C.cx1;
-    // band of this row decides where the row scratch lives
-    int pi = (y - F.startY) / F.bandH;
-    if (pi > F.nParts - 1) pi = F.nParts - 1;
-    const int cnt = uni(C.parts[pi].count);
-    // a large band's row whose selection fits LDS still runs there
-    bool inLds = cnt <= P.ldsCap;
-    if (!inLds && cnt <= P.gcap) inLds = row_selection_count(C) <= P.ldsCap;
-    RowLds& L = inLds ? Ll : Lg;
-    int32_t kind = (inLds || cnt <= P.gcap) ? row_program(C, L) : -1;
-    RowRuns rr;
-    rr.off = 0;
-    rr.n = 0;
-    rr.kind = kind;
-    rr.pad = 0;
-    if (kind < 0) {
-      atomicOr(&P.counters[kCntError], 8);
-    } else if (kind != kRowNone) {
-      // compact D[0..wlen] (nonzero deltas) and, for ops rows, the area
-      // factors of the pixels whose code is an area code.  Coverage rows
-      // need no codes, only the nonzero deltas.  Pass 0 counts (ballots),
-      // pass 1 writes.  D has ww + 4 entries and is zero beyond wlen, so
-      // a lane with i0 <= wlen moves its 4 entries as one 16-byte access.
-      const int wlen = C.wx1 - C.wx0;
-      const bool ops = kind == kRowOps;
-      int total = 0;
-      for (int pass = 0; pass < 2; pass++) {
-        int carry = 0, written = 0;
-        for (int base = 0; base <= wlen; base += 4 * kWave) {
-          const int i0 = base + 4 * lane;
-          int4 dv = make_int4(0, 0, 0, 0);
-          if (i0 <= wlen) dv = *(const int4*)(L.D + i0);
-          const int32_t d[4] = {dv.x, dv.y, dv.z, dv.w};
-          bool area[4] = {false, false, false, false};
-          if (ops) {
-            const int32_t c0 = d[0], c1 = c0 + d[1], c2 = c1 + d[2], c3 = c2 + d[3];
-            int32_t tot;
-            const int32_t off = carry + wave_excl_scan(c3, &tot);
-            carry += tot;
-            const int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
-#pragma unroll
-            for (int q = 0; q < 4; q++) area[q] = i0 + q < wlen && code[q] >= kArea && code[q] < kClear;
+    for (int r = r0; r < r1; r++) {
+      const int y = g.rowStart + r;
+      const int task = g.taskBase + r * g.nWin + win;
+      RowCtx C;
+      C.F = &Fu;
+      C.parts = P.parts + g.bandBase;
+      C.E = P.entries;
+      C.EW = P.entW;
+      C.y = y;
+      C.cx0 = mask ? 0 : F.xlo;
+      C.cx1 = mask ? P.W : F.xhi;
+      C.wx0 = g.winBase + win * kWinA;
+      C.wx1 = C.wx0 + kWinA < C.cx1 ? C.wx0 + kWinA : C.cx1;
+      // band of this row decides where the row scratch lives
+      int pi = (y - F.startY) / F.bandH;
+      if (pi > F.nParts - 1) pi = F.nParts - 1;
+      const int cnt = uni(C.parts[pi].count);
+      // a large band's row whose selection fits LDS still runs there
+      bool inLds = cnt <= P.ldsCap;
+      if (!inLds && cnt <= P.gcap) inLds = row_selection_count(C) <= P.ldsCap;
+      RowLds& L = inLds ? Ll : Lg;
+      int32_t kind = (inLds || cnt <= P.gcap) ? row_program(C, L) : -1;
+      RowRuns rr;
+      rr.off = 0;
+      rr.n = 0;
+      rr.kind = kind;
+      rr.pad = 0;
+      if (kind < 0) {
+        atomicOr(&P.counters[kCntError], 8);
+      } else if (kind != kRowNone) {
+        // compact D[0..wlen] (nonzero deltas) and, for ops rows, the area
+        // factors of the pixels whose code is an area code.  Coverage rows
+        // need no codes, only the nonzero deltas.  Pass 0 counts (ballots),
+        // pass 1 writes.  D has ww + 4 entries and is zero beyond wlen, so
+        // a lane with i0 <= wlen moves its 4 entries as one 16-byte access.
+        const int wlen = C.wx1 - C.wx0;
+        const bool ops = kind == kRowOps;
+        int total = 0;
+        for (int pass = 0; pass < 2; pass++) {
+          int carry = 0, written = 0;
+          for (int base = 0; base <= wlen; base += 4 * kWave) {
+            const int i0 = base + 4 * lane;
+            int4 dv = make_int4(0, 0, 0, 0);
+            if (i0 <= wlen) dv = *(const int4*)(L.D + i0);
+            const int32_t d[4] = {dv.x, dv.y, dv.z, dv.w};
+            bool area[4] = {false, false, false, false};
+            if (ops) {
+              const int32_t c0 = d[0], c1 = c0 + d[1], c2 = c1 + d[2], c3 = c2 + d[3];
+              int32_t tot;
+              const int32_t off = carry + wave_excl_scan(c3, &tot);
+              carry += tot;
+              const int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
+  #pragma unroll
+              for (int q = 0; q < 4; q++) area[q] = i0 + q < wlen && code[q] >= kArea && code[q] < kClear;
+            }
+            if (pass == 0) {
+  #pragma unroll
+              for (int q = 0; q < 4; q++) {
+                written += (int)__popcll(__ballot(d[q] != 0));
+                if (ops) written += (int)__popcll(__ballot(area[q]));
+              }
+              continue;
+            }
+            int n = 0;
+  #pragma unroll
+            for (int q = 0; q < 4; q++) n += (d[q] != 0 ? 1 : 0) + (area[q] ? 1 : 0);
+            int ntot;
+            const int p = wave_excl_scan(n, &ntot);
+            if (i0 <= wlen) {
+              int2* out = P.runs + rr.off + written + p;
+              int w = 0;
+  #pragma unroll
+              for (int q = 0; q < 4; q++) {
+                const int x = C.wx0 + i0 + q;
+                if (d[q] != 0) out[w++] = make_int2(x, d[q]);
+                if (area[q]) out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
+              }
+              *(int4*)(L.D + i0) = make_int4(0, 0, 0, 0);
+            }
+            written += ntot;
           }
           if (pass == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              written += (int)__popcll(__ballot(d[q] != 0));
-              if (ops) written += (int)__popcll(__ballot(area[q]));
+            total = written;
+            if (cursor + total > chunkEnd) {
+              cursor = queue_take(&P.counters[kCntRuns], kRunChunk);
+              chunkEnd = cursor + kRunChunk;
             }
-            continue;
-          }
-          int n = 0;
-#pragma unroll
-          for (int q = 0; q < 4; q++) n += (d[q] != 0 ? 1 : 0) + (area[q] ? 1 : 0);
-          int ntot;
-          const int p = wave_excl_scan(n, &ntot);
-          if (i0 <= wlen) {
-            int2* out = P.runs + rr.off + written + p;
-            int w = 0;
-#pragma unroll
-            for (int q = 0; q < 4; q++) {
-              const int x = C.wx0 + i0 + q;
-              if (d[q] != 0) out[w++] = make_int2(x, d[q]);
-              if (area[q]) out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
+            rr.off = cursor;
+            if (chunkEnd > P.runCap) {
+              atomicOr(&P.counters[kCntError], 16);
+              for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
+              total = 0;
+              break;
             }
-            *(int4*)(L.D + i0) = make_int4(0, 0, 0, 0);
-          }
-          written += ntot;
-        }
-        if (pass == 0) {
-          total = written;
-          if (cursor + total > chunkEnd) {
-            cursor = queue_take(&P.counters[kCntRuns], kRunChunk);
-            chunkEnd = cursor + kRunChunk;
-          }
-          rr.off = cursor;
-          if (chunkEnd > P.runCap) {
-            atomicOr(&P.counters[kCntError], 16);
-            for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
-            total = 0;
-            break;
           }
         }
+        rr.n = total;
+        cursor += total;
+        used += total;
+        __syncthreads();
       }
-      rr.n = total;
-      cursor += total;
-      used += total;
-      __syncthreads();
+      P.tasks[task] = rr;  // every lane stores the same value
     }
-    P.tasks[task] = rr;  // every lane stores the same value
   }
   if (threadIdx.x == 0 && used) atomicAdd(&P.counters[kCntRunsUsed], used);
 }
