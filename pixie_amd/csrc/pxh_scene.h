@@ -94,9 +94,9 @@ constexpr int32_t kRunXMask = (1 << 28) - 1;
 
 // Phase-A waves take run-pool space in chunks of kRunChunk entries.  One
 // task of a 512-px window writes at most kMaxTaskRuns entries (513 deltas +
-// 512 area factors), and a wave leaves a chunk only when the next task does
-// not fit, so every chunk a wave leaves holds more than kRunChunk -
-// kMaxTaskRuns written entries.  Whatever order the waves take tasks in, a
+// 512 area factors), and a wave leaves a chunk only when the most the next
+// task can write does not fit, so every chunk a wave leaves holds more than
+// kRunChunk - kMaxTaskRuns written entries.  Whatever order the waves take tasks in, a
 // render of U written entries takes at most U / (kRunChunk - kMaxTaskRuns)
 // left chunks plus one open chunk per wave (scene_run_bound).
 constexpr int kRowGroup = 8;       // rows of a grouped phase-A work item

@@ -1851,19 +1851,30 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
         atomicOr(&P.counters[kCntError], 8);
       } else if (kind != kRowNone) {
         // compact D[0..wlen] (nonzero deltas) and, for ops rows, the area
-        // factors of the pixels whose code is an area code.  Coverage rows
-        // need no codes, only the nonzero deltas.  Pass 0 counts (ballots),
-        // pass 1 writes.  D has ww + 4 entries and is zero beyond wlen, so
-        // a lane with i0 <= wlen moves its 4 entries as one 16-byte access.
+        // factors of the pixels whose code is an area code, in one pass.
+        // Coverage rows need no codes, only the nonzero deltas.  The wave's
+        // run chunk is first made to hold the most the row can write (wlen + 1
+        // deltas, and wlen area factors on ops rows).  D has ww + 4 entries
+        // and is zero beyond wlen, so a lane with i0 <= wlen moves its 4
+        // entries as one 16-byte access.
         const int wlen = C.wx1 - C.wx0;
         const bool ops = kind == kRowOps;
-        int total = 0;
-        for (int pass = 0; pass < 2; pass++) {
-          int carry = 0, written = 0;
+        const int most = ops ? 2 * wlen + 1 : wlen + 1;
+        if (cursor + most > chunkEnd) {
+          cursor = queue_take(&P.counters[kCntRuns], kRunChunk);
+          chunkEnd = cursor + kRunChunk;
+        }
+        rr.off = cursor;
+        int written = 0;
+        if (chunkEnd > P.runCap) {
+          atomicOr(&P.counters[kCntError], 16);
+          for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
+        } else {
+          int carry = 0;
           for (int base = 0; base <= wlen; base += 4 * kWave) {
             const int i0 = base + 4 * lane;
             int4 dv = make_int4(0, 0, 0, 0);
-            if (i0 <= wlen) dv = *(const int4*)(L.D + i0);
+            if (i0 <= wlen) dv = *(const int4*)(Ll.D + i0);
             const int32_t d[4] = {dv.x, dv.y, dv.z, dv.w};
             bool area[4] = {false, false, false, false};
             if (ops) {
@@ -1872,53 +1883,31 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
               const int32_t off = carry + wave_excl_scan(c3, &tot);
               carry += tot;
               const int32_t code[4] = {off + c0, off + c1, off + c2, off + c3};
-  #pragma unroll
+#pragma unroll
               for (int q = 0; q < 4; q++) area[q] = i0 + q < wlen && code[q] >= kArea && code[q] < kClear;
             }
-            if (pass == 0) {
-  #pragma unroll
-              for (int q = 0; q < 4; q++) {
-                written += (int)__popcll(__ballot(d[q] != 0));
-                if (ops) written += (int)__popcll(__ballot(area[q]));
-              }
-              continue;
-            }
             int n = 0;
-  #pragma unroll
+#pragma unroll
             for (int q = 0; q < 4; q++) n += (d[q] != 0 ? 1 : 0) + (area[q] ? 1 : 0);
             int ntot;
             const int p = wave_excl_scan(n, &ntot);
             if (i0 <= wlen) {
               int2* out = P.runs + rr.off + written + p;
               int w = 0;
-  #pragma unroll
+#pragma unroll
               for (int q = 0; q < 4; q++) {
                 const int x = C.wx0 + i0 + q;
                 if (d[q] != 0) out[w++] = make_int2(x, d[q]);
                 if (area[q]) out[w++] = make_int2(x | kRunArea, (int32_t)L.AX[i0 + q]);
               }
-              *(int4*)(L.D + i0) = make_int4(0, 0, 0, 0);
+              *(int4*)(Ll.D + i0) = make_int4(0, 0, 0, 0);
             }
             written += ntot;
           }
-          if (pass == 0) {
-            total = written;
-            if (cursor + total > chunkEnd) {
-              cursor = queue_take(&P.counters[kCntRuns], kRunChunk);
-              chunkEnd = cursor + kRunChunk;
-            }
-            rr.off = cursor;
-            if (chunkEnd > P.runCap) {
-              atomicOr(&P.counters[kCntError], 16);
-              for (int i = lane; i <= wlen; i += 64) L.D[i] = 0;
-              total = 0;
-              break;
-            }
-          }
         }
-        rr.n = total;
-        cursor += total;
-        used += total;
+        rr.n = written;
+        cursor += written;
+        used += written;
         __syncthreads();
       }
       P.tasks[task] = rr;  // every lane stores the same value
