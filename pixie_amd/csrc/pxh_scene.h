@@ -100,6 +100,7 @@ constexpr int32_t kRunXMask = (1 << 28) - 1;
 // render of U written entries takes at most U / (kRunChunk - kMaxTaskRuns)
 // left chunks plus one open chunk per wave (scene_run_bound).
 constexpr int kRowGroup = 8;       // rows of a grouped phase-A work item
+constexpr int kRowGranule = 32;    // phase-A work items a queue takes in a row
 constexpr int kGroupMaxClass = 7;  // grouped rows' bands hold fewer than 128 entries
 constexpr int kRunChunk = 2048;
 constexpr int kMaxTaskRuns = 1025;

@@ -1281,12 +1281,29 @@ __device__ __forceinline__ void band_rows(const FillInfo& f, int k, float* top, 
 }
 
 
+// A grid-stride loop over items [0, n) laid out by XCD: workgroups go to
+// the kXcds XCDs round robin, and XCD x takes the x-th contiguous share of
+// the items, so that neighbouring items (the bands of one call, which read
+// the same segments) share its L2.
+constexpr int kXcds = 8;
+struct XcdSpan {
+  int begin, end, step;
+};
+__device__ __forceinline__ XcdSpan xcd_span(int n) {
+  const int per = (int)gridDim.x / kXcds;
+  if (per == 0) return XcdSpan{(int)blockIdx.x, n, (int)gridDim.x};
+  const int xcd = (int)blockIdx.x % kXcds, li = (int)blockIdx.x / kXcds;
+  if (li >= per) return XcdSpan{0, 0, 1};  // beyond the last full round of XCDs
+  return XcdSpan{(int)((int64_t)n * xcd / kXcds) + li, (int)((int64_t)n * (xcd + 1) / kXcds), per};
+}
+
 // One wave per band: entry count and requiresAntiAliasing.
 __global__ void __launch_bounds__(64) scene_band_count_kernel(ScenePools P) {
   const int lane = threadIdx.x;
   const int nb = P.counters[kCntBands];
   if (nb > P.bandCap) return;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+  const XcdSpan sp = xcd_span(nb);
+  for (int b = sp.begin; b < sp.end; b += sp.step) {
     const int it = uni(P.bandItem[b]);
     ItemGeom g = P.geo[it];
     g.bandBase = uni(g.bandBase);
@@ -1431,7 +1448,8 @@ __global__ void __launch_bounds__(64) scene_band_write_kernel(ScenePools P) {
   const int lane = threadIdx.x;
   const int nb = P.counters[kCntBands];
   if (nb > P.bandCap || P.counters[kCntEntries] > P.entryCap) return;
-  for (int b = blockIdx.x; b < nb; b += gridDim.x) {
+  const XcdSpan sp = xcd_span(nb);
+  for (int b = sp.begin; b < sp.end; b += sp.step) {
     const int it = uni(P.bandItem[b]);
     ItemGeom g = P.geo[it];
     g.bandBase = uni(g.bandBase);
@@ -1624,11 +1642,17 @@ __global__ void __launch_bounds__(kScanBlock) scene_bin_scan_kernel(ScenePools P
 // ---------------------------------------------------------------------------
 
 // Next work item of this wave's queue, or a value >= n when it is empty.
+// The items are dealt to the kQueues queues in granules of G consecutive
+// items, round robin; a queue hands its items out in order.  Workgroups go
+// to the XCDs round robin, so queue q is served by XCD q % 8 alone: a
+// granule's items (neighbouring rows of one call, in phase A) share that
+// XCD's L2.
 // Every lane takes part in the atomic (lane 0 adds 1, the others 0) and lane
 // 0's old value is broadcast with a shuffle: broadcasting it through LDS
 // from a lane-0-only branch is not safe, because __syncthreads() compiles to
 // nothing in a one-wave workgroup and the compiler may then let lanes 1..63
 // loop back to the LDS read without lane 0, spinning on a stale value.
+template <int G = 1>
 __device__ __forceinline__ int queue_next(int32_t* heads, int n) {
   const int lane = threadIdx.x & 63;
   const int q = (int)(blockIdx.x % kQueues);
@@ -1636,7 +1660,9 @@ __device__ __forceinline__ int queue_next(int32_t* heads, int n) {
   k = __shfl(k, 0, 64);
   // the same in every lane: keep it (and what derives from it) scalar
   k = __builtin_amdgcn_readfirstlane(k);
-  return k < (n + kQueues - 1) / kQueues ? q + kQueues * k : n;
+  // the item is increasing in k, so a queue is empty once one is >= n
+  const int64_t item = ((int64_t)(k / G) * kQueues + q) * G + k % G;
+  return item < n ? (int)item : n;
 }
 
 // Takes n entries of a pool counter for the whole wave (same convergent
@@ -1788,7 +1814,7 @@ __global__ void __launch_bounds__(64, 5) scene_rows_kernel(ScenePools P) {
   // time (one atomic per chunk, not per task).
   int cursor = 0, chunkEnd = 0, used = 0;
   for (;;) {
-    const int q = tBegin + queue_next(heads, tEnd - tBegin);
+    const int q = tBegin + queue_next<kRowGranule>(heads, tEnd - tBegin);
     if (q >= tEnd) break;
     // Work item, call and the call's records are the same in every lane:
     // read them into scalar registers.
