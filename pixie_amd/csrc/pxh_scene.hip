@@ -1673,14 +1673,13 @@ __device__ __forceinline__ int queue_take(int32_t* counter, int n) {
   return __builtin_amdgcn_readfirstlane(__shfl(base, 0, 64));
 }
 
-// Call of every phase-A work item (one workgroup per call writes its
-// range).  Cost class of row r (0 .. rowCount-1) of a call: the bit length
-// of the entry count of its band (0 for rows outside the path).
-__device__ __forceinline__ int row_class(const ScenePools& P, const ItemGeom& g, const FillInfo& F, int r) {
-  const int y = g.rowStart + r;
-  if (y < F.startY || y >= F.pathHeight || F.nParts <= 0 || F.bandH <= 0) return 0;
-  int pi = (y - F.startY) / F.bandH;
-  if (pi > F.nParts - 1) pi = F.nParts - 1;
+// Band of row y of a banded call (partitionSegments' layout).
+__device__ __forceinline__ int band_of_row(const FillInfo& F, int y) {
+  const int pi = (y - F.startY) / F.bandH;
+  return pi > F.nParts - 1 ? F.nParts - 1 : pi;
+}
+// Cost class of band pi of a call: the bit length of its entry count.
+__device__ __forceinline__ int band_class(const ScenePools& P, const ItemGeom& g, int pi) {
   const int cnt = P.parts[g.bandBase + pi].count;
   return cnt <= 0 ? 0 : 32 - __clz(cnt);
 }
@@ -1715,19 +1714,20 @@ __global__ void __launch_bounds__(kTaskMapBlock) scene_task_map_kernel(ScenePool
       int c = -1;
       bool grouped = false;
       if (on) {
-        // classes of the rows of this task's group
+        // classes of this task's row and of its group's rows (the bands
+        // from the group's first row to its last; every row of a call lies
+        // in [startY, pathHeight))
         const int r = t / g.nWin, rg = r - r % kRowGroup;
+        const int re = (rg + kRowGroup < g.rowCount ? rg + kRowGroup : g.rowCount) - 1;
         int own = 0, chunk = 0;
-        for (int q = rg; q < rg + kRowGroup && q < g.rowCount; q++) {
-          const int y = g.rowStart + q;
-          int rc = 0;
-          if (banded && y >= F.startY && y < F.pathHeight) {
-            int pi = (y - F.startY) / F.bandH;
-            if (pi > nb - 1) pi = nb - 1;
-            rc = pi < kLdsBandClasses ? cls[pi] : row_class(P, g, F, q);
+        if (banded) {
+          const int pr = band_of_row(F, g.rowStart + r), p0 = band_of_row(F, g.rowStart + rg),
+                    p1 = band_of_row(F, g.rowStart + re);
+          for (int pi = p0; pi <= p1; pi++) {
+            const int rc = pi < kLdsBandClasses ? cls[pi] : band_class(P, g, pi);
+            if (pi == pr) own = rc;
+            chunk = rc > chunk ? rc : chunk;
           }
-          if (q == r) own = rc;
-          chunk = rc > chunk ? rc : chunk;
         }
         grouped = chunk <= kGroupMaxClass;
         c = !grouped ? own : (r == rg ? chunk : -1);
@@ -2109,9 +2109,9 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         if (kind == kRowNone) continue;
         const int lo = mask ? b0 : (xlo > b0 ? xlo : b0);
         const int hi = mask ? b1 : (xhi < b1 ? xhi : b1);
-        const int cLo = (lo - b0) / 256;
-        int cHi = (hi - b0) / 256;
-        if (cHi > kChunksB - 1) cHi = kChunksB - 1;
+        // The call's deltas lie in [xlo & ~3, xhi] (phase-A windows start on
+        // a multiple of 4, and a run may end at xhi): the 64-px slices from
+        // lo's to hi's hold them all, since xlo & ~3 shares xlo's slice.
         // Pixel codes in 64-px slices (lane l takes pixel l of the slice).
         // The pixels this call changes are queued, packed as code << 11 |
         // index, and the blend runs on full batches of 64 queued pixels, so
@@ -2119,8 +2119,8 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
         // call changes each pixel at most once, so a batch never holds the
         // same pixel twice.
         const bool maskCov = kind == kRowCoverage && mask;  // every pixel is written
-        int s = cLo * 4;
-        const int sEnd = cHi * 4 + 4;
+        int s = (lo - b0) / 64;
+        const int sEnd = (hi - b0) / 64 + 1 < kWinB / 64 ? (hi - b0) / 64 + 1 : kWinB / 64;
         int nq = 0;
         for (;;) {
           while (s < sEnd && nq < kWave) {
