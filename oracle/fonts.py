@@ -24,7 +24,7 @@ from fontTools.ttLib import TTFont
 
 f32 = np.float32
 LF, SP = 10, 32
-MOVE, LINE, QUAD, CLOSE = 1, 2, 7, 0
+MOVE, LINE, CUBIC, QUAD, CLOSE = 1, 2, 5, 7, 0
 
 
 def nim_round(x):
@@ -58,7 +58,7 @@ IDENTITY = [f32(v) for v in (1, 0, 0, 0, 1, 0, 0, 0, 1)]
 
 def path_transform(cmds, m):
     """Path.transform (paths.nim:264-333) for the commands glyph paths hold
-    (moves, lines, quadratic curves, closes)."""
+    (moves, lines, quadratic and cubic curves, closes)."""
     m = [f32(v) for v in m]
     if m == IDENTITY:
         return list(cmds)
@@ -69,7 +69,7 @@ def path_transform(cmds, m):
         i += 1
         if k == CLOSE:
             continue
-        n = {MOVE: 1, LINE: 1, QUAD: 2}[k]
+        n = {MOVE: 1, LINE: 1, QUAD: 2, CUBIC: 3}[k]
         for p in range(n):
             out[i + 2 * p], out[i + 2 * p + 1] = m3_apply(m, out[i + 2 * p], out[i + 2 * p + 1])
         i += 2 * n
@@ -77,7 +77,7 @@ def path_transform(cmds, m):
 
 
 class OTypeface:
-    """One OpenType font with glyf outlines, decoded by fontTools."""
+    """One OpenType font with glyf or CFF outlines, decoded by fontTools."""
 
     def __init__(self, data):
         font = TTFont(io.BytesIO(data), lazy=False)
@@ -107,7 +107,20 @@ class OTypeface:
         hm = font["hmtx"].metrics
         n = hhea.numberOfHMetrics
         self.advances = [hm[self.order[g]][0] for g in range(min(n, len(self.order)))]
-        self.glyf = font["glyf"]
+        self.glyf = font["glyf"] if "glyf" in font else None
+        self.cff = None
+        if self.glyf is None:  # CFF outlines (opentype.nim:1299-1678)
+            cff = font["CFF "].cff
+            top = cff.topDictIndex[0]
+            self.cff = top.CharStrings
+            self.cff_gsubrs = cff.GlobalSubrs
+            self.cff_cid = hasattr(top, "ROS")
+            if self.cff_cid:
+                self.cff_fdselect = top.FDSelect
+                self.cff_fdarray = top.FDArray
+            else:
+                priv = top.Private
+                self.cff_subrs = priv.Subrs if hasattr(priv, "Subrs") else []
         self.paths = {}
         self.fallbacks = []
         self._kerning_tables()
@@ -229,11 +242,167 @@ class OTypeface:
             out += [CLOSE]
         return out
 
+    def _cff_path(self, gid):
+        """parseCFFCharstring (opentype.nim:1028-1297) over the charstring
+        bytes fontTools decoded: float32 stack and pen, operands shifted off
+        the stack's front, moveTo without closing the contour before it,
+        endchar closing the path without ending the charstring."""
+        cs = self.cff[self.order[gid]]
+        if self.cff_cid:
+            priv = self.cff_fdarray[self.cff_fdselect[gid]].Private
+            subrs = priv.Subrs if hasattr(priv, "Subrs") else []
+        else:
+            subrs = self.cff_subrs
+
+        def bias(s):  # calcCFFSubroutineBias 1020-1026
+            return 107 if len(s) < 1240 else 1131 if len(s) < 33900 else 32768
+
+        out = []
+        st = {"stack": [], "nStems": 0, "haveWidth": False, "x": f32(0), "y": f32(0)}
+        stack = st["stack"]
+
+        def shift():
+            return stack.pop(0)
+
+        def stems():
+            if len(stack) % 2 != 0 and not st["haveWidth"]:
+                shift()
+            st["nStems"] += len(stack) >> 1
+            del stack[:]
+            st["haveWidth"] = True
+
+        def width_if(more):
+            if len(stack) > more and not st["haveWidth"]:
+                shift()
+                st["haveWidth"] = True
+
+        def curve(c1x, c1y, c2x, c2y):
+            out.extend([CUBIC, c1x, c1y, c2x, c2y, st["x"], st["y"]])
+
+        def run(code, depth):
+            assert depth <= 64
+            i = 0
+            while i < len(code):
+                v = code[i]
+                i += 1
+                x, y = st["x"], st["y"]
+                if v in (1, 3, 18, 23):
+                    stems()
+                elif v == 4:
+                    width_if(1)
+                    st["y"] = f32(y + stack.pop())
+                    out.extend([MOVE, st["x"], st["y"]])
+                elif v == 5:
+                    while stack:
+                        st["x"] = f32(st["x"] + shift())
+                        st["y"] = f32(st["y"] + shift())
+                        out.extend([LINE, st["x"], st["y"]])
+                elif v in (6, 7):
+                    horiz = v == 6
+                    while stack:
+                        k = "x" if horiz else "y"
+                        st[k] = f32(st[k] + shift())
+                        out.extend([LINE, st["x"], st["y"]])
+                        horiz = not horiz
+                elif v == 8 or v == 24:
+                    while len(stack) > (2 if v == 24 else 0):
+                        c1x = f32(st["x"] + shift()); c1y = f32(st["y"] + shift())
+                        c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                        st["x"] = f32(c2x + shift()); st["y"] = f32(c2y + shift())
+                        curve(c1x, c1y, c2x, c2y)
+                    if v == 24:
+                        st["x"] = f32(st["x"] + shift()); st["y"] = f32(st["y"] + shift())
+                        out.extend([LINE, st["x"], st["y"]])
+                elif v in (10, 29):
+                    s = subrs if v == 10 else self.cff_gsubrs
+                    k = int(stack.pop()) + bias(s)
+                    sub = s[k]
+                    if not hasattr(sub, "bytecode") or sub.bytecode is None:
+                        sub.compile()
+                    if len(sub.bytecode):
+                        run(sub.bytecode, depth + 1)
+                elif v == 11:
+                    return
+                elif v == 14:
+                    width_if(2)
+                    out.append(CLOSE)
+                elif v in (19, 20):
+                    stems()
+                    i += (st["nStems"] + 7) >> 3
+                elif v == 21:
+                    width_if(2)
+                    st["y"] = f32(y + stack.pop())
+                    st["x"] = f32(x + stack.pop())
+                    out.extend([MOVE, st["x"], st["y"]])
+                elif v == 22:
+                    width_if(1)
+                    st["x"] = f32(x + stack.pop())
+                    out.extend([MOVE, st["x"], st["y"]])
+                elif v == 25:
+                    while len(stack) > 6:
+                        st["x"] = f32(st["x"] + shift()); st["y"] = f32(st["y"] + shift())
+                        out.extend([LINE, st["x"], st["y"]])
+                    c1x = f32(st["x"] + shift()); c1y = f32(st["y"] + shift())
+                    c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                    st["x"] = f32(c2x + shift()); st["y"] = f32(c2y + shift())
+                    curve(c1x, c1y, c2x, c2y)
+                elif v == 26:
+                    if len(stack) % 2:
+                        st["x"] = f32(st["x"] + shift())
+                    while stack:
+                        c1x = st["x"]; c1y = f32(st["y"] + shift())
+                        c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                        st["x"] = c2x; st["y"] = f32(c2y + shift())
+                        curve(c1x, c1y, c2x, c2y)
+                elif v == 27:
+                    if len(stack) % 2:
+                        st["y"] = f32(st["y"] + shift())
+                    while stack:
+                        c1x = f32(st["x"] + shift()); c1y = st["y"]
+                        c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                        st["x"] = f32(c2x + shift()); st["y"] = c2y
+                        curve(c1x, c1y, c2x, c2y)
+                elif v == 28:
+                    stack.append(f32(int.from_bytes(bytes(code[i:i + 2]), "big", signed=True)))
+                    i += 2
+                elif v in (30, 31):
+                    vert = v == 30
+                    while stack:
+                        if vert:
+                            c1x = st["x"]; c1y = f32(st["y"] + shift())
+                            c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                            st["x"] = f32(c2x + shift())
+                            st["y"] = f32(c2y + (shift() if len(stack) == 1 else f32(0)))
+                        else:
+                            c1x = f32(st["x"] + shift()); c1y = st["y"]
+                            c2x = f32(c1x + shift()); c2y = f32(c1y + shift())
+                            st["y"] = f32(c2y + shift())
+                            st["x"] = f32(c2x + (shift() if len(stack) == 1 else f32(0)))
+                        curve(c1x, c1y, c2x, c2y)
+                        vert = not vert
+                elif v < 32:
+                    raise ValueError("CFF unknown operator: %d" % v)
+                elif v < 247:
+                    stack.append(f32(v - 139))
+                elif v < 251:
+                    stack.append(f32((v - 247) * 256 + code[i] + 108)); i += 1
+                elif v < 255:
+                    stack.append(f32(-(v - 251) * 256 - code[i] - 108)); i += 1
+                else:
+                    u = int.from_bytes(bytes(code[i:i + 4]), "big")
+                    stack.append(f32(f32(u) / f32(65536))); i += 4
+
+        if cs.bytecode is None:
+            cs.compile()
+        run(cs.bytecode, 0)
+        return out
+
     def ot_glyph_path(self, rune):
         """getGlyphPath (opentype.nim:2456-2463): flipped to y down, cached."""
         if rune not in self.paths:
-            self.paths[rune] = path_transform(self._glyf_path(self.glyph_id(rune)),
-                                              [1, 0, 0, 0, -1, 0, 0, 0, 1])
+            gid = self.glyph_id(rune)
+            raw = self._glyf_path(gid) if self.glyf is not None else self._cff_path(gid)
+            self.paths[rune] = path_transform(raw, [1, 0, 0, 0, -1, 0, 0, 0, 1])
         return self.paths[rune]
 
     def ot_advance(self, rune):  # opentype.nim:2474-2479
@@ -515,9 +684,12 @@ def layout_bounds(a):  # fonts.nim:478-492
     return (x, y)
 
 
-def _rect(x, y, w, h):  # paths.nim:506-515, clockwise
+def _rect(x, y, w, h, clockwise=True):  # paths.nim:506-521
     x, y, w, h = f32(x), f32(y), f32(w), f32(h)
-    return [MOVE, x, y, LINE, f32(x + w), y, LINE, f32(x + w), f32(y + h), LINE, x, f32(y + h),
+    if clockwise:
+        return [MOVE, x, y, LINE, f32(x + w), y, LINE, f32(x + w), f32(y + h), LINE, x, f32(y + h),
+                CLOSE]
+    return [MOVE, x, y, LINE, x, f32(y + h), LINE, f32(x + w), f32(y + h), LINE, f32(x + w), y,
             CLOSE]
 
 
@@ -544,10 +716,11 @@ def compute_paths(a):
                     decorate = False
             if decorate:
                 r = a.rects[ri]
+                cw = t.glyf is not None  # isCCW (opentype.nim:2517-2521)
                 if font.underline:
-                    g += _rect(r[0], f32(f32(py - up) + f32(ut / f32(2))), r[2], ut)
+                    g += _rect(r[0], f32(f32(py - up) + f32(ut / f32(2))), r[2], ut, cw)
                 if font.strikethrough:
-                    g += _rect(r[0], f32(py - sp), r[2], st)
+                    g += _rect(r[0], f32(py - sp), r[2], st, cw)
             path += g
         out.append(np.array(path, np.float32))
     return out

@@ -16,6 +16,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -105,6 +106,18 @@ struct pxh_typeface {
   std::vector<uint16_t> advances;  // hMetrics advanceWidth
   std::vector<uint32_t> glyf;      // glyph data offsets (glyf table + loca)
   bool hasGlyf = false;
+  // CFF outlines: charstring, global and local subroutine ranges [a, b) in
+  // data; CID-keyed fonts pick their local subroutines per glyph (FDSelect)
+  using Ranges = std::vector<std::pair<uint32_t, uint32_t>>;
+  struct CffPrivate {
+    Ranges subrs;
+  };
+  bool isCff = false;
+  Ranges cffChars, cffGsubrs;
+  CffPrivate cffTop;
+  bool cffCid = false;
+  std::vector<CffPrivate> cffFds;
+  std::vector<uint8_t> cffFdSelect;
   bool hasKern = false;
   std::unordered_map<uint32_t, float> kernPairs;
   bool hasGpos = false;
@@ -289,6 +302,161 @@ void parse_os2(pxh_typeface* t, int64_t o) {  // opentype.nim:711-771
     i += 10;
   }
   if (version >= 5) b.check(i + 4);
+}
+
+// ---- CFF (opentype.nim:862-1030, 1299-1678) ----
+
+// An INDEX at `start`: the [a, b) ranges of its objects; start moves past it
+// (parseCFFIndexOffsets 862-906).
+pxh_typeface::Ranges cff_index(const Buf& b, int64_t* start) {
+  pxh_typeface::Ranges out;
+  const int count = b.u16(*start);
+  if (count == 0) {
+    *start += 2;
+    return out;
+  }
+  const int offSize = b.u8(*start + 2);
+  const int64_t objects = *start + (int64_t)(count + 1) * offSize + 2;
+  std::vector<int64_t> offs(count + 1);
+  int64_t pos = *start + 3;
+  for (int i = 0; i <= count; i++, pos += offSize) {
+    int64_t v = 0;
+    for (int k = 0; k < offSize; k++) v = (v << 8) | b.u8(pos + k);
+    offs[i] = v;
+  }
+  for (int i = 0; i < count; i++) {
+    const int64_t a = objects + offs[i], e = objects + offs[i + 1];
+    if (a < 0 || e < a || e > b.n) throw FontError("Unexpected error reading font data, EOF");
+    out.push_back({(uint32_t)a, (uint32_t)e});
+  }
+  *start = objects + offs[count];
+  return out;
+}
+
+// A DICT (parseCFFDict 1322-1407): operator -> operands, for the operators
+// the reference keeps (interpretDict: unknown ones are skipped, a repeated
+// known one fails).
+std::unordered_map<int, std::vector<double>> cff_dict(const Buf& b, int64_t a, int64_t e,
+                                                      const std::vector<int>& known) {
+  std::unordered_map<int, std::vector<double>> out;
+  std::vector<double> operands;
+  int64_t i = a;
+  while (i < e) {
+    int op = b.u8(i++);
+    if (op <= 21) {
+      if (op == 12) op = 1200 + b.u8(i++);
+      if (std::find(known.begin(), known.end(), op) != known.end()) {
+        if (out.count(op)) unsupported("CFF duplicate key");
+        out[op] = operands;
+      }
+      operands.clear();
+      continue;
+    }
+    double v;
+    if (op == 28) {  // read unsigned, as the reference does
+      v = (double)((b.u8(i) << 8) | b.u8(i + 1));
+      i += 2;
+    } else if (op == 29) {
+      v = (double)(int32_t)b.u32(i);
+      i += 4;
+    } else if (op == 30) {  // real: nibbles
+      static const char* const nib[15] = {"0", "1", "2", "3", "4", "5", "6", "7",
+                                          "8", "9", ".", "E", "E-", "null", "-"};
+      std::string str;
+      for (;;) {
+        const int byte = b.u8(i++);
+        const int n1 = byte >> 4, n2 = byte & 15;
+        if (n1 == 15) break;
+        str += nib[n1];
+        if (n2 == 15) break;
+        str += nib[n2];
+      }
+      char* end = nullptr;
+      v = strtod(str.c_str(), &end);
+      if (str.empty() || *end) unsupported("float " + str);
+    } else if (op >= 32 && op <= 246) {
+      v = op - 139;
+    } else if (op >= 247 && op <= 250) {
+      v = (op - 247) * 256 + b.u8(i++) + 108;
+    } else if (op >= 251 && op <= 254) {
+      v = -(op - 251) * 256 - b.u8(i++) - 108;
+    } else {
+      unsupported("Invalid b0 " + std::to_string(op));
+    }
+    operands.push_back(v);
+  }
+  return out;
+}
+
+int dict_int(const std::unordered_map<int, std::vector<double>>& d, int key, int dflt, size_t k = 0) {
+  auto it = d.find(key);
+  if (it == d.end() || it->second.size() <= k) return dflt;
+  return (int)it->second[k];
+}
+
+// A Top DICT's (or FD's) private DICT and its local subroutines
+// (gatherCFFTopDicts 1528-1541): Subrs is relative to the private DICT.
+pxh_typeface::CffPrivate cff_private(const Buf& b, int64_t cff, const std::unordered_map<int, std::vector<double>>& top) {
+  pxh_typeface::CffPrivate pr;
+  const int size = dict_int(top, 18, 0, 0), off = dict_int(top, 18, 0, 1);
+  if (off != 0) {
+    const auto priv = cff_dict(b, cff + off, cff + off + size, {19, 20, 21});
+    const int subrs = dict_int(priv, 19, 0);
+    if (subrs > 0) {
+      int64_t at = cff + off + subrs;
+      pr.subrs = cff_index(b, &at);
+    }
+  }
+  return pr;
+}
+
+void parse_cff(pxh_typeface* t, int64_t offset) {  // parseCFFTable 1299-1678
+  const Buf& b = t->buf;
+  b.check(offset + 32);
+  int64_t at = offset + 4;
+  cff_index(b, &at);  // names
+  const auto topIndex = cff_index(b, &at);
+  const auto strings = cff_index(b, &at);
+  t->cffGsubrs = cff_index(b, &at);
+  if (topIndex.empty()) unsupported("CFF no topDict");
+  if (topIndex.size() > 1) unsupported("CFF multiple topDict");
+  const std::vector<int> topKeys = {0, 1, 2, 3, 4, 5, 13, 14, 15, 16, 17, 18, 1200, 1201, 1202, 1203, 1204,
+                                    1205, 1206, 1207, 1208, 1230, 1231, 1232, 1233, 1234, 1235, 1236, 1237, 1238};
+  const auto top = cff_dict(b, topIndex[0].first, topIndex[0].second, topKeys);
+  t->cffTop = cff_private(b, offset, top);
+  // CID-keyed when the registry and ordering strings are not empty
+  auto sid_nonempty = [&](int sid) {
+    if (sid <= 390) return true;  // standard strings
+    return sid - 391 < (int)strings.size() && strings[sid - 391].second > strings[sid - 391].first;
+  };
+  auto ros = top.find(1230);
+  if (ros != top.end() && ros->second.size() >= 2 && sid_nonempty((int)ros->second[0]) &&
+      sid_nonempty((int)ros->second[1])) {
+    t->cffCid = true;
+    int64_t fdArray = offset + dict_int(top, 1236, 0), fdSelect = offset + dict_int(top, 1237, 0);
+    if (fdArray == 0 || fdSelect == 0) unsupported("CFF CID tables are missing");
+    for (const auto& fd : cff_index(b, &fdArray))
+      t->cffFds.push_back(cff_private(b, offset, cff_dict(b, fd.first, fd.second, topKeys)));
+    // FDSelect, format 3 only (parseCFFFDSelect 1595-1624)
+    int64_t i = fdSelect;
+    if (b.u8(i++) != 3) unsupported("CFF Table CID Font FDSelect format");
+    const int nRanges = b.u16(i);
+    int first = b.u16(i + 2), next = 0;
+    i += 4;
+    if (first != 0) unsupported("CFF Table CID Font FDSelect format 3 range has bad initial GID");
+    for (int r = 0; r < nRanges; r++, i += 3) {
+      const int fd = b.u8(i);
+      next = b.u16(i + 1);
+      if (fd >= (int)t->cffFds.size()) unsupported("CFF table CID Font FDSelect has bad FD index value");
+      if (next > t->numGlyphs) unsupported("CFF Table CID Font FDSelect format 3 range has bad GID");
+      for (; first < next; first++) t->cffFdSelect.push_back((uint8_t)fd);
+      first = next;
+    }
+    if (next != t->numGlyphs) unsupported("CFF Table CID Font FDSelect format 3 range has bad final GID");
+  }
+  int64_t cs = offset + dict_int(top, 17, 0);
+  t->cffChars = cff_index(b, &cs);
+  t->isCff = true;
 }
 
 void parse_loca_glyf(pxh_typeface* t, int64_t loca, int64_t glyf) {  // opentype.nim:773-798
@@ -575,7 +743,7 @@ void parse_open_type(pxh_typeface* t, int64_t start) {
   if (tables.count("loca") && tables.count("glyf")) {
     parse_loca_glyf(t, tables["loca"], tables["glyf"]);
   } else if (tables.count("CFF ")) {
-    throw FontError("Unsupported font CFF outlines (only TrueType glyf outlines are ingested)");
+    parse_cff(t, tables["CFF "]);
   } else {
     unsupported("glyph outlines");
   }
@@ -810,6 +978,217 @@ void parse_glyf_glyph(const pxh_typeface* t, uint32_t glyphId, std::vector<float
     parse_simple_glyph(t->buf, (int64_t)o + 10, numberOfContours, out);
 }
 
+// calcCFFSubroutineBias (opentype.nim:1020-1026).
+int cff_bias(const pxh_typeface::Ranges& subrs) {
+  return subrs.size() < 1240 ? 107 : subrs.size() < 33900 ? 1131 : 32768;
+}
+
+// parseCFFCharstring (opentype.nim:1028-1297): the Type 2 charstring of glyph
+// glyphId as moveTo / lineTo / bezierCurveTo commands in float32, with the
+// reference's rules: the stack's front is shifted off as operands are used,
+// a new contour starts with moveTo without closing the one before, endchar
+// closes the path and does not stop the charstring, flex and other escaped
+// operators are unsupported, and a 255 operand is read as an unsigned 32-bit
+// number over 65536.
+struct CffRun {
+  const pxh_typeface* t;
+  const pxh_typeface::Ranges* subrs;
+  int subrsBias;
+  std::vector<float>* out;
+  std::vector<float> stack;
+  size_t head = 0;  // stack[head..] is the stack
+  int nStems = 0;
+  bool haveWidth = false;
+  float x = 0.0f, y = 0.0f;
+  size_t len() const { return stack.size() - head; }
+  float shift() {
+    if (head >= stack.size()) throw FontError("Invalid CFF charstring: stack underflow");
+    return stack[head++];
+  }
+  float pop() {
+    if (head >= stack.size()) throw FontError("Invalid CFF charstring: stack underflow");
+    const float v = stack.back();
+    stack.pop_back();
+    return v;
+  }
+  void clear() {
+    stack.clear();
+    head = 0;
+  }
+  void curve(float c1x, float c1y, float c2x, float c2y) {
+    out->insert(out->end(), {5.0f, c1x, c1y, c2x, c2y, x, y});
+  }
+  void stems() {
+    if (len() % 2 != 0 && !haveWidth) shift();  // the width
+    nStems += (int)(len() >> 1);
+    clear();
+    haveWidth = true;
+  }
+  void width_if(size_t more) {
+    if (len() > more && !haveWidth) {
+      shift();
+      haveWidth = true;
+    }
+  }
+  void parse(uint32_t a, uint32_t e, int depth) {
+    if (depth > 64) throw FontError("Invalid CFF charstring: subroutines nest too deep");
+    const Buf& b = t->buf;
+    for (uint32_t i = a; i < e;) {
+      const int v = b.u8(i++);
+      switch (v) {
+        case 1: case 3: case 18: case 23:  // hstem vstem hstemhm vstemhm
+          stems();
+          break;
+        case 4:  // vmoveto
+          width_if(1);
+          y += pop();
+          path_move(out, x, y);
+          break;
+        case 5:  // rlineto
+          while (len() > 0) {
+            x += shift();
+            y += shift();
+            path_line(out, x, y);
+          }
+          break;
+        case 6: case 7:  // hlineto vlineto
+          for (bool horiz = v == 6; len() > 0; horiz = !horiz) {
+            if (horiz)
+              x += shift();
+            else
+              y += shift();
+            path_line(out, x, y);
+          }
+          break;
+        case 8:  // rrcurveto
+          while (len() > 0) {
+            const float c1x = x + shift(), c1y = y + shift(), c2x = c1x + shift(), c2y = c1y + shift();
+            x = c2x + shift();
+            y = c2y + shift();
+            curve(c1x, c1y, c2x, c2y);
+          }
+          break;
+        case 10: case 29: {  // callsubr callgsubr
+          const pxh_typeface::Ranges& s = v == 10 ? *subrs : t->cffGsubrs;
+          const int k = (int)pop() + (v == 10 ? subrsBias : cff_bias(t->cffGsubrs));
+          if (k < 0 || k >= (int)s.size()) throw FontError("Invalid CFF charstring: subroutine index");
+          if (s[k].second > s[k].first) parse(s[k].first, s[k].second, depth + 1);
+          break;
+        }
+        case 11:  // return
+          return;
+        case 14:  // endchar
+          width_if(2);
+          path_close(out);
+          break;
+        case 19: case 20:  // hintmask cntrmask
+          stems();
+          i += (uint32_t)((nStems + 7) >> 3);
+          break;
+        case 21: {  // rmoveto
+          width_if(2);
+          y += pop();
+          x += pop();
+          path_move(out, x, y);
+          break;
+        }
+        case 22:  // hmoveto
+          width_if(1);
+          x += pop();
+          path_move(out, x, y);
+          break;
+        case 24: {  // rcurveline
+          while (len() > 2) {
+            const float c1x = x + shift(), c1y = y + shift(), c2x = c1x + shift(), c2y = c1y + shift();
+            x = c2x + shift();
+            y = c2y + shift();
+            curve(c1x, c1y, c2x, c2y);
+          }
+          x += shift();
+          y += shift();
+          path_line(out, x, y);
+          break;
+        }
+        case 25: {  // rlinecurve
+          while (len() > 6) {
+            x += shift();
+            y += shift();
+            path_line(out, x, y);
+          }
+          const float c1x = x + shift(), c1y = y + shift(), c2x = c1x + shift(), c2y = c1y + shift();
+          x = c2x + shift();
+          y = c2y + shift();
+          curve(c1x, c1y, c2x, c2y);
+          break;
+        }
+        case 26:  // vvcurveto
+          if (len() % 2 != 0) x += shift();
+          while (len() > 0) {
+            const float c1x = x, c1y = y + shift(), c2x = c1x + shift(), c2y = c1y + shift();
+            x = c2x;
+            y = c2y + shift();
+            curve(c1x, c1y, c2x, c2y);
+          }
+          break;
+        case 27:  // hhcurveto
+          if (len() % 2 != 0) y += shift();
+          while (len() > 0) {
+            const float c1x = x + shift(), c1y = y, c2x = c1x + shift(), c2y = c1y + shift();
+            x = c2x + shift();
+            y = c2y;
+            curve(c1x, c1y, c2x, c2y);
+          }
+          break;
+        case 28:  // shortint
+          stack.push_back((float)b.i16(i));
+          i += 2;
+          break;
+        case 30: case 31:  // vhcurveto hvcurveto: alternate vertical / horizontal starts
+          for (bool vert = v == 30; len() > 0; vert = !vert) {
+            if (vert) {
+              const float c1x = x, c1y = y + shift(), c2x = c1x + shift(), c2y = c1y + shift();
+              x = c2x + shift();
+              y = c2y + (len() == 1 ? shift() : 0.0f);
+              curve(c1x, c1y, c2x, c2y);
+            } else {
+              const float c1x = x + shift(), c1y = y, c2x = c1x + shift(), c2y = c1y + shift();
+              y = c2y + shift();
+              x = c2x + (len() == 1 ? shift() : 0.0f);
+              curve(c1x, c1y, c2x, c2y);
+            }
+          }
+          break;
+        default:
+          if (v < 32) {
+            unsupported("CFF unknown operator: " + std::to_string(v));
+          } else if (v < 247) {
+            stack.push_back((float)(v - 139));
+          } else if (v < 251) {
+            stack.push_back((float)((v - 247) * 256 + b.u8(i) + 108));
+            i += 1;
+          } else if (v < 255) {
+            stack.push_back((float)(-(v - 251) * 256 - b.u8(i) - 108));
+            i += 1;
+          } else {
+            stack.push_back((float)(int64_t)b.u32(i) / 65536.0f);
+            i += 4;
+          }
+      }
+    }
+  }
+};
+
+void parse_cff_glyph(const pxh_typeface* t, uint32_t glyphId, std::vector<float>* out) {
+  if (glyphId >= t->cffChars.size()) throw FontError("Invalid glyph ID " + std::to_string(glyphId));
+  const pxh_typeface::Ranges* subrs = &t->cffTop.subrs;
+  if (t->cffCid) {
+    if (glyphId >= t->cffFdSelect.size()) throw FontError("Invalid glyph ID " + std::to_string(glyphId));
+    subrs = &t->cffFds[t->cffFdSelect[glyphId]].subrs;
+  }
+  CffRun run{t, subrs, cff_bias(*subrs), out};
+  run.parse(t->cffChars[glyphId].first, t->cffChars[glyphId].second, 0);
+}
+
 uint16_t glyph_id(const pxh_typeface* t, uint32_t rune) {  // opentype.nim:2176-2177
   auto it = t->cmap.find(rune);
   return it == t->cmap.end() ? 0 : it->second;
@@ -820,7 +1199,10 @@ const std::vector<float>& ot_glyph_path(pxh_typeface* t, uint32_t rune) {
   auto it = t->glyphPaths.find(rune);
   if (it != t->glyphPaths.end()) return it->second;
   std::vector<float> p;
-  parse_glyf_glyph(t, glyph_id(t, rune), &p, 0);
+  if (t->isCff)
+    parse_cff_glyph(t, glyph_id(t, rune), &p);
+  else
+    parse_glyf_glyph(t, glyph_id(t, rune), &p, 0);
   const float flip[9] = {1, 0, 0, 0, -1, 0, 0, 0, 1};
   path_transform(&p, flip);
   return t->glyphPaths.emplace(rune, std::move(p)).first->second;
@@ -1255,8 +1637,10 @@ void span_path(const pxh_arrangement* a, size_t si, int32_t* line, std::vector<f
     }
     if (decorate) {
       const float* r = a->rects.data() + 4 * ri;
-      if (font->underline) path_rect(out, r[0], py - up + ut / 2.0f, r[2], ut, true);
-      if (font->strikethrough) path_rect(out, r[0], py - sp, r[2], st, true);
+      // isCCW (opentype.nim:2517-2521): clockwise rects unless the font is CFF
+      const bool cw = !font->typeface->isCff;
+      if (font->underline) path_rect(out, r[0], py - up + ut / 2.0f, r[2], ut, cw);
+      if (font->strikethrough) path_rect(out, r[0], py - sp, r[2], st, cw);
     }
   }
 }
@@ -1295,8 +1679,10 @@ int span_instances(const pxh_arrangement* a, size_t si, int32_t* line, std::vect
     }
     if (decorate) {
       const float* r = a->rects.data() + 4 * ri;
-      if (font->underline) out->push_back({-1, 0, {r[0], py - up + ut / 2.0f, r[2], ut, 0, 0}});
-      if (font->strikethrough) out->push_back({-1, 0, {r[0], py - sp, r[2], st, 0, 0}});
+      // m[4] = 1: a counter-clockwise rect (isCCW, opentype.nim:2517-2521)
+      const float ccw = font->typeface->isCff ? 1.0f : 0.0f;
+      if (font->underline) out->push_back({-1, 0, {r[0], py - up + ut / 2.0f, r[2], ut, ccw, 0}});
+      if (font->strikethrough) out->push_back({-1, 0, {r[0], py - sp, r[2], st, ccw, 0}});
     }
   }
   return 0;

@@ -2229,15 +2229,16 @@ __global__ void __launch_bounds__(256) scene_expand_kernel(const GlyphInst* inst
   for (int w = blockIdx.x * 4 + (threadIdx.x >> 6); w < n; w += gridDim.x * 4) {
     const GlyphInst g = inst[w];
     const float* m = g.m;
-    if (g.srcCmd < 0) {  // path.rect(x, y, w, h), clockwise (paths.nim:506-515)
+    if (g.srcCmd < 0) {  // path.rect(x, y, w, h), clockwise unless m[4] (paths.nim:506-521)
       const float x = m[0], y = m[1], rw = m[2], rh = m[3];
+      const bool ccw = m[4] != 0.0f;
       if (lane < 5) {
         SceneCmd c = {};
         c.kind = lane == 0 ? 1.0f : lane == 4 ? 0.0f : 2.0f;
         if (lane == 0) { c.p[0] = x; c.p[1] = y; }
-        if (lane == 1) { c.p[0] = x + rw; c.p[1] = y; }
+        if (lane == 1) { c.p[0] = ccw ? x : x + rw; c.p[1] = ccw ? y + rh : y; }
         if (lane == 2) { c.p[0] = x + rw; c.p[1] = y + rh; }
-        if (lane == 3) { c.p[0] = x; c.p[1] = y + rh; }
+        if (lane == 3) { c.p[0] = ccw ? x + rw : x; c.p[1] = ccw ? y : y + rh; }
         c.sx = x;
         c.sy = y;
         c.pad[0] = g.polyIdx;

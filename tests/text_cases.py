@@ -25,6 +25,14 @@ f32 = np.float32
 R, U, A, I, N = ("Roboto-Regular_1.ttf", "Ubuntu-Regular_1.ttf", "Aclonica-Regular_1.ttf",
                  "IBMPlexSans-Regular_2.ttf", "NotoSans-Regular_4.ttf")
 P = "PinyonScript.ttf"
+IN, IB, EMO = "Inter-Regular.ttf", "Inter-Bold.ttf", "NotoEmoji-subset.otf"  # CFF outlines
+# the emoji text of test_fonts.nim:7-20
+EMOJI_TEXT = ('🚑🐑👭🔉🚷🦣💆🔁💺🚵🕦🔦🗓🦟😶🦄⌛🍙😄🇽\n'
+              '🐠💓🦗🎭🏛🔴🫕🧶🍖🦁🏋🌗🛬🕐💡👉🎯🕔🚏🚲\n'
+              '🐵🎍💳🥬🟦🪘📠📊🎧🎦🎁🌌🪲🦩🤢☎🚺🚾👺🚃\n'
+              '🐨🌆🥉💭🗳🦵🟪📆🥮⏯🩴💷🦲➗🌶🧜🖖⏰🛗🔻\n'
+              '📁🧞😃🌴🚶\U000fe82b🦙🔎⏲🔵🖐☦😪🌯🙆🇺😂🍅🇿🚟🤜\n'
+              '📼👰🍁📽☪🔄🤝🔧🦸🏰🏳🔜🎥🚋🇫🦨🏜🆖🏤🪖⏏')
 BLACK = (0.0, 0.0, 0.0, 1.0)
 
 
@@ -98,6 +106,13 @@ def _cases():
             "strikethrough2": ("strikethrough", False, rgba(0, 0, 0, 127)),
             "strikethrough3": ("strikethrough", True, BLACK)}.items():
         c[k] = (200, 100, [T(W8, R, 24, (200, 0), stroke=stroke, paints=[paint], **{deco: True})])
+    # CFF outlines (test_fonts.nim:7-20, 1051-1089, 1130-1138)
+    grumpy = "Grumpy wizards make toxic brew for the evil Queen and Jack."
+    c["emoji"] = (800, 300, [T(EMOJI_TEXT, EMO, 26)])
+    c["cff"] = (800, 100, [T(grumpy, IN, 26)])
+    c["cff_underline"] = (800, 100, [T(grumpy, IN, 26, underline=True)])
+    c["cff_strikethrough"] = (800, 100, [T(grumpy, IN, 26, strikethrough=True)])
+    c["tofu_advance"] = (800, 100, [T("This[]Advance!", IN, 26)])
     c["paints1"] = (200, 100, [T("Multiple fills", R, 36, (200, 0),
                                  paints=[rgba(0, 0, 255, 127), rgba(255, 0, 0, 127)])])
     names = [["TopLeft", "TopCenter", "TopRight"], ["MiddleLeft", "MiddleCenter", "MiddleRight"],
