@@ -82,6 +82,30 @@ def test_glyph_paths_cmap_advances(px, of, name):
                  o.underlineThickness, o.strikeoutPosition, o.strikeoutSize]
 
 
+@pytest.mark.parametrize("name", ["Inter-Regular.ttf", "Inter-Bold.ttf", "NotoEmoji-subset.otf"])
+def test_cff_outlines_match_fonttools(of, name):
+    """The oracle's CFF charstring interpreter (a restatement of
+    opentype.nim:1028-1297) against fontTools' own: on every mapped glyph the
+    same points in the same order.  Where a contour is closed differs by
+    design (fontTools closes each one; pixie only at endchar), so only the
+    points are compared."""
+    from fontTools.pens.recordingPen import RecordingPen
+    o = tc.o_typeface(name)
+    gs = o.font.getGlyphSet()
+    count = {0: 0, 1: 2, 2: 2, 5: 6, 7: 4}
+    for r in sorted(o.cmap):
+        pen = RecordingPen()
+        gs[o.order[o.glyph_id(r)]].draw(pen)
+        want = [c for _, args in pen.value for pt in args for c in (float(pt[0]), float(pt[1]))]
+        got, cmds, k = [], list(o.glyph_path(r)), 0
+        while k < len(cmds):
+            n = count[int(cmds[k])]
+            for j in range(k + 1, k + 1 + n, 2):
+                got += [float(cmds[j]), -float(cmds[j + 1])]  # back to y up
+            k += 1 + n
+        assert got == want, (name, r)
+
+
 @pytest.mark.parametrize("name", FONT_FILES)
 def test_kerning_pairs(px, of, name):
     p, o = tc.p_typeface(name), tc.o_typeface(name)
