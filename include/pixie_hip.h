@@ -269,10 +269,11 @@ double pxh_server_now(pxh_server* server);
 int pxh_server_release(pxh_server* server, int64_t ticket);
 
 /* Text (fonts.nim, fontformats/opentype.nim) ---------------------------- */
-/* A typeface: an OpenType font with TrueType (glyf) outlines parsed from
- * bytes (copied), as parseTtf / parseOtf (fonts.nim:500-505,
- * opentype.nim:2531-2588): cmap formats 4 and 12, head, hhea, maxp, hmtx,
- * name, OS/2, loca + glyf, kern, GPOS pair adjustments, post.  Errors are
+/* A typeface: an OpenType font with TrueType (glyf) or CFF (Type 2
+ * charstring) outlines parsed from bytes (copied), as parseTtf / parseOtf
+ * (fonts.nim:500-505, opentype.nim:2531-2588): cmap formats 4 and 12, head,
+ * hhea, maxp, hmtx, name, OS/2, loca + glyf or CFF (opentype.nim:862-1678),
+ * kern, GPOS pair adjustments, post.  Errors are
  * pixie's PixieError texts ("Unexpected error reading font data, EOF",
  * "Missing required font table: ...", "Unsupported font ...").  Glyph paths
  * are cached per rune on first use (opentype.nim:2456-2463). */
