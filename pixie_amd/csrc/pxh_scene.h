@@ -101,6 +101,9 @@ constexpr int32_t kRunXMask = (1 << 28) - 1;
 // left chunks plus one open chunk per wave (scene_run_bound).
 constexpr int kRowGroup = 8;       // rows of a grouped phase-A work item
 constexpr int kRowGranule = 32;    // phase-A work items a queue takes in a row
+// phase-B units a queue takes in a row: neighbouring units share a row bin's
+// call list and call records (64 units are one 16-row bin of a 4096-px canvas)
+constexpr int kComposeGranule = 64;
 constexpr int kGroupMaxClass = 7;  // grouped rows' bands hold fewer than 128 entries
 constexpr int kRunChunk = 2048;
 constexpr int kMaxTaskRuns = 1025;

@@ -1970,7 +1970,7 @@ __global__ void __launch_bounds__(64) scene_compose_kernel(ScenePools P, void* c
     return;
   }
   for (;;) {
-    const int unit = queue_next(P.queues + 2 * kQueues * kQueueStride, nUnits);
+    const int unit = queue_next<kComposeGranule>(P.queues + 2 * kQueues * kQueueStride, nUnits);
     if (unit >= nUnits) break;
     const int y = unit / winsPerRow;
     const int b0 = (unit % winsPerRow) * kWinB;
